@@ -1,0 +1,663 @@
+// spark_capi.cpp -- the extern "C" boundary (include/spark_hip.h): batch
+// bookkeeping, device memory layout and the launch sequence of one RK step.
+//
+// Device layout of a batch (all pools allocated once per batch):
+//   state pools U (solution) and S (stage state): per block, per variable,
+//     a contiguous guard-celled box (x fastest), variable stride padded to
+//     64 doubles; variable order dens eint ener gamc game pres shok temp
+//     velx vely velz scalars.
+//   one-variable cell pool: flattening coefficients / sound-speed scratch.
+//   face pools (one per direction): per block, per flux variable, faces on
+//     an (NX+1)(NY+1)(NZ+1) box; face i is the low face of cell i.
+//   flux-register pools (flux_correct only): interior faces per block.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/spark_hip.h"
+#include "spark_launch.hpp"
+
+using spark::BlockMeta;
+using spark::DevParams;
+using spark::PlainLaunch;
+
+namespace {
+
+thread_local std::string g_err;
+int g_device = -1;
+hipStream_t g_stream = nullptr;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                           \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess) return fail(SPARK_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                      __FILE__, __LINE__);                                     \
+  } while (0)
+
+inline int64_t round_up(int64_t a, int64_t m) { return (a + m - 1) / m * m; }
+
+struct ShapeClass {
+  int nx, ny, nz;
+  std::vector<int> ids;
+  int* ids_d = nullptr;
+};
+
+}  // namespace
+
+struct spark_batch {
+  spark_params p;
+  DevParams dp;
+  int nblk = 0;
+  int nvar = 0;
+  std::vector<spark_block_desc> desc;
+  std::vector<BlockMeta> meta_h;
+  BlockMeta* meta_d = nullptr;
+  std::vector<ShapeClass> classes;
+  int64_t pool = 0, opool = 0, fpool = 0, bpool = 0;  // sizes in doubles
+  double* U = nullptr;
+  double* S = nullptr;
+  double* flat = nullptr;
+  double* fl[3] = {nullptr, nullptr, nullptr};
+  double* fb[3] = {nullptr, nullptr, nullptr};
+  double* dtp = nullptr;
+  long long* loc = nullptr;
+  double* dt_hist = nullptr;
+  int hist_cap = 0;
+  void* dtparts = nullptr;
+  int dtchunks = 1;
+  int* err = nullptr;
+  int ug[3] = {0, 0, 0};
+  bool has_ug = false;
+  bool step_open = false;
+  int64_t max_cells = 0;
+  int last_stage = 3;
+  double coeff[3][3];
+  int limits_array[3];
+  double weights[3];
+};
+
+namespace {
+
+void rk_tables(const spark_params& p, int* last, double c[3][3], int la[3], double w[3]) {
+  // Spark/Hydro.ini:2-48 (RK3 constants 1/3, 2/3, 1/6 supplied)
+  const int ns = p.nstencil;
+  if (p.rk_order == 3) {
+    const double onethird = 1. / 3., twothirds = 2. / 3., onesixth = 1. / 6.;
+    const double t[3][3] = {{1., 0., 1.}, {0.75, 0.25, 0.25}, {onethird, twothirds, twothirds}};
+    memcpy(c, t, sizeof(t));
+    *last = 3;
+    la[0] = 2 * ns; la[1] = ns; la[2] = 0;
+    w[0] = onesixth; w[1] = onesixth; w[2] = twothirds;
+  } else {
+    const double t[3][3] = {{1., 0., 1.}, {0.5, 0.5, 0.5}, {0., 0., 0.}};
+    memcpy(c, t, sizeof(t));
+    *last = 2;
+    la[0] = ns; la[1] = 0; la[2] = 0;
+    w[0] = 0.5; w[1] = 0.5; w[2] = 0.;
+  }
+}
+
+PlainLaunch make_launch(spark_batch* b, const ShapeClass& c) {
+  PlainLaunch L;
+  L.P = b->dp;
+  L.stream = g_stream;
+  L.meta = b->meta_d;
+  L.blist = c.ids_d;
+  L.nblk = (int)c.ids.size();
+  L.nx = c.nx; L.ny = c.ny; L.nz = c.nz;
+  L.U0 = b->U;
+  L.S = b->S;
+  L.flat = b->flat;
+  for (int d = 0; d < 3; d++) { L.fl[d] = b->fl[d]; L.fb[d] = b->p.flux_correct ? b->fb[d] : nullptr; }
+  L.weight = 0.;
+  L.add_flux = 0;
+  L.dtp = b->dtp;
+  L.err = b->err;
+  return L;
+}
+
+int required_guard(const spark_params& p) {
+  int la[3], last;
+  double c[3][3], w[3];
+  rk_tables(p, &last, c, la, w);
+  const int maxoff = (p.guard_mode == SPARK_GUARD_REFILL) ? 0 : la[0];
+  return maxoff + p.nstencil;
+}
+
+int run_dt(spark_batch* b, int hist_slot) {
+  for (auto& c : b->classes) {
+    PlainLaunch L = make_launch(b, c);
+    HIPCHK(spark::plain_dt_part(L, b->U, b->dtparts, b->dtchunks));
+  }
+  HIPCHK(spark::plain_dt_final(g_stream, b->dp, b->dtparts, b->nblk, b->dtchunks, b->dtp, b->loc,
+                               hist_slot >= 0 ? b->dt_hist : nullptr, hist_slot < 0 ? 0 : hist_slot));
+  return SPARK_OK;
+}
+
+int run_fill(spark_batch* b, double* pool) {
+  PlainLaunch L = make_launch(b, b->classes[0]);
+  L.blist = nullptr;
+  L.nblk = b->nblk;
+  HIPCHK(spark::plain_ug_fill(L, b->ug[0], b->ug[1], b->ug[2], pool, b->nvar));
+  return SPARK_OK;
+}
+
+int run_step_begin(spark_batch* b) {
+  for (auto& c : b->classes) {
+    PlainLaunch L = make_launch(b, c);
+    HIPCHK(spark::plain_shock_detect(L, b->U, b->flat));
+  }
+  HIPCHK(hipMemcpyAsync(b->S, b->U, sizeof(double) * b->pool, hipMemcpyDeviceToDevice, g_stream));
+  b->step_open = true;
+  return SPARK_OK;
+}
+
+int run_stage(spark_batch* b, int stage, int ngoff) {
+  const double* co = b->coeff[stage - 1];
+  for (auto& c : b->classes) {
+    PlainLaunch L = make_launch(b, c);
+    L.weight = b->weights[stage - 1];
+    L.add_flux = stage > 1;
+    HIPCHK(spark::plain_stage(L, ngoff, co[0], co[1], co[2], stage == b->last_stage));
+  }
+  return SPARK_OK;
+}
+
+int run_step_end(spark_batch* b) {
+  for (auto& c : b->classes) {
+    PlainLaunch L = make_launch(b, c);
+    HIPCHK(spark::plain_copy_interior(L, b->S, b->U, b->nvar));
+  }
+  b->step_open = false;
+  return SPARK_OK;
+}
+
+int run_full_step(spark_batch* b) {
+  int rc;
+  if ((rc = run_step_begin(b))) return rc;
+  const bool refill = b->p.guard_mode == SPARK_GUARD_REFILL;
+  for (int s = 1; s <= b->last_stage; s++) {
+    if ((rc = run_stage(b, s, refill ? 0 : b->limits_array[s - 1]))) return rc;
+    if (refill && s < b->last_stage)
+      if ((rc = run_fill(b, b->S))) return rc;
+  }
+  return run_step_end(b);
+}
+
+void free_batch(spark_batch* b) {
+  if (!b) return;
+  auto F = [](void* p) { if (p) (void)hipFree(p); };
+  F(b->meta_d);
+  for (auto& c : b->classes) F(c.ids_d);
+  F(b->U); F(b->S); F(b->flat);
+  for (int d = 0; d < 3; d++) { F(b->fl[d]); F(b->fb[d]); }
+  F(b->dtp); F(b->loc); F(b->dt_hist); F(b->dtparts); F(b->err);
+  delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int spark_params_default(spark_params* p) {
+  if (!p) return fail(SPARK_ERR_ARG, "spark_params_default: null pointer");
+  memset(p, 0, sizeof(*p));
+  p->gamma = 1.6667;
+  p->cfl = 0.8;
+  p->cvisc = 0.1;
+  p->smlrho = 1e-10;
+  p->smallp = 1e-10;
+  p->smalle = 1e-10;
+  p->smallx = 1e-10;
+  p->eintSwitch = 0.0;
+  p->tiny = 1.e-32;
+  p->gasConst = 8.3144598E7;
+  p->abar = 1.0;
+  p->ndim = 3;
+  p->nguard = 6;
+  p->nstencil = 3;
+  p->rk_order = 2;
+  p->recon = SPARK_RECON_WENO5;
+  p->riemann = SPARK_RIEMANN_HLLC;
+  p->use_flattening = 1;
+  p->use_hybrid_riemann = 0;
+  p->nscalars = 0;
+  p->use_gravity = 0;
+  p->flux_correct = 0;
+  p->form = SPARK_FORM_PLAIN_FP64;
+  p->guard_mode = SPARK_GUARD_TELESCOPE;
+  p->nunk_vars = SPARK_NPROP;
+  for (int v = 0; v < SPARK_NPROP + SPARK_MAX_SCALARS; v++) p->var_index[v] = v;
+  return SPARK_OK;
+}
+
+int spark_init(int device) {
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(SPARK_ERR_ARG, "spark_init: device %d of %d", device, n);
+  HIPCHK(hipSetDevice(device));
+  if (g_stream == nullptr || g_device != device) {
+    if (g_stream) (void)hipStreamDestroy(g_stream);
+    HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  }
+  g_device = device;
+  return SPARK_OK;
+}
+
+int spark_finalize(void) {
+  if (g_stream) {
+    HIPCHK(hipStreamSynchronize(g_stream));
+    HIPCHK(hipStreamDestroy(g_stream));
+  }
+  g_stream = nullptr;
+  g_device = -1;
+  return SPARK_OK;
+}
+
+int spark_batch_create(const spark_params* p, int nblocks, const spark_block_desc* blocks, spark_batch** out) {
+  if (!p || !blocks || !out) return fail(SPARK_ERR_ARG, "spark_batch_create: null argument");
+  *out = nullptr;
+  if (g_stream == nullptr) return fail(SPARK_ERR_STATE, "spark_batch_create: call spark_init first");
+  if (nblocks <= 0) return fail(SPARK_ERR_ARG, "spark_batch_create: nblocks = %d", nblocks);
+  if (p->ndim < 1 || p->ndim > 3) return fail(SPARK_ERR_ARG, "ndim = %d", p->ndim);
+  if (p->nscalars < 0 || p->nscalars > SPARK_MAX_SCALARS)
+    return fail(SPARK_ERR_ARG, "nscalars = %d (0..%d)", p->nscalars, SPARK_MAX_SCALARS);
+  if (p->rk_order != 2 && p->rk_order != 3) return fail(SPARK_ERR_ARG, "rk_order = %d", p->rk_order);
+  if (p->recon < 0 || p->recon > 3) return fail(SPARK_ERR_ARG, "recon = %d", p->recon);
+  if (p->riemann < 0 || p->riemann > 1) return fail(SPARK_ERR_ARG, "riemann = %d", p->riemann);
+  if (p->form != SPARK_FORM_PLAIN_FP64 && p->form != SPARK_FORM_FP32_FLUX)
+    return fail(SPARK_ERR_ARG, "form = %d", p->form);
+  const int radius = (p->recon == SPARK_RECON_PLM) ? 1 : (p->recon == SPARK_RECON_FOG) ? 0 : 2;
+  if (p->nstencil < radius + 1) return fail(SPARK_ERR_ARG, "nstencil = %d too small for the reconstruction", p->nstencil);
+  if (p->nguard < required_guard(*p))
+    return fail(SPARK_ERR_ARG, "nguard = %d, this RK order / guard mode needs >= %d", p->nguard, required_guard(*p));
+  const int nvar = SPARK_NPROP + p->nscalars;
+  if (p->nunk_vars < nvar) return fail(SPARK_ERR_ARG, "nunk_vars = %d < %d", p->nunk_vars, nvar);
+  for (int v = 0; v < nvar; v++)
+    if (p->var_index[v] < 0 || p->var_index[v] >= p->nunk_vars)
+      return fail(SPARK_ERR_ARG, "var_index[%d] = %d out of range", v, p->var_index[v]);
+  if (p->form == SPARK_FORM_FP32_FLUX)
+    return fail(SPARK_ERR_ARG, "form SPARK_FORM_FP32_FLUX is served by spark_batched_* entry points");
+
+  spark_batch* b = new spark_batch();
+  b->p = *p;
+  b->nblk = nblocks;
+  b->nvar = nvar;
+  b->desc.assign(blocks, blocks + nblocks);
+  rk_tables(b->p, &b->last_stage, b->coeff, b->limits_array, b->weights);
+
+  DevParams& d = b->dp;
+  d.gamma = p->gamma; d.cvisc = p->cvisc; d.smlrho = p->smlrho; d.smallp = p->smallp; d.smalle = p->smalle;
+  d.smallx = p->smallx; d.eintSwitch = p->eintSwitch; d.tiny = p->tiny; d.cfl = p->cfl;
+  {
+    // Gamma/Eos_init.F90:86 and Gamma/Eos.F90:177,240-241
+    const double eos_gammam1 = 1.0 / (p->gamma - 1.0);
+    const double ggprod = eos_gammam1 * p->gasConst;
+    d.ggprodinv = 1. / ggprod;
+    d.gam1inv = 1. / eos_gammam1;
+  }
+  d.abar = p->abar;
+  d.grav[0] = p->grav[0]; d.grav[1] = p->grav[1]; d.grav[2] = p->grav[2];
+  d.ndim = p->ndim; d.ng = p->nguard; d.ns = p->nscalars;
+  d.use_grav = p->use_gravity; d.use_flat = p->use_flattening; d.hybrid = p->use_hybrid_riemann;
+  d.recon = p->recon; d.riemann = p->riemann;
+
+  const int g = p->nguard;
+  const int nf = 5 + p->nscalars;
+  std::map<std::tuple<int, int, int>, int> cls;
+  int64_t max_int = 0;
+  for (int i = 0; i < nblocks; i++) {
+    const spark_block_desc& bd = blocks[i];
+    int n[3] = {bd.nx, p->ndim > 1 ? bd.ny : 1, p->ndim > 2 ? bd.nz : 1};
+    for (int q = 0; q < p->ndim; q++)
+      if (n[q] < 1) { free_batch(b); return fail(SPARK_ERR_ARG, "block %d has an empty dimension", i); }
+    BlockMeta m;
+    memset(&m, 0, sizeof(m));
+    m.nx = n[0]; m.ny = n[1]; m.nz = n[2];
+    m.NX = n[0] + 2 * g;
+    m.NY = p->ndim > 1 ? n[1] + 2 * g : 1;
+    m.NZ = p->ndim > 2 ? n[2] + 2 * g : 1;
+    m.dx = bd.dx; m.dy = bd.dy; m.dz = bd.dz;
+    m.level = bd.level; m.id = i;
+    const int64_t ncell = (int64_t)m.NX * m.NY * m.NZ;
+    m.cstride = round_up(ncell, 64);
+    m.base = b->pool;
+    b->pool += (int64_t)nvar * m.cstride;
+    m.obase = b->opool;
+    b->opool += m.cstride;
+    m.fstride = round_up((int64_t)(m.NX + 1) * (m.NY + 1) * (m.NZ + 1), 64);
+    m.fbase = b->fpool;
+    b->fpool += (int64_t)nf * m.fstride;
+    int64_t reg = 1;
+    for (int q = 0; q < 3; q++) reg *= n[q] + (q < p->ndim ? 1 : 0);
+    m.bbase = b->bpool;
+    b->bpool += (int64_t)nf * round_up(reg, 64);
+    b->max_cells = std::max<int64_t>(b->max_cells, ncell);
+    max_int = std::max<int64_t>(max_int, (int64_t)n[0] * n[1] * n[2]);
+    b->meta_h.push_back(m);
+    auto key = std::make_tuple(n[0], n[1], n[2]);
+    auto it = cls.find(key);
+    if (it == cls.end()) {
+      cls[key] = (int)b->classes.size();
+      ShapeClass c;
+      c.nx = n[0]; c.ny = n[1]; c.nz = n[2];
+      b->classes.push_back(c);
+      it = cls.find(key);
+    }
+    b->classes[it->second].ids.push_back(i);
+  }
+  b->dtchunks = (int)std::min<int64_t>(16, std::max<int64_t>(1, (max_int + 255) / 256));
+
+#define ALLOC(ptr, bytes)                                                              \
+  do {                                                                                 \
+    hipError_t e_ = hipMalloc((void**)&(ptr), (bytes));                                \
+    if (e_ != hipSuccess) {                                                            \
+      free_batch(b);                                                                   \
+      return fail(SPARK_ERR_HIP, "hipMalloc(%zu bytes): %s", (size_t)(bytes), hipGetErrorString(e_)); \
+    }                                                                                  \
+  } while (0)
+  ALLOC(b->meta_d, sizeof(BlockMeta) * nblocks);
+  for (auto& c : b->classes) ALLOC(c.ids_d, sizeof(int) * c.ids.size());
+  ALLOC(b->U, sizeof(double) * b->pool);
+  ALLOC(b->S, sizeof(double) * b->pool);
+  ALLOC(b->flat, sizeof(double) * b->opool);
+  for (int q = 0; q < 3; q++) ALLOC(b->fl[q], sizeof(double) * b->fpool);
+  if (p->flux_correct)
+    for (int q = 0; q < 3; q++) ALLOC(b->fb[q], sizeof(double) * b->bpool);
+  ALLOC(b->dtp, sizeof(double));
+  ALLOC(b->loc, sizeof(long long) * 2);
+  b->hist_cap = 4096;
+  ALLOC(b->dt_hist, sizeof(double) * b->hist_cap);
+  ALLOC(b->dtparts, spark::plain_dt_part_size() * (size_t)nblocks * b->dtchunks);
+  ALLOC(b->err, sizeof(int));
+#undef ALLOC
+  HIPCHK(hipMemcpyAsync(b->meta_d, b->meta_h.data(), sizeof(BlockMeta) * nblocks, hipMemcpyHostToDevice, g_stream));
+  for (auto& c : b->classes)
+    HIPCHK(hipMemcpyAsync(c.ids_d, c.ids.data(), sizeof(int) * c.ids.size(), hipMemcpyHostToDevice, g_stream));
+  HIPCHK(hipMemsetAsync(b->U, 0, sizeof(double) * b->pool, g_stream));
+  HIPCHK(hipMemsetAsync(b->S, 0, sizeof(double) * b->pool, g_stream));
+  HIPCHK(hipMemsetAsync(b->flat, 0, sizeof(double) * b->opool, g_stream));
+  for (int q = 0; q < 3; q++) HIPCHK(hipMemsetAsync(b->fl[q], 0, sizeof(double) * b->fpool, g_stream));
+  if (p->flux_correct)
+    for (int q = 0; q < 3; q++) HIPCHK(hipMemsetAsync(b->fb[q], 0, sizeof(double) * b->bpool, g_stream));
+  HIPCHK(hipMemsetAsync(b->err, 0, sizeof(int), g_stream));
+  HIPCHK(hipMemsetAsync(b->loc, 0, sizeof(long long) * 2, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  *out = b;
+  return SPARK_OK;
+}
+
+int spark_batch_destroy(spark_batch* b) {
+  if (!b) return SPARK_OK;
+  if (g_stream) HIPCHK(hipStreamSynchronize(g_stream));
+  free_batch(b);
+  return SPARK_OK;
+}
+
+static int transfer(spark_batch* b, double* const* host, int to_device, int interior_only, double* pool) {
+  const int nunk = b->p.nunk_vars;
+  // stage at most ~256 MiB of AoS data per round trip
+  const int64_t cap = std::max<int64_t>((int64_t)nunk * b->max_cells, (int64_t)32 << 20);
+  double* stage = nullptr;
+  int64_t* off_d = nullptr;
+  int* vidx_d = nullptr;
+  HIPCHK(hipMalloc((void**)&stage, sizeof(double) * cap));
+  HIPCHK(hipMalloc((void**)&off_d, sizeof(int64_t) * b->nblk));
+  HIPCHK(hipMalloc((void**)&vidx_d, sizeof(int) * b->nvar));
+  HIPCHK(hipMemcpy(vidx_d, b->p.var_index, sizeof(int) * b->nvar, hipMemcpyHostToDevice));
+  int rc = SPARK_OK;
+  int i0 = 0;
+  while (i0 < b->nblk && rc == SPARK_OK) {
+    std::vector<int64_t> off;
+    int64_t used = 0;
+    int i1 = i0;
+    while (i1 < b->nblk) {
+      const BlockMeta& m = b->meta_h[i1];
+      const int64_t sz = (int64_t)nunk * m.NX * m.NY * m.NZ;
+      if (used + sz > cap && i1 > i0) break;
+      off.push_back(used);
+      used += sz;
+      i1++;
+    }
+    hipError_t e = hipMemcpy(off_d, off.data(), sizeof(int64_t) * off.size(), hipMemcpyHostToDevice);
+    for (int i = i0; i < i1 && e == hipSuccess; i++) {
+      const BlockMeta& m = b->meta_h[i];
+      const int64_t sz = (int64_t)nunk * m.NX * m.NY * m.NZ;
+      // unpack also stages the host block first, so unk slots outside the
+      // variable map (and guards when interior_only) keep their host values
+      e = hipMemcpy(stage + off[i - i0], host[i], sizeof(double) * sz, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) {
+      if (to_device)
+        e = spark::plain_aos_to_soa(g_stream, b->meta_d + i0, i1 - i0, (int)b->max_cells, stage, off_d, nunk, vidx_d,
+                                    b->nvar, pool);
+      else
+        e = spark::plain_soa_to_aos(g_stream, b->dp, b->meta_d + i0, i1 - i0, (int)b->max_cells, pool, stage, off_d,
+                                    nunk, vidx_d, b->nvar, interior_only);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+    for (int i = i0; i < i1 && e == hipSuccess && !to_device; i++) {
+      const BlockMeta& m = b->meta_h[i];
+      const int64_t sz = (int64_t)nunk * m.NX * m.NY * m.NZ;
+      e = hipMemcpy(host[i], stage + off[i - i0], sizeof(double) * sz, hipMemcpyDeviceToHost);
+    }
+    if (e != hipSuccess) rc = fail(SPARK_ERR_HIP, "pack/unpack: %s", hipGetErrorString(e));
+    i0 = i1;
+  }
+  (void)hipFree(stage);
+  (void)hipFree(off_d);
+  (void)hipFree(vidx_d);
+  return rc;
+}
+
+int spark_pack(spark_batch* b, const double* const* unk_blocks) {
+  if (!b || !unk_blocks) return fail(SPARK_ERR_ARG, "spark_pack: null argument");
+  for (int i = 0; i < b->nblk; i++)
+    if (!unk_blocks[i]) return fail(SPARK_ERR_ARG, "spark_pack: block %d pointer is null", i);
+  return transfer(b, const_cast<double* const*>(unk_blocks), 1, 0, b->U);
+}
+
+int spark_unpack(spark_batch* b, double* const* unk_blocks, int interior_only, int which) {
+  if (!b || !unk_blocks) return fail(SPARK_ERR_ARG, "spark_unpack: null argument");
+  for (int i = 0; i < b->nblk; i++)
+    if (!unk_blocks[i]) return fail(SPARK_ERR_ARG, "spark_unpack: block %d pointer is null", i);
+  HIPCHK(hipStreamSynchronize(g_stream));
+  return transfer(b, unk_blocks, 0, interior_only, which ? b->S : b->U);
+}
+
+int spark_set_uniform_grid(spark_batch* b, int nbx, int nby, int nbz) {
+  if (!b) return fail(SPARK_ERR_ARG, "spark_set_uniform_grid: null batch");
+  if (nbx < 1 || nby < 1 || nbz < 1 || (int64_t)nbx * nby * nbz != b->nblk)
+    return fail(SPARK_ERR_ARG, "uniform grid %d x %d x %d does not match %d blocks", nbx, nby, nbz, b->nblk);
+  if (b->classes.size() != 1) return fail(SPARK_ERR_ARG, "uniform grid needs blocks of one shape");
+  if ((b->p.ndim < 2 && nby != 1) || (b->p.ndim < 3 && nbz != 1))
+    return fail(SPARK_ERR_ARG, "uniform grid extent in an unused dimension");
+  b->ug[0] = nbx; b->ug[1] = nby; b->ug[2] = nbz;
+  b->has_ug = true;
+  return SPARK_OK;
+}
+
+int spark_fill_guardcells(spark_batch* b) {
+  if (!b) return fail(SPARK_ERR_ARG, "spark_fill_guardcells: null batch");
+  if (!b->has_ug) return fail(SPARK_ERR_STATE, "spark_fill_guardcells: no uniform grid declared");
+  return run_fill(b, b->U);
+}
+
+int spark_compute_dt(spark_batch* b, double* dt, int minloc[5]) {
+  if (!b || !dt) return fail(SPARK_ERR_ARG, "spark_compute_dt: null argument");
+  int rc = run_dt(b, -1);
+  if (rc) return rc;
+  long long loc[2];
+  HIPCHK(hipMemcpyAsync(dt, b->dtp, sizeof(double), hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(hipMemcpyAsync(loc, b->loc, sizeof(loc), hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  if (minloc) {
+    const int bid = (int)loc[0];
+    const BlockMeta& m = b->meta_h[bid < 0 || bid >= b->nblk ? 0 : bid];
+    const long long c = loc[1];
+    minloc[0] = bid;
+    minloc[1] = (int)(c % m.NX) + 1;
+    minloc[2] = (int)((c / m.NX) % m.NY) + 1;
+    minloc[3] = (int)(c / ((long long)m.NX * m.NY)) + 1;
+    minloc[4] = m.level;
+  }
+  return SPARK_OK;
+}
+
+int spark_hydro_step(spark_batch* b, double dt, double dtOld) {
+  (void)dtOld;  // only the Poisson-gravity extrapolation reads dtOld (hy_rk_updateSoln.F90-mc:291-292)
+  if (!b) return fail(SPARK_ERR_ARG, "spark_hydro_step: null batch");
+  if (b->p.guard_mode == SPARK_GUARD_REFILL && !b->has_ug)
+    return fail(SPARK_ERR_STATE, "refill guard mode needs spark_set_uniform_grid");
+  HIPCHK(spark::plain_set_scalar(g_stream, b->dtp, dt));
+  return run_full_step(b);
+}
+
+int spark_step_begin(spark_batch* b) {
+  if (!b) return fail(SPARK_ERR_ARG, "spark_step_begin: null batch");
+  return run_step_begin(b);
+}
+
+int spark_stage(spark_batch* b, int stage, double dt, int ngoff) {
+  if (!b) return fail(SPARK_ERR_ARG, "spark_stage: null batch");
+  if (!b->step_open) return fail(SPARK_ERR_STATE, "spark_stage: call spark_step_begin first");
+  if (stage < 1 || stage > b->last_stage) return fail(SPARK_ERR_ARG, "stage %d of %d", stage, b->last_stage);
+  if (ngoff < 0 || ngoff + b->p.nstencil > b->p.nguard)
+    return fail(SPARK_ERR_ARG, "ngoff = %d needs nguard >= %d", ngoff, ngoff + b->p.nstencil);
+  HIPCHK(spark::plain_set_scalar(g_stream, b->dtp, dt));
+  return run_stage(b, stage, ngoff);
+}
+
+int spark_step_end(spark_batch* b) {
+  if (!b) return fail(SPARK_ERR_ARG, "spark_step_end: null batch");
+  if (!b->step_open) return fail(SPARK_ERR_STATE, "spark_step_end: no open step");
+  return run_step_end(b);
+}
+
+int spark_advance(spark_batch* b, int nsteps, double* dts) {
+  if (!b) return fail(SPARK_ERR_ARG, "spark_advance: null batch");
+  if (nsteps < 0) return fail(SPARK_ERR_ARG, "nsteps = %d", nsteps);
+  if (b->p.guard_mode == SPARK_GUARD_REFILL && !b->has_ug)
+    return fail(SPARK_ERR_STATE, "refill guard mode needs spark_set_uniform_grid");
+  if (nsteps > b->hist_cap) {
+    (void)hipFree(b->dt_hist);
+    b->dt_hist = nullptr;
+    HIPCHK(hipMalloc((void**)&b->dt_hist, sizeof(double) * nsteps));
+    b->hist_cap = nsteps;
+  }
+  for (int s = 0; s < nsteps; s++) {
+    int rc = run_dt(b, s);
+    if (rc) return rc;
+    if (b->has_ug && (rc = run_fill(b, b->U))) return rc;
+    if ((rc = run_full_step(b))) return rc;
+  }
+  if (dts && nsteps > 0) {
+    HIPCHK(hipMemcpyAsync(dts, b->dt_hist, sizeof(double) * nsteps, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+  }
+  return SPARK_OK;
+}
+
+static int reg_extent(const spark_batch* b, int i, int dir, int e[3]) {
+  const BlockMeta& m = b->meta_h[i];
+  e[0] = m.nx; e[1] = m.ny; e[2] = m.nz;
+  e[dir] += 1;
+  return e[0] * e[1] * e[2];
+}
+
+int spark_get_flux_registers(spark_batch* b, double* const* fbx, double* const* fby, double* const* fbz) {
+  if (!b) return fail(SPARK_ERR_ARG, "spark_get_flux_registers: null batch");
+  if (!b->p.flux_correct) return fail(SPARK_ERR_STATE, "flux registers need flux_correct = 1");
+  double* const* outs[3] = {fbx, fby, fbz};
+  const int nf = 5 + b->p.nscalars;
+  HIPCHK(hipStreamSynchronize(g_stream));
+  std::vector<double> tmp;
+  for (int dir = 0; dir < b->p.ndim; dir++) {
+    if (!outs[dir]) return fail(SPARK_ERR_ARG, "flux register array %d is null", dir);
+    for (int i = 0; i < b->nblk; i++) {
+      int e[3];
+      const int tot = reg_extent(b, i, dir, e);
+      tmp.resize((size_t)nf * tot);
+      HIPCHK(hipMemcpy(tmp.data(), b->fb[dir] + b->meta_h[i].bbase, sizeof(double) * nf * tot, hipMemcpyDeviceToHost));
+      double* o = outs[dir][i];
+      for (int t = 0; t < tot; t++)
+        for (int n = 0; n < nf; n++) o[(size_t)t * nf + n] = tmp[(size_t)n * tot + t];
+    }
+  }
+  return SPARK_OK;
+}
+
+int spark_set_flux_corrections(spark_batch* b, const double* const* fbx, const double* const* fby,
+                               const double* const* fbz) {
+  if (!b) return fail(SPARK_ERR_ARG, "spark_set_flux_corrections: null batch");
+  if (!b->p.flux_correct) return fail(SPARK_ERR_STATE, "flux corrections need flux_correct = 1");
+  const double* const* ins[3] = {fbx, fby, fbz};
+  const int nf = 5 + b->p.nscalars;
+  HIPCHK(hipStreamSynchronize(g_stream));
+  std::vector<double> tmp;
+  for (int dir = 0; dir < b->p.ndim; dir++) {
+    if (!ins[dir]) return fail(SPARK_ERR_ARG, "flux correction array %d is null", dir);
+    for (int i = 0; i < b->nblk; i++) {
+      int e[3];
+      const int tot = reg_extent(b, i, dir, e);
+      tmp.resize((size_t)nf * tot);
+      const double* in = ins[dir][i];
+      for (int t = 0; t < tot; t++)
+        for (int n = 0; n < nf; n++) tmp[(size_t)n * tot + t] = in[(size_t)t * nf + n];
+      HIPCHK(hipMemcpy(b->fb[dir] + b->meta_h[i].bbase, tmp.data(), sizeof(double) * nf * tot, hipMemcpyHostToDevice));
+    }
+  }
+  return SPARK_OK;
+}
+
+int spark_flux_correct(spark_batch* b, double dt) {
+  if (!b) return fail(SPARK_ERR_ARG, "spark_flux_correct: null batch");
+  if (!b->p.flux_correct) return fail(SPARK_ERR_STATE, "spark_flux_correct needs flux_correct = 1");
+  HIPCHK(spark::plain_set_scalar(g_stream, b->dtp, dt));
+  for (auto& c : b->classes) {
+    PlainLaunch L = make_launch(b, c);
+    HIPCHK(spark::plain_correct(L, b->U));
+  }
+  return SPARK_OK;
+}
+
+int spark_synchronize(spark_batch* b) {
+  (void)b;
+  if (g_stream) HIPCHK(hipStreamSynchronize(g_stream));
+  return SPARK_OK;
+}
+
+int spark_get_counters(spark_batch* b, long long* counters, int n) {
+  if (!b || !counters || n < 1) return fail(SPARK_ERR_ARG, "spark_get_counters: bad argument");
+  int e = 0;
+  HIPCHK(hipMemcpyAsync(&e, b->err, sizeof(int), hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  counters[0] = e;
+  for (int i = 1; i < n; i++) counters[i] = 0;
+  return SPARK_OK;
+}
+
+int spark_last_error(char* buf, int len) {
+  if (!buf || len <= 0) return SPARK_ERR_ARG;
+  snprintf(buf, (size_t)len, "%s", g_err.c_str());
+  return SPARK_OK;
+}
+
+}  // extern "C"

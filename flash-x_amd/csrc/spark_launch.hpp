@@ -1,0 +1,46 @@
+// spark_launch.hpp -- host-side launch interface between the C-ABI layer
+// (spark_capi.cpp) and the kernel translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "spark_physics.hpp"
+
+namespace spark {
+
+// Everything one plain-form launch sequence needs for a list of blocks that
+// share one interior shape.
+struct PlainLaunch {
+  DevParams P;
+  hipStream_t stream;
+  const BlockMeta* meta;  // device block table (all blocks of the batch)
+  const int* blist;       // device list of block ids in this shape class
+  int nblk;               // length of blist
+  int nx, ny, nz;         // interior shape of the class
+  const double* U0;       // step input (hy_tmpState role)
+  double* S;              // stage state (hy_starState role)
+  double* flat;           // flattening coefficients / scratch (cell pool)
+  double* fl[3];          // face-flux pools
+  double* fb[3];          // flux-register pools (nullptr when not correcting)
+  double weight;          // flux-register weight of this stage
+  int add_flux;           // 0: overwrite (stage 1), 1: accumulate
+  const double* dtp;      // device scalar holding dt
+  int* err;               // device counter of negative a^2 events
+};
+
+hipError_t plain_stage(const PlainLaunch& L, int ngoff, double c1, double c2, double c3, int final_stage);
+hipError_t plain_copy_interior(const PlainLaunch& L, const double* S, double* U, int nvar);
+hipError_t plain_shock_detect(const PlainLaunch& L, double* U, double* scratch);
+hipError_t plain_dt_part(const PlainLaunch& L, const double* U, void* parts, int chunks);
+hipError_t plain_dt_final(hipStream_t st, const DevParams& P, const void* parts, int nblk_total, int chunks,
+                          double* dt_out, long long* loc_out, double* dt_hist, int hist_slot);
+hipError_t plain_set_scalar(hipStream_t st, double* p, double v);
+size_t plain_dt_part_size();
+hipError_t plain_ug_fill(const PlainLaunch& L, int nbx, int nby, int nbz, double* U, int nvar);
+hipError_t plain_correct(const PlainLaunch& L, double* U);
+hipError_t plain_aos_to_soa(hipStream_t st, const BlockMeta* meta, int nblk, int maxcells, const double* aos,
+                            const int64_t* aos_off, int nunk, const int* var_index, int nvar, double* U);
+hipError_t plain_soa_to_aos(hipStream_t st, const DevParams& P, const BlockMeta* meta, int nblk, int maxcells,
+                            const double* U, double* aos, const int64_t* aos_off, int nunk, const int* var_index,
+                            int nvar, int interior);
+
+}  // namespace spark

@@ -1,0 +1,790 @@
+// spark_plain.hip -- the plain FP64 form of the Spark RK stage on gfx950.
+//
+// One launch per kernel per stage over all blocks of one shape (blockIdx.y
+// walks the block list).  The dataflow mirrors the reference per stage
+// (Spark/Hydro.F90-mc:144-194): flattening -> face fluxes per direction
+// (materialised in face arrays) -> conserved update + floors + Gamma EOS.
+// Built with -ffp-contract=off: every FP64 expression rounds exactly as the
+// Fortran line it restates, so results match the CPU restatement bit for
+// bit (IEEE division and square root on both sides).
+#include "spark_physics.hpp"
+#include "spark_launch.hpp"
+
+namespace spark {
+
+struct Lims { int lo[3], hi[3]; };
+
+__device__ __forceinline__ Lims stage_lims(const DevParams& P, const BlockMeta& m, int ngoff) {
+  Lims L;
+  const int n[3] = {m.nx, m.ny, m.nz};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    if (d < P.ndim) { L.lo[d] = P.ng - ngoff; L.hi[d] = P.ng + n[d] - 1 + ngoff; }
+    else { L.lo[d] = 0; L.hi[d] = 0; }
+  }
+  return L;
+}
+
+__device__ __forceinline__ int64_t cix(const BlockMeta& m, int i, int j, int k) {
+  return (int64_t)i + (int64_t)m.NX * ((int64_t)j + (int64_t)m.NY * (int64_t)k);
+}
+__device__ __forceinline__ int64_t fix(const BlockMeta& m, int i, int j, int k) {
+  return (int64_t)i + (int64_t)(m.NX + 1) * ((int64_t)j + (int64_t)(m.NY + 1) * (int64_t)k);
+}
+
+// ---------------------------------------------------------------------------
+// flattening over limits +/- 1 (hy_rk_getFaceFlux.F90-mc:200-276)
+__global__ void __launch_bounds__(256) k_flatten(DevParams P, const BlockMeta* __restrict__ meta,
+                                                 const int* __restrict__ blist, const double* __restrict__ S,
+                                                 double* __restrict__ flat, int ngoff) {
+  const BlockMeta m = meta[blist[blockIdx.y]];
+  Lims L = stage_lims(P, m, ngoff);
+  const int kx = 1, ky = (P.ndim > 1), kz = (P.ndim == 3);
+  const int lo0 = L.lo[0] - kx, lo1 = L.lo[1] - ky, lo2 = L.lo[2] - kz;
+  const int ex = L.hi[0] + kx - lo0 + 1, ey = L.hi[1] + ky - lo1 + 1, ez = L.hi[2] + kz - lo2 + 1;
+  const int total = ex * ey * ez;
+  const double* sp = S + m.base;
+  const int64_t cs = m.cstride;
+  const int64_t str[3] = {1, (int64_t)m.NX, (int64_t)m.NX * m.NY};
+  const int velv[3] = {V_VELX, V_VELY, V_VELZ};
+  const double betaMin = 0.75, betaMax = 0.85, Zmin = 0.25, Zmax = 0.75;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    int i = lo0 + t % ex, j = lo1 + (t / ex) % ey, k = lo2 + t / (ex * ey);
+    int64_t c = cix(m, i, j, k);
+    double mn = 0.;
+    for (int d = 0; d < P.ndim; d++) {
+      const int64_t s = str[d];
+      const double* p = sp + V_PRES * cs;
+      double dp1 = __builtin_fabs(p[c + s] - p[c - s]);
+      double beta = dp1 / fmax2(Num<double>::flat_floor, __builtin_fabs(p[c + 2 * s] - p[c - 2 * s]));
+      double Z = dp1 / (sp[V_GAMC * cs + c] * p[c]);
+      double ft = fmax2(0., fmin2(1., (betaMax - beta) / (betaMax - betaMin)));
+      const double* u = sp + velv[d] * cs;
+      if (u[c + s] < u[c]) ft = fmax2(ft, fmin2(1., (Zmax - Z) / (Zmax - Zmin)));
+      else ft = 1.0;
+      mn = (d == 0) ? ft : fmin2(mn, ft);
+    }
+    flat[m.obase + c] = mn;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// face fluxes for one sweep direction.  A workgroup owns `lanes` pencils
+// (DIR 0: rows of x-pencils; DIR 1/2: x-columns of y/z pencils, so adjacent
+// lanes always touch adjacent x cells and every global access coalesces).
+// Each thread reconstructs one cell; the i-1/2 face of that cell takes the
+// i+1/2 state of the cell below through LDS.
+template <int DIR, int RECON, int RIEMANN, int NSM>
+__global__ void __launch_bounds__(1024) k_flux(DevParams P, const BlockMeta* __restrict__ meta,
+                                               const int* __restrict__ blist, const double* __restrict__ S,
+                                               const double* __restrict__ flat, double* __restrict__ fl, int ngoff,
+                                               int lanes, int* __restrict__ err) {
+  constexpr int R = ReconRadius<RECON>::R;
+  constexpr int NRM = R_NUM + NSM;
+  constexpr int NFM = F_NUM + NSM;
+  extern __shared__ double sm[];
+  const int ns = P.ns;
+  const BlockMeta m = meta[blist[blockIdx.y]];
+  Lims L = stage_lims(P, m, ngoff);
+  constexpr int T1 = (DIR == 0) ? 1 : 0;
+  constexpr int T2 = (DIR == 2) ? 1 : 2;
+  const int Lr = L.hi[DIR] - L.lo[DIR] + 3;
+  const int nt1 = L.hi[T1] - L.lo[T1] + 1;
+  const int chunks = (nt1 + lanes - 1) / lanes;
+  const int c1 = blockIdx.x % chunks, c2 = blockIdx.x / chunks;
+  const int NT = blockDim.x;
+  const int tid = threadIdx.x;
+  int s, a;
+  if (DIR == 0) { s = tid % Lr; a = tid / Lr; }
+  else { a = tid % lanes; s = tid / lanes; }
+  const int t1c = L.lo[T1] + c1 * lanes + a;
+  const bool valid = (a < lanes) && (t1c <= L.hi[T1]) && (s < Lr);
+  int ijk[3];
+  ijk[DIR] = L.lo[DIR] - 1 + s;
+  ijk[T1] = t1c;
+  ijk[T2] = L.lo[T2] + c2;
+  const int64_t c = cix(m, ijk[0], ijk[1], ijk[2]);
+  const int64_t st = (DIR == 0) ? 1 : (DIR == 1) ? (int64_t)m.NX : (int64_t)m.NX * m.NY;
+  const double* sp = S + m.base;
+  const int64_t cs = m.cstride;
+
+  double umr[NRM];
+  if (valid) {
+    const double fl_c = (RECON == RECON_WENO5 && P.use_flat) ? flat[m.obase + c] : 1.0;
+    double ud[5], u[5], up, um;
+#pragma unroll
+    for (int q = -R; q <= R; q++) ud[q + 2] = sp[V_DENS * cs + c + q * st];
+    recon1<RECON>(ud, fl_c, up, um);
+    sm[R_DENS * NT + tid] = up;
+    umr[R_DENS] = um;
+    const int svar[5] = {V_VELX, V_VELY, V_VELZ, V_PRES, V_GAMC};
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+#pragma unroll
+      for (int q = -R; q <= R; q++) u[q + 2] = sp[svar[v] * cs + c + q * st];
+      recon1<RECON>(u, fl_c, up, um);
+      sm[(1 + v) * NT + tid] = up;
+      umr[1 + v] = um;
+      if (v < 3) sm[(NRM + 1 + v) * NT + tid] = u[2];
+    }
+#pragma unroll
+    for (int q = -R; q <= R; q++) u[q + 2] = ud[q + 2] * sp[V_EINT * cs + c + q * st];
+    recon1<RECON>(u, fl_c, up, um);
+    sm[R_RHOE * NT + tid] = up;
+    umr[R_RHOE] = um;
+    sm[(NRM + 4) * NT + tid] = u[2];
+    sm[(NRM + 0) * NT + tid] = ud[2];
+    sm[(NRM + 5) * NT + tid] = sp[V_SHOK * cs + c];
+#pragma unroll
+    for (int v = 0; v < NSM; v++) {
+      if (v < ns) {
+#pragma unroll
+        for (int q = -R; q <= R; q++) u[q + 2] = sp[(V_NPROP + v) * cs + c + q * st];
+        recon1<RECON>(u, fl_c, up, um);
+        sm[(R_NUM + v) * NT + tid] = up;
+        umr[R_NUM + v] = um;
+      }
+    }
+  }
+  __syncthreads();
+  if (!valid || s < 1) return;
+  const int tn = (DIR == 0) ? tid - 1 : tid - lanes;
+  double left[NRM], right[NRM];
+#pragma unroll
+  for (int v = 0; v < NRM; v++) {
+    if (v < R_NUM + ns) {
+      left[v] = sm[v * NT + tn];
+      right[v] = umr[v];
+    }
+  }
+  left[R_DENS] = fmax2(P.smlrho, left[R_DENS]);
+  left[R_PRES] = fmax2(P.smallp, left[R_PRES]);
+  right[R_DENS] = fmax2(P.smlrho, right[R_DENS]);
+  right[R_PRES] = fmax2(P.smallp, right[R_PRES]);
+  const bool inShock = (sm[(NRM + 5) * NT + tn] != 0.0) || (sm[(NRM + 5) * NT + tid] != 0.0);
+  double F[NFM];
+  int ierr;
+  if (RIEMANN == RIEMANN_HLL) ierr = hll<DIR, double>(left, right, F);
+  else ierr = hllc<DIR, double>(left, right, P.hybrid != 0, inShock, P.tiny, F);
+  if (ierr) {
+#pragma unroll
+    for (int n = 0; n < F_NUM; n++) F[n] = 0.;
+    atomicAdd(err, 1);
+  }
+  // artificial viscosity (Hydro.ini:285-301)
+  {
+    const double dL = sm[(NRM + 0) * NT + tn], dH = sm[(NRM + 0) * NT + tid];
+    const double xL = sm[(NRM + 1) * NT + tn], xH = sm[(NRM + 1) * NT + tid];
+    const double yL = sm[(NRM + 2) * NT + tn], yH = sm[(NRM + 2) * NT + tid];
+    const double zL = sm[(NRM + 3) * NT + tn], zH = sm[(NRM + 3) * NT + tid];
+    const double eL = sm[(NRM + 4) * NT + tn], eH = sm[(NRM + 4) * NT + tid];
+    const double nL = (DIR == 0) ? xL : (DIR == 1) ? yL : zL;
+    const double nH = (DIR == 0) ? xH : (DIR == 1) ? yH : zH;
+    double cvisc = P.cvisc * fmax2(-(nH - nL), 0.);
+    double VenerLo = dL * 0.5 * (xL * xL + yL * yL + zL * zL) + eL;
+    double VenerHi = dH * 0.5 * (xH * xH + yH * yH + zH * zH) + eH;
+    F[F_MASS] = F[F_MASS] + cvisc * (dL - dH);
+    F[F_XMOM] = F[F_XMOM] + cvisc * (dL * xL - dH * xH);
+    F[F_YMOM] = F[F_YMOM] + cvisc * (dL * yL - dH * yH);
+    F[F_ZMOM] = F[F_ZMOM] + cvisc * (dL * zL - dH * zH);
+    F[F_ENER] = F[F_ENER] + cvisc * (VenerLo - VenerHi);
+  }
+#pragma unroll
+  for (int v = 0; v < NSM; v++) {
+    if (v < ns) {
+      if (F[F_MASS] > 0.) F[F_NUM + v] = left[R_NUM + v] * F[F_MASS];
+      else F[F_NUM + v] = right[R_NUM + v] * F[F_MASS];
+    }
+  }
+  const int64_t f = m.fbase + fix(m, ijk[0], ijk[1], ijk[2]);
+#pragma unroll
+  for (int n = 0; n < NFM; n++)
+    if (n < F_NUM + ns) fl[f + n * m.fstride] = F[n];
+}
+
+// ---------------------------------------------------------------------------
+// conserved update + floors (hy_rk_updateSoln.F90-mc:211-287) and the
+// Gamma EOS in MODE_DENS_EI (Eos_wrapped over the same limits).
+template <int NSM>
+__global__ void __launch_bounds__(256) k_update(DevParams P, const BlockMeta* __restrict__ meta,
+                                                const int* __restrict__ blist, const double* __restrict__ U0,
+                                                double* __restrict__ S, const double* __restrict__ flx,
+                                                const double* __restrict__ fly, const double* __restrict__ flz,
+                                                const double* __restrict__ dtp, double c1, double c2, double c3,
+                                                int ngoff, int final_stage) {
+  constexpr int NFM = F_NUM + NSM;
+  const BlockMeta m = meta[blist[blockIdx.y]];
+  Lims L = stage_lims(P, m, ngoff);
+  const int ex = L.hi[0] - L.lo[0] + 1, ey = L.hi[1] - L.lo[1] + 1, ez = L.hi[2] - L.lo[2] + 1;
+  const int total = ex * ey * ez;
+  const int ns = P.ns;
+  const double dt = *dtp;
+  const double dx = m.dx, dy = m.dy, dz = m.dz;
+  const int64_t cs = m.cstride, fs = m.fstride;
+  const double* u0 = U0 + m.base;
+  double* us = S + m.base;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const int i = L.lo[0] + t % ex, j = L.lo[1] + (t / ex) % ey, k = L.lo[2] + t / (ex * ey);
+    const int64_t c = cix(m, i, j, k);
+    double A[NFM], B[NFM];
+    const double d0 = u0[V_DENS * cs + c];
+    A[F_MASS] = d0;
+    A[F_XMOM] = d0 * u0[V_VELX * cs + c];
+    A[F_YMOM] = d0 * u0[V_VELY * cs + c];
+    A[F_ZMOM] = d0 * u0[V_VELZ * cs + c];
+    A[F_ENER] = d0 * u0[V_ENER * cs + c];
+    const double ds = us[V_DENS * cs + c];
+    B[F_MASS] = ds;
+    B[F_XMOM] = ds * us[V_VELX * cs + c];
+    B[F_YMOM] = ds * us[V_VELY * cs + c];
+    B[F_ZMOM] = ds * us[V_VELZ * cs + c];
+    B[F_ENER] = ds * us[V_ENER * cs + c];
+#pragma unroll
+    for (int v = 0; v < NSM; v++) {
+      if (v < ns) {
+        A[F_NUM + v] = u0[(V_NPROP + v) * cs + c] * d0;
+        B[F_NUM + v] = us[(V_NPROP + v) * cs + c] * ds;
+      }
+    }
+    double Sg[NFM];
+#pragma unroll
+    for (int n = 0; n < NFM; n++) Sg[n] = 0.;
+    if (P.use_grav) {
+      Sg[F_XMOM] = B[F_MASS] * P.grav[0];
+      Sg[F_YMOM] = B[F_MASS] * P.grav[1];
+      Sg[F_ZMOM] = B[F_MASS] * P.grav[2];
+      Sg[F_ENER] = B[F_XMOM] * P.grav[0] + B[F_YMOM] * P.grav[1] + B[F_ZMOM] * P.grav[2];
+    }
+    const int64_t fm = m.fbase + fix(m, i, j, k);
+    const int64_t fpx = m.fbase + fix(m, i + 1, j, k);
+    const int64_t fpy = m.fbase + fix(m, i, j + 1, k);
+    const int64_t fpz = m.fbase + fix(m, i, j, k + 1);
+#pragma unroll
+    for (int n = 0; n < NFM; n++) {
+      if (n < F_NUM + ns) {
+        double inner = -(dt / dx * (flx[fpx + n * fs] - flx[fm + n * fs]));
+        if (P.ndim > 1) inner = inner - dt / dy * (fly[fpy + n * fs] - fly[fm + n * fs]);
+        if (P.ndim == 3) inner = inner - dt / dz * (flz[fpz + n * fs] - flz[fm + n * fs]);
+        inner = inner + dt * (0.0 + Sg[n]);
+        B[n] = c1 * A[n] + c2 * B[n] + c3 * inner;
+      }
+    }
+    const double dens = fmax2(B[F_MASS], P.smlrho);
+    const double vx = B[F_XMOM] / dens, vy = B[F_YMOM] / dens, vz = B[F_ZMOM] / dens;
+    const double ener = fmax2(P.smalle, B[F_ENER] / dens);
+    const double ekin = .5 * (vx * vx + vy * vy + vz * vz);
+    const double eint = fmax2(P.smalle, ener - ekin);
+#pragma unroll
+    for (int v = 0; v < NSM; v++)
+      if (v < ns) us[(V_NPROP + v) * cs + c] = B[F_NUM + v] / dens;
+    EosOut e = eos_dens_ei(P, dens, vx, vy, vz, ener, eint);
+    us[V_DENS * cs + c] = dens;
+    us[V_VELX * cs + c] = vx;
+    us[V_VELY * cs + c] = vy;
+    us[V_VELZ * cs + c] = vz;
+    us[V_PRES * cs + c] = e.pres;
+    us[V_GAMC * cs + c] = P.gamma;
+    us[V_EINT * cs + c] = e.eint;
+    us[V_ENER * cs + c] = e.ener;
+    if (final_stage) {
+      us[V_TEMP * cs + c] = e.temp;
+      us[V_GAME * cs + c] = e.game;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// updateState (Hydro_funcs.F90-mc:573-574): interior of S -> U, all vars
+__global__ void __launch_bounds__(256) k_copy_interior(DevParams P, const BlockMeta* __restrict__ meta,
+                                                       const int* __restrict__ blist, const double* __restrict__ S,
+                                                       double* __restrict__ U, int nvar) {
+  const BlockMeta m = meta[blist[blockIdx.y]];
+  Lims L = stage_lims(P, m, 0);
+  const int ex = L.hi[0] - L.lo[0] + 1, ey = L.hi[1] - L.lo[1] + 1, ez = L.hi[2] - L.lo[2] + 1;
+  const int total = ex * ey * ez;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const int i = L.lo[0] + t % ex, j = L.lo[1] + (t / ex) % ey, k = L.lo[2] + t / (ex * ey);
+    const int64_t c = m.base + cix(m, i, j, k);
+    for (int v = 0; v < nvar; v++) U[c + v * m.cstride] = S[c + v * m.cstride];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// shockDetect (Hydro_funcs.F90-mc:600-706): pass 1 zeroes SHOK and stores
+// the sound speed over the guard-celled box, pass 2 flags shocks over
+// interior +/- (NGUARD-1).
+__global__ void __launch_bounds__(256) k_shock_vc(DevParams P, const BlockMeta* __restrict__ meta,
+                                                  const int* __restrict__ blist, double* __restrict__ U,
+                                                  double* __restrict__ vc) {
+  const BlockMeta m = meta[blist[blockIdx.y]];
+  const int total = m.NX * m.NY * m.NZ;
+  double* up = U + m.base;
+  const int64_t cs = m.cstride;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    up[V_SHOK * cs + t] = 0.;
+    vc[m.obase + t] = Num<double>::sqr(up[V_GAMC * cs + t] * up[V_PRES * cs + t] / fmax2(up[V_DENS * cs + t], P.tiny));
+  }
+}
+
+__global__ void __launch_bounds__(256) k_shock(DevParams P, const BlockMeta* __restrict__ meta,
+                                               const int* __restrict__ blist, double* __restrict__ U,
+                                               const double* __restrict__ vc) {
+  const BlockMeta m = meta[blist[blockIdx.y]];
+  Lims L = stage_lims(P, m, P.ng - 1);
+  const int ex = L.hi[0] - L.lo[0] + 1, ey = L.hi[1] - L.lo[1] + 1, ez = L.hi[2] - L.lo[2] + 1;
+  const int total = ex * ey * ez;
+  double* up = U + m.base;
+  const double* cp = vc + m.obase;
+  const int64_t cs = m.cstride;
+  const int ky = (P.ndim > 1), kz = (P.ndim == 3);
+  const int64_t sx = 1, sy = m.NX, sz = (int64_t)m.NX * m.NY;
+  const double beta = 0.5, delta = 0.1;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const int i = L.lo[0] + t % ex, j = L.lo[1] + (t / ex) % ey, k = L.lo[2] + t / (ex * ey);
+    const int64_t c = cix(m, i, j, k);
+    const double* p = up + V_PRES * cs;
+    double minP = 0., minC = 0.;
+    bool first = true;
+    for (int kk = -kz; kk <= kz; kk++)
+      for (int jj = -ky; jj <= ky; jj++)
+        for (int ii = -1; ii <= 1; ii++) {
+          const int64_t cc = cix(m, i + ii, j + jj, k + kk);
+          if (first) { minP = p[cc]; minC = cp[cc]; first = false; }
+          else { minP = fmin2(minP, p[cc]); minC = fmin2(minC, cp[cc]); }
+        }
+    double divv = up[V_VELX * cs + c + sx] - up[V_VELX * cs + c - sx];
+    if (P.ndim > 1) divv = divv + up[V_VELY * cs + c + sy] - up[V_VELY * cs + c - sy];
+    if (P.ndim == 3) divv = divv + up[V_VELZ * cs + c + sz] - up[V_VELZ * cs + c - sz];
+    divv = 0.5 * divv;
+    double gradPx = 0.5 * (p[c + sx] - p[c - sx]);
+    double gradPy = 0., gradPz = 0.;
+    if (P.ndim > 1) gradPy = 0.5 * (p[c + sy] - p[c - sy]);
+    if (P.ndim == 3) gradPz = 0.5 * (p[c + sz] - p[c - sz]);
+    const bool SW1 = (__builtin_fabs(gradPx) + __builtin_fabs(gradPy) + __builtin_fabs(gradPz) >= beta * minP);
+    const bool SW2 = (-delta * minC >= divv);
+    if (SW1 && SW2) up[V_SHOK * cs + c] = 1.;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// CFL dt (Hydro_computeDt.F90:77-148).  Pass 1: per (block, chunk) the first
+// cell in (k,j,i) order with the largest dt_ltemp*cfl.  Pass 2 (one
+// workgroup): per block dt = cfl/dt_temp, then the minimum over blocks with
+// the first block winning ties (Driver_computeDt's strict '<').
+struct DtPart { double key, lt; long long idx; };
+
+__global__ void __launch_bounds__(256) k_dt_part(DevParams P, const BlockMeta* __restrict__ meta,
+                                                 const int* __restrict__ blist, const double* __restrict__ U,
+                                                 DtPart* __restrict__ out, int chunks) {
+  __shared__ double skey[256], slt[256];
+  __shared__ long long sidx[256];
+  const int b = blockIdx.y;
+  const BlockMeta m = meta[blist[b]];
+  Lims L = stage_lims(P, m, 0);
+  const int ex = L.hi[0] - L.lo[0] + 1, ey = L.hi[1] - L.lo[1] + 1, ez = L.hi[2] - L.lo[2] + 1;
+  const int total = ex * ey * ez;
+  const double* up = U + m.base;
+  const int64_t cs = m.cstride;
+  const double dxi = 1.0 / m.dx, dyi = 1.0 / m.dy, dzi = 1.0 / m.dz;
+  double bk = 0., bl = 0.;
+  long long bi = 0x7fffffffffffffffLL;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += chunks * blockDim.x) {
+    const int i = L.lo[0] + t % ex, j = L.lo[1] + (t / ex) % ey, k = L.lo[2] + t / (ex * ey);
+    const int64_t c = cix(m, i, j, k);
+    const double snd2 = up[V_GAMC * cs + c] * up[V_PRES * cs + c] / up[V_DENS * cs + c];
+    const double cf = Num<double>::sqr(snd2);
+    double lt = (__builtin_fabs(up[V_VELX * cs + c]) + cf) * dxi;
+    if (P.ndim > 1) lt = fmax2(lt, (__builtin_fabs(up[V_VELY * cs + c]) + cf) * dyi);
+    if (P.ndim > 2) lt = fmax2(lt, (__builtin_fabs(up[V_VELZ * cs + c]) + cf) * dzi);
+    const double key = lt * P.cfl;
+    if (key > bk || (key == bk && (long long)c < bi)) { bk = key; bl = lt; bi = c; }
+  }
+  skey[threadIdx.x] = bk; slt[threadIdx.x] = bl; sidx[threadIdx.x] = bi;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      const int o = threadIdx.x + w;
+      if (skey[o] > skey[threadIdx.x] || (skey[o] == skey[threadIdx.x] && sidx[o] < sidx[threadIdx.x])) {
+        skey[threadIdx.x] = skey[o]; slt[threadIdx.x] = slt[o]; sidx[threadIdx.x] = sidx[o];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    DtPart r; r.key = skey[0]; r.lt = slt[0]; r.idx = sidx[0];
+    out[(size_t)blist[b] * chunks + blockIdx.x] = r;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_dt_final(DevParams P, const DtPart* __restrict__ parts, int nblocks,
+                                                  int chunks, double* __restrict__ dt_out,
+                                                  long long* __restrict__ loc_out, double* __restrict__ dt_hist,
+                                                  int hist_slot) {
+  __shared__ double sdt[256];
+  __shared__ int sb[256];
+  __shared__ long long sc[256];
+  double best = 1.7976931348623157e308;
+  int bb = 0x7fffffff;
+  long long bc = 0;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+    double bk = 0., bl = 0.;
+    long long bi = 0x7fffffffffffffffLL;
+    for (int q = 0; q < chunks; q++) {
+      const DtPart r = parts[(size_t)b * chunks + q];
+      if (r.key > bk || (r.key == bk && r.idx < bi)) { bk = r.key; bl = r.lt; bi = r.idx; }
+    }
+    const double dtb = (bl != 0.0) ? P.cfl / bl : 1.7976931348623157e308;
+    if (dtb < best || (dtb == best && b < bb)) { best = dtb; bb = b; bc = bi; }
+  }
+  sdt[threadIdx.x] = best; sb[threadIdx.x] = bb; sc[threadIdx.x] = bc;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      const int o = threadIdx.x + w;
+      if (sdt[o] < sdt[threadIdx.x] || (sdt[o] == sdt[threadIdx.x] && sb[o] < sb[threadIdx.x])) {
+        sdt[threadIdx.x] = sdt[o]; sb[threadIdx.x] = sb[o]; sc[threadIdx.x] = sc[o];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    dt_out[0] = sdt[0];
+    if (dt_hist) dt_hist[hist_slot] = sdt[0];
+    if (sb[0] < nblocks) {
+      loc_out[0] = sb[0];
+      loc_out[1] = sc[0];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Uniform block grid guard fill: same-level neighbour copy and OUTFLOW at
+// the domain faces (Grid_bcApplyToRegion.F90:526-532), i.e. every guard
+// cell takes the value at its global index clamped into the domain.
+// Blocks are numbered x-fastest over an nb[0] x nb[1] x nb[2] grid.
+__global__ void __launch_bounds__(256) k_ug_fill(DevParams P, const BlockMeta* __restrict__ meta, int nbx,
+                                                 int nby, int nbz, double* __restrict__ U, int nvar) {
+  const int bid = blockIdx.y;
+  const BlockMeta m = meta[bid];
+  const int total = m.NX * m.NY * m.NZ;
+  const int n[3] = {m.nx, (P.ndim > 1) ? m.ny : 1, (P.ndim > 2) ? m.nz : 1};
+  const int gg[3] = {P.ng, (P.ndim > 1) ? P.ng : 0, (P.ndim > 2) ? P.ng : 0};
+  const int nb[3] = {nbx, nby, nbz};
+  const int bi[3] = {bid % nbx, (bid / nbx) % nby, bid / (nbx * nby)};
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const int loc[3] = {t % m.NX, (t / m.NX) % m.NY, t / (m.NX * m.NY)};
+    bool inside = true;
+    int sb[3], sl[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      if (loc[d] < gg[d] || loc[d] >= gg[d] + n[d]) inside = false;
+      int g = bi[d] * n[d] + (loc[d] - gg[d]);
+      const int gmax = nb[d] * n[d] - 1;
+      g = g < 0 ? 0 : (g > gmax ? gmax : g);
+      sb[d] = g / n[d];
+      sl[d] = g % n[d] + gg[d];
+    }
+    if (inside) continue;
+    const BlockMeta ms = meta[sb[0] + nbx * (sb[1] + nby * sb[2])];
+    const int64_t src = ms.base + cix(ms, sl[0], sl[1], sl[2]);
+    const int64_t dst = m.base + t;
+    for (int v = 0; v < nvar; v++) U[dst + v * m.cstride] = U[src + v * ms.cstride];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// flux-register accumulation (addFluxes, Hydro_funcs.F90-mc:481-507):
+// buffers hold the interior faces of each block, dims (n+1 along dir).
+__global__ void __launch_bounds__(256) k_add_fluxes(DevParams P, const BlockMeta* __restrict__ meta,
+                                                    const int* __restrict__ blist, const double* __restrict__ fl,
+                                                    double* __restrict__ fb, int dir, double weight, int add) {
+  const BlockMeta m = meta[blist[blockIdx.y]];
+  Lims L = stage_lims(P, m, 0);
+  int e[3] = {L.hi[0] - L.lo[0] + 1, L.hi[1] - L.lo[1] + 1, L.hi[2] - L.lo[2] + 1};
+  e[dir] += 1;
+  const int total = e[0] * e[1] * e[2];
+  const int nf = F_NUM + P.ns;
+  const int64_t bstride = (int64_t)total;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const int i = t % e[0], j = (t / e[0]) % e[1], k = t / (e[0] * e[1]);
+    const int64_t f = m.fbase + fix(m, L.lo[0] + i, L.lo[1] + j, L.lo[2] + k);
+    const int64_t o = m.bbase + t;
+    for (int n = 0; n < nf; n++) {
+      const double w = weight * fl[f + n * m.fstride];
+      fb[o + n * bstride] = add ? fb[o + n * bstride] + w : w;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// flux correction on the boundary layer of one direction
+// (hy_rk_correctFluxes.F90:126-338): correctZone + per-cell EOS.
+__global__ void __launch_bounds__(256) k_correct(DevParams P, const BlockMeta* __restrict__ meta,
+                                                 const int* __restrict__ blist, double* __restrict__ U,
+                                                 const double* __restrict__ fb, int dir,
+                                                 const double* __restrict__ dtp) {
+  const BlockMeta m = meta[blist[blockIdx.y]];
+  Lims L = stage_lims(P, m, 0);
+  const int n[3] = {L.hi[0] - L.lo[0] + 1, L.hi[1] - L.lo[1] + 1, L.hi[2] - L.lo[2] + 1};
+  int e[3] = {n[0], n[1], n[2]};
+  e[dir] += 1;
+  const int64_t bstride = (int64_t)e[0] * e[1] * e[2];
+  const int d2 = (dir == 0) ? 1 : 0, d3 = (dir == 2) ? 1 : 2;
+  const int total = n[d2] * n[d3];
+  const int nf = F_NUM + P.ns, ns = P.ns;
+  const double dt = *dtp;
+  const double del = (dir == 0) ? m.dx : (dir == 1) ? m.dy : m.dz;
+  const int64_t cs = m.cstride;
+  double* up = U + m.base;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const int a2 = t % n[d2], a3 = t / n[d2];
+    for (int side = 0; side < 2; side++) {
+      int c[3];
+      c[d2] = a2; c[d3] = a3;
+      c[dir] = side ? n[dir] - 1 : 0;
+      int f[3] = {c[0], c[1], c[2]};
+      if (side) f[dir] += 1;
+      const int64_t bo = m.bbase + (int64_t)f[0] + (int64_t)e[0] * ((int64_t)f[1] + (int64_t)e[1] * f[2]);
+      const int64_t cc = cix(m, L.lo[0] + c[0], L.lo[1] + c[1], L.lo[2] + c[2]);
+      double Us[F_NUM + 8];
+      const double d0 = up[V_DENS * cs + cc];
+      Us[F_MASS] = d0;
+      Us[F_XMOM] = d0 * up[V_VELX * cs + cc];
+      Us[F_YMOM] = d0 * up[V_VELY * cs + cc];
+      Us[F_ZMOM] = d0 * up[V_VELZ * cs + cc];
+      Us[F_ENER] = d0 * up[V_ENER * cs + cc];
+      for (int v = 0; v < ns; v++) Us[F_NUM + v] = up[(V_NPROP + v) * cs + cc] * d0;
+      const double fac = 1.0;
+      for (int q = 0; q < nf; q++) {
+        const double b = fb[bo + q * bstride];
+        const double dF = side ? b : -b;
+        Us[q] = Us[q] - dt / del * (fac * dF);
+      }
+      const double dens = fmax2(Us[F_MASS], P.smlrho);
+      const double vx = Us[F_XMOM] / dens, vy = Us[F_YMOM] / dens, vz = Us[F_ZMOM] / dens;
+      const double ener = fmax2(P.smalle, Us[F_ENER] / dens);
+      const double ekin = .5 * (vx * vx + vy * vy + vz * vz);
+      const double eint = fmax2(P.smalle, ener - ekin);
+      for (int v = 0; v < ns; v++) up[(V_NPROP + v) * cs + cc] = Us[F_NUM + v] / dens;
+      EosOut o = eos_dens_ei(P, dens, vx, vy, vz, ener, eint);
+      up[V_DENS * cs + cc] = dens;
+      up[V_VELX * cs + cc] = vx;
+      up[V_VELY * cs + cc] = vy;
+      up[V_VELZ * cs + cc] = vz;
+      up[V_PRES * cs + cc] = o.pres;
+      up[V_GAMC * cs + cc] = P.gamma;
+      up[V_EINT * cs + cc] = o.eint;
+      up[V_ENER * cs + cc] = o.ener;
+      up[V_TEMP * cs + cc] = o.temp;
+      up[V_GAME * cs + cc] = o.game;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// AoS (unk: variable fastest, host-packed into a contiguous staging buffer)
+// <-> SoA transposes.  var_index[v] gives the unk slot of canonical var v.
+__global__ void __launch_bounds__(256) k_aos_to_soa(const BlockMeta* __restrict__ meta, const double* __restrict__ aos,
+                                                    const int64_t* __restrict__ aos_off, int nunk,
+                                                    const int* __restrict__ var_index, int nvar,
+                                                    double* __restrict__ U) {
+  const BlockMeta m = meta[blockIdx.y];
+  const int total = m.NX * m.NY * m.NZ;
+  const double* src = aos + aos_off[blockIdx.y];
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x)
+    for (int v = 0; v < nvar; v++) U[m.base + v * m.cstride + t] = src[(int64_t)t * nunk + var_index[v]];
+}
+
+__global__ void __launch_bounds__(256) k_soa_to_aos(DevParams P, const BlockMeta* __restrict__ meta,
+                                                    const double* __restrict__ U, double* __restrict__ aos,
+                                                    const int64_t* __restrict__ aos_off, int nunk,
+                                                    const int* __restrict__ var_index, int nvar, int interior) {
+  const BlockMeta m = meta[blockIdx.y];
+  const int total = m.NX * m.NY * m.NZ;
+  double* dst = aos + aos_off[blockIdx.y];
+  Lims L = stage_lims(P, m, 0);
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    if (interior) {
+      const int i = t % m.NX, j = (t / m.NX) % m.NY, k = t / (m.NX * m.NY);
+      if (i < L.lo[0] || i > L.hi[0] || j < L.lo[1] || j > L.hi[1] || k < L.lo[2] || k > L.hi[2]) continue;
+    }
+    for (int v = 0; v < nvar; v++) dst[(int64_t)t * nunk + var_index[v]] = U[m.base + v * m.cstride + t];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers
+static inline int grid_x_for(int total, int cap) {
+  int g = (total + 255) / 256;
+  return g < 1 ? 1 : (g > cap ? cap : g);
+}
+
+template <int DIR, int RECON, int RIEMANN, int NSM>
+static hipError_t launch_flux_t(const PlainLaunch& L, int ngoff) {
+  // geometry of this stage (all blocks in the list share one shape)
+  const int n[3] = {L.nx, L.ny, L.nz};
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; d++) {
+    if (d < L.P.ndim) { lo[d] = L.P.ng - ngoff; hi[d] = L.P.ng + n[d] - 1 + ngoff; }
+    else { lo[d] = 0; hi[d] = 0; }
+  }
+  const int T1 = (DIR == 0) ? 1 : 0, T2 = (DIR == 2) ? 1 : 2;
+  const int Lr = hi[DIR] - lo[DIR] + 3;
+  const int nt1 = hi[T1] - lo[T1] + 1, nt2 = hi[T2] - lo[T2] + 1;
+  int lanes = 256 / Lr;
+  if (DIR != 0 && lanes < 16) lanes = 16;
+  if (lanes < 1) lanes = 1;
+  if (lanes * Lr > 1024) lanes = 1024 / Lr;
+  if (lanes > nt1) lanes = nt1;
+  constexpr int NRM = R_NUM + NSM;
+  {
+    // keep the LDS exchange buffer within 64 KiB per workgroup
+    const int cap = 65536 / (int)(sizeof(double) * (NRM + 6) * Lr);
+    if (lanes > cap) lanes = cap;
+    if (lanes < 1) lanes = 1;
+  }
+  const int chunks = (nt1 + lanes - 1) / lanes;
+  dim3 grid(chunks * nt2, L.nblk);
+  dim3 block(lanes * Lr);
+  const size_t shm = sizeof(double) * (size_t)(NRM + 6) * block.x;
+  hipLaunchKernelGGL((k_flux<DIR, RECON, RIEMANN, NSM>), grid, block, shm, L.stream, L.P, L.meta, L.blist, L.S,
+                     L.flat, L.fl[DIR], ngoff, lanes, L.err);
+  return hipGetLastError();
+}
+
+template <int DIR, int NSM>
+static hipError_t launch_flux_r(const PlainLaunch& L, int ngoff) {
+  const int r = L.P.recon, q = L.P.riemann;
+  if (q == RIEMANN_HLL) {
+    switch (r) {
+      case RECON_MP5: return launch_flux_t<DIR, RECON_MP5, RIEMANN_HLL, NSM>(L, ngoff);
+      case RECON_PLM: return launch_flux_t<DIR, RECON_PLM, RIEMANN_HLL, NSM>(L, ngoff);
+      case RECON_FOG: return launch_flux_t<DIR, RECON_FOG, RIEMANN_HLL, NSM>(L, ngoff);
+      default: return launch_flux_t<DIR, RECON_WENO5, RIEMANN_HLL, NSM>(L, ngoff);
+    }
+  }
+  switch (r) {
+    case RECON_MP5: return launch_flux_t<DIR, RECON_MP5, RIEMANN_HLLC, NSM>(L, ngoff);
+    case RECON_PLM: return launch_flux_t<DIR, RECON_PLM, RIEMANN_HLLC, NSM>(L, ngoff);
+    case RECON_FOG: return launch_flux_t<DIR, RECON_FOG, RIEMANN_HLLC, NSM>(L, ngoff);
+    default: return launch_flux_t<DIR, RECON_WENO5, RIEMANN_HLLC, NSM>(L, ngoff);
+  }
+}
+
+hipError_t plain_stage(const PlainLaunch& L, int ngoff, double c1, double c2, double c3, int final_stage) {
+  hipError_t e;
+  const int n[3] = {L.nx, L.ny, L.nz};
+  int box = 1, boxf = 1;
+  for (int d = 0; d < 3; d++) {
+    if (d < L.P.ndim) { box *= n[d] + 2 * ngoff; boxf *= n[d] + 2 * ngoff + 2; }
+  }
+  if (L.P.use_flat && L.P.recon == RECON_WENO5) {
+    hipLaunchKernelGGL(k_flatten, dim3(grid_x_for(boxf, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta, L.blist,
+                       L.S, L.flat, ngoff);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  for (int dir = 0; dir < L.P.ndim; dir++) {
+    if (L.P.ns == 0) {
+      e = (dir == 0) ? launch_flux_r<0, 0>(L, ngoff) : (dir == 1) ? launch_flux_r<1, 0>(L, ngoff)
+                                                                : launch_flux_r<2, 0>(L, ngoff);
+    } else {
+      e = (dir == 0) ? launch_flux_r<0, 8>(L, ngoff) : (dir == 1) ? launch_flux_r<1, 8>(L, ngoff)
+                                                                : launch_flux_r<2, 8>(L, ngoff);
+    }
+    if (e != hipSuccess) return e;
+  }
+  if (L.fb[0] != nullptr) {
+    for (int dir = 0; dir < L.P.ndim; dir++) {
+      hipLaunchKernelGGL(k_add_fluxes, dim3(grid_x_for(box * 2, 256), L.nblk), dim3(256), 0, L.stream, L.P, L.meta,
+                         L.blist, L.fl[dir], L.fb[dir], dir, L.weight, L.add_flux);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+  }
+  if (L.P.ns == 0)
+    hipLaunchKernelGGL(k_update<0>, dim3(grid_x_for(box, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta, L.blist,
+                       L.U0, L.S, L.fl[0], L.fl[1], L.fl[2], L.dtp, c1, c2, c3, ngoff, final_stage);
+  else
+    hipLaunchKernelGGL(k_update<8>, dim3(grid_x_for(box, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta, L.blist,
+                       L.U0, L.S, L.fl[0], L.fl[1], L.fl[2], L.dtp, c1, c2, c3, ngoff, final_stage);
+  return hipGetLastError();
+}
+
+hipError_t plain_copy_interior(const PlainLaunch& L, const double* S, double* U, int nvar) {
+  int box = L.nx * (L.P.ndim > 1 ? L.ny : 1) * (L.P.ndim > 2 ? L.nz : 1);
+  hipLaunchKernelGGL(k_copy_interior, dim3(grid_x_for(box, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta,
+                     L.blist, S, U, nvar);
+  return hipGetLastError();
+}
+
+hipError_t plain_shock_detect(const PlainLaunch& L, double* U, double* scratch) {
+  const int NX = L.nx + 2 * L.P.ng, NY = L.P.ndim > 1 ? L.ny + 2 * L.P.ng : 1,
+            NZ = L.P.ndim > 2 ? L.nz + 2 * L.P.ng : 1;
+  hipLaunchKernelGGL(k_shock_vc, dim3(grid_x_for(NX * NY * NZ, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta,
+                     L.blist, U, scratch);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_shock, dim3(grid_x_for(NX * NY * NZ, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta,
+                     L.blist, U, scratch);
+  return hipGetLastError();
+}
+
+hipError_t plain_dt_part(const PlainLaunch& L, const double* U, void* parts, int chunks) {
+  hipLaunchKernelGGL(k_dt_part, dim3(chunks, L.nblk), dim3(256), 0, L.stream, L.P, L.meta, L.blist, U,
+                     (DtPart*)parts, chunks);
+  return hipGetLastError();
+}
+
+hipError_t plain_dt_final(hipStream_t st, const DevParams& P, const void* parts, int nblk_total, int chunks,
+                          double* dt_out, long long* loc_out, double* dt_hist, int hist_slot) {
+  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, st, P, (const DtPart*)parts, nblk_total, chunks, dt_out,
+                     loc_out, dt_hist, hist_slot);
+  return hipGetLastError();
+}
+
+__global__ void k_set_scalar(double* p, double v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = v;
+}
+
+hipError_t plain_set_scalar(hipStream_t st, double* p, double v) {
+  hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(64), 0, st, p, v);
+  return hipGetLastError();
+}
+
+size_t plain_dt_part_size() { return sizeof(DtPart); }
+
+hipError_t plain_ug_fill(const PlainLaunch& L, int nbx, int nby, int nbz, double* U, int nvar) {
+  const int NX = L.nx + 2 * L.P.ng, NY = L.P.ndim > 1 ? L.ny + 2 * L.P.ng : 1,
+            NZ = L.P.ndim > 2 ? L.nz + 2 * L.P.ng : 1;
+  hipLaunchKernelGGL(k_ug_fill, dim3(grid_x_for(NX * NY * NZ, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta,
+                     nbx, nby, nbz, U, nvar);
+  return hipGetLastError();
+}
+
+hipError_t plain_correct(const PlainLaunch& L, double* U) {
+  for (int dir = 0; dir < L.P.ndim; dir++) {
+    const int n[3] = {L.nx, L.P.ndim > 1 ? L.ny : 1, L.P.ndim > 2 ? L.nz : 1};
+    const int d2 = (dir == 0) ? 1 : 0, d3 = (dir == 2) ? 1 : 2;
+    hipLaunchKernelGGL(k_correct, dim3(grid_x_for(n[d2] * n[d3], 64), L.nblk), dim3(256), 0, L.stream, L.P, L.meta,
+                       L.blist, U, L.fb[dir], dir, L.dtp);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t plain_aos_to_soa(hipStream_t st, const BlockMeta* meta, int nblk, int maxcells, const double* aos,
+                            const int64_t* aos_off, int nunk, const int* var_index, int nvar, double* U) {
+  hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_x_for(maxcells, 512), nblk), dim3(256), 0, st, meta, aos, aos_off, nunk,
+                     var_index, nvar, U);
+  return hipGetLastError();
+}
+
+hipError_t plain_soa_to_aos(hipStream_t st, const DevParams& P, const BlockMeta* meta, int nblk, int maxcells,
+                            const double* U, double* aos, const int64_t* aos_off, int nunk, const int* var_index,
+                            int nvar, int interior) {
+  hipLaunchKernelGGL(k_soa_to_aos, dim3(grid_x_for(maxcells, 512), nblk), dim3(256), 0, st, P, meta, U, aos, aos_off,
+                     nunk, var_index, nvar, interior);
+  return hipGetLastError();
+}
+
+}  // namespace spark

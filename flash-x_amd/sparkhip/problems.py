@@ -1,0 +1,211 @@
+"""Synthetic block sets for the benchmark configurations.
+
+Each generator returns a Problem: the run configuration (spark_params
+fields), the block descriptors and one numpy array per block holding
+unk(var, i, j, k) over the guard-celled box (C shape (NZ, NY, NX, nunk),
+so the variable index is fastest, as in the reference unk layout).
+
+  sod2d      2D Sod tube, interface x + y = 1, 16x16 blocks of 16^2
+  sedov3d    3D Sedov blast, 8^3 blocks of 16^3, E = 1 in r < 3.5 cells
+  batched    4096 independent 3D blocks of 8^3/16^3/32^3 (2:5:1), levels
+             0..4, smooth random Fourier fields plus one planar shock per
+             16 blocks, fields evaluated in every cell including guards
+  rayleigh_taylor   3-level 16^3 block hierarchy, rho 2 over 1, g = -1,
+             8 mass scalars with tanh profiles
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .capi import VI, SPARK_NPROP
+
+GAS_CONST = 8.3144598e7
+
+
+@dataclass
+class Problem:
+    name: str
+    cfg: dict
+    blocks: list
+    arrays: list
+    ug: tuple | None = None          # (nbx, nby, nbz) for uniform block grids
+    info: dict = field(default_factory=dict)
+
+    @property
+    def zones(self) -> int:
+        nd = self.cfg["ndim"]
+        return int(sum(b["nx"] * (b["ny"] if nd > 1 else 1) * (b["nz"] if nd > 2 else 1) for b in self.blocks))
+
+
+def gc_shape(cfg, b):
+    g = cfg["nguard"]
+    nd = cfg["ndim"]
+    NX = b["nx"] + 2 * g
+    NY = b["ny"] + 2 * g if nd > 1 else 1
+    NZ = b["nz"] + 2 * g if nd > 2 else 1
+    return NZ, NY, NX
+
+
+def base_cfg(**kw):
+    cfg = dict(gamma=1.4, cfl=0.8, cvisc=0.1, smlrho=1e-10, smallp=1e-10, smalle=1e-10, smallx=1e-10,
+               eintSwitch=0.0, tiny=1e-32, gasConst=GAS_CONST, abar=1.0, grav=(0.0, 0.0, 0.0), ndim=3,
+               nguard=10, nstencil=3, rk_order=3, recon="weno5", riemann="hllc", use_flattening=1,
+               use_hybrid_riemann=0, nscalars=0, use_gravity=0, flux_correct=0, form="plain_fp64",
+               guard_mode="telescope")
+    cfg.update(kw)
+    return cfg
+
+
+def fill_thermo(U, dens, pres, vx, vy, vz, gamma, abar=1.0, scalars=None):
+    """Primitive fields -> a consistent Gamma-law unk state."""
+    eint = pres / ((gamma - 1.0) * dens)
+    U[..., VI["dens"]] = dens
+    U[..., VI["pres"]] = pres
+    U[..., VI["velx"]] = vx
+    U[..., VI["vely"]] = vy
+    U[..., VI["velz"]] = vz
+    U[..., VI["eint"]] = eint
+    U[..., VI["ener"]] = eint + 0.5 * (vx * vx + vy * vy + vz * vz)
+    U[..., VI["gamc"]] = gamma
+    U[..., VI["game"]] = gamma
+    U[..., VI["temp"]] = eint * (gamma - 1.0) / GAS_CONST * abar
+    U[..., VI["shok"]] = 0.0
+    if scalars is not None:
+        for s, X in enumerate(scalars):
+            U[..., SPARK_NPROP + s] = X
+
+
+def _centers(cfg, b, origin, NZ, NY, NX):
+    g = cfg["nguard"]
+    nd = cfg["ndim"]
+    x = origin[0] + (np.arange(NX) - g + 0.5) * b["dx"]
+    y = origin[1] + (np.arange(NY) - (g if nd > 1 else 0) + 0.5) * b["dy"]
+    z = origin[2] + (np.arange(NZ) - (g if nd > 2 else 0) + 0.5) * b["dz"]
+    return np.meshgrid(z, y, x, indexing="ij")  # Z, Y, X each (NZ, NY, NX)
+
+
+def sod2d(nbx=16, nby=16, n=16, nguard=10, gamma=1.4, **kw) -> Problem:
+    """2D Sod problem with the interface on x + y = 1 (cells straddling it
+    take the average of both states, as Sod's Simulation_initBlock does)."""
+    cfg = base_cfg(ndim=2, nguard=nguard, gamma=gamma, **kw)
+    dx = 1.0 / (nbx * n)
+    dy = 1.0 / (nby * n)
+    blocks, arrays = [], []
+    nunk = SPARK_NPROP + cfg["nscalars"]
+    for bj in range(nby):
+        for bi in range(nbx):
+            b = dict(nx=n, ny=n, nz=1, level=1, dx=dx, dy=dy, dz=1.0)
+            NZ, NY, NX = gc_shape(cfg, b)
+            Z, Y, X = _centers(cfg, b, (bi * n * dx, bj * n * dy, 0.0), NZ, NY, NX)
+            xl, xr = X - 0.5 * dx, X + 0.5 * dx
+            lpos = 1.0 - Y
+            left = xr <= lpos
+            mid = (xl < lpos) & (xr > lpos)
+            dens = np.where(left, 1.0, np.where(mid, 0.5 * (1.0 + 0.125), 0.125))
+            pres = np.where(left, 1.0, np.where(mid, 0.5 * (1.0 + 0.1), 0.1))
+            U = np.zeros((NZ, NY, NX, nunk))
+            zero = np.zeros_like(dens)
+            fill_thermo(U, dens, pres, zero, zero, zero, gamma)
+            blocks.append(b)
+            arrays.append(U)
+    return Problem("sod2d", cfg, blocks, arrays, ug=(nbx, nby, 1))
+
+
+def sedov3d(nb=8, n=16, nguard=6, gamma=1.4, E=1.0, r_cells=3.5, p_amb=1e-5, rho=1.0, **kw) -> Problem:
+    """3D Sedov blast on [0,1]^3: E deposited as pressure in the cells whose
+    centres lie within r_cells cell widths of the domain centre (p scaled so
+    the deposited energy is exactly E)."""
+    cfg = base_cfg(ndim=3, nguard=nguard, gamma=gamma, **kw)
+    N = nb * n
+    dx = 1.0 / N
+    r0 = r_cells * dx
+    idx = (np.arange(N) + 0.5) * dx - 0.5
+    r2 = idx[:, None, None] ** 2 + idx[None, :, None] ** 2 + idx[None, None, :] ** 2
+    n_in = int(np.count_nonzero(r2 < r0 * r0))
+    p_in = (gamma - 1.0) * E / (n_in * dx ** 3)
+    blocks, arrays = [], []
+    nunk = SPARK_NPROP + cfg["nscalars"]
+    for bk in range(nb):
+        for bj in range(nb):
+            for bi in range(nb):
+                b = dict(nx=n, ny=n, nz=n, level=1, dx=dx, dy=dx, dz=dx)
+                NZ, NY, NX = gc_shape(cfg, b)
+                Z, Y, X = _centers(cfg, b, (bi * n * dx, bj * n * dx, bk * n * dx), NZ, NY, NX)
+                rr = (X - 0.5) ** 2 + (Y - 0.5) ** 2 + (Z - 0.5) ** 2
+                pres = np.where(rr < r0 * r0, p_in, p_amb)
+                dens = np.full_like(pres, rho)
+                U = np.zeros((NZ, NY, NX, nunk))
+                zero = np.zeros_like(pres)
+                fill_thermo(U, dens, pres, zero, zero, zero, gamma)
+                blocks.append(b)
+                arrays.append(U)
+    return Problem("sedov3d", cfg, blocks, arrays, ug=(nb, nb, nb), info=dict(p_in=p_in, n_in=n_in, E=E))
+
+
+class _Fourier:
+    """Smooth random field s(x) in [0, 1]: 0.5 + 0.5 sum a_k cos(2 pi k.x + phi_k) / sum |a_k|."""
+
+    def __init__(self, rng, nmodes=6, kmax=3):
+        self.k = rng.integers(-kmax, kmax + 1, size=(nmodes, 3)).astype(np.float64)
+        self.k[np.all(self.k == 0, axis=1)] = 1.0
+        self.a = rng.normal(size=nmodes)
+        self.phi = rng.uniform(0, 2 * np.pi, size=nmodes)
+        self.norm = np.sum(np.abs(self.a))
+
+    def __call__(self, X, Y, Z):
+        s = np.zeros_like(X)
+        for k, a, ph in zip(self.k, self.a, self.phi):
+            s += a * np.cos(2 * np.pi * (k[0] * X + k[1] * Y + k[2] * Z) + ph)
+        return 0.5 + 0.5 * s / self.norm
+
+
+def batched(nblocks=4096, nguard=10, gamma=1.4, seed_geom=20261202, seed_field=20261203, sizes=(8, 16, 32),
+            probs=(0.25, 0.625, 0.125), levels=(0, 4), shock_every=16, nscalars=0, **kw) -> Problem:
+    """Independent 3D blocks of uneven size and level (frozen guards)."""
+    cfg = base_cfg(ndim=3, nguard=nguard, gamma=gamma, nscalars=nscalars, **kw)
+    rg = np.random.Generator(np.random.PCG64(seed_geom))
+    rf = np.random.Generator(np.random.PCG64(seed_field))
+    size_idx = rg.choice(len(sizes), size=nblocks, p=np.asarray(probs) / np.sum(probs))
+    lev = rg.integers(levels[0], levels[1] + 1, size=nblocks)
+    origins = rg.uniform(0.0, 1.0, size=(nblocks, 3))
+    f_rho, f_p = _Fourier(rf), _Fourier(rf)
+    f_u = [_Fourier(rf) for _ in range(3)]
+    u_scale = 0.3 / np.sqrt(1.0 / 8.0)  # cos-sum std ~ 1/sqrt(8) after normalisation
+    blocks, arrays = [], []
+    nunk = SPARK_NPROP + nscalars
+    nshock = 0
+    for i in range(nblocks):
+        n = int(sizes[size_idx[i]])
+        dx = 2.0 ** (-int(lev[i])) / 16.0
+        b = dict(nx=n, ny=n, nz=n, level=int(lev[i]) + 1, dx=dx, dy=dx, dz=dx)
+        NZ, NY, NX = gc_shape(cfg, b)
+        Z, Y, X = _centers(cfg, b, origins[i], NZ, NY, NX)
+        s = f_rho(X, Y, Z)
+        dens = 0.5 * 4.0 ** s
+        pres = 0.5 + 1.5 * f_p(X, Y, Z)
+        vel = [u_scale * (f(X, Y, Z) - 0.5) for f in f_u]
+        if shock_every and i % shock_every == 0:
+            nshock += 1
+            nv = rf.normal(size=3)
+            nv /= np.linalg.norm(nv)
+            M = rf.uniform(2.0, 5.0)
+            c = origins[i] + (0.5 + rf.uniform(-0.25, 0.25)) * n * dx
+            post = (nv[0] * (X - c[0]) + nv[1] * (Y - c[1]) + nv[2] * (Z - c[2])) < 0.0
+            cs = np.sqrt(gamma * pres / dens)
+            r21 = (gamma + 1) * M * M / ((gamma - 1) * M * M + 2)
+            p21 = (2 * gamma * M * M - (gamma - 1)) / (gamma + 1)
+            up = M * cs * (1.0 - 1.0 / r21)
+            vel = [np.where(post, v + nv[d] * up, v) for d, v in enumerate(vel)]
+            dens = np.where(post, dens * r21, dens)
+            pres = np.where(post, pres * p21, pres)
+        scal = None
+        if nscalars:
+            scal = [0.5 * (1 + np.tanh((X - 0.5) / (0.05 * (q + 1)))) for q in range(nscalars)]
+        U = np.empty((NZ, NY, NX, nunk))
+        fill_thermo(U, dens, pres, vel[0], vel[1], vel[2], gamma, scalars=scal)
+        blocks.append(b)
+        arrays.append(U)
+    return Problem("batched", cfg, blocks, arrays, info=dict(seeds=(seed_geom, seed_field), nshock=nshock,
+                                                             size_counts=np.bincount(size_idx, minlength=len(sizes))))

@@ -1,0 +1,170 @@
+/*
+ * spark_hip.h -- C ABI of the MI355X-native Spark unsplit-hydro RK stage.
+ *
+ * This is the drop-in boundary that a new Spark variant's Fortran Hydro(),
+ * Hydro_computeDt() and Hydro_init() call through ISO_C_BINDING
+ * (flash-x_amd/fortran/spark_hip_mod.F90 holds the bind(C) interfaces).
+ * All entry points are extern "C", take plain pointers and sizes, return
+ * an int status (SPARK_OK = 0) and never abort: a Fortran caller maps a
+ * non-zero status to Driver_abort(spark_last_error text), which is how the
+ * reference reports failures (Driver/Driver_abort.F90).
+ *
+ * Paths below are relative to Flash-X/source/.
+ *
+ * Data model.  The host hands over LEAF blocks as the reference tile
+ * iterator exposes them: one pointer per block to unk(NUNK_VARS, i, j, k)
+ * over the guard-celled box, variable index fastest
+ * (Grid/GridMain/AMR/Paramesh4/Grid_tile.F90:157-267, getDataPtr).  The
+ * library keeps its own device copy in structure-of-arrays form for the
+ * whole batch, from spark_pack() to spark_unpack().
+ */
+#ifndef SPARK_HIP_H
+#define SPARK_HIP_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPARK_OK 0
+#define SPARK_ERR_ARG 1         /* invalid argument or unsupported option */
+#define SPARK_ERR_HIP 2         /* HIP runtime error */
+#define SPARK_ERR_STATE 3       /* call out of sequence */
+#define SPARK_ERR_NUMERIC 4     /* negative a^2 faces were met (Riemann ierr) */
+
+#define SPARK_MAX_SCALARS 8
+#define SPARK_NPROP 11          /* dens eint ener gamc game pres shok temp velx vely velz */
+
+enum spark_recon { SPARK_RECON_WENO5 = 0, SPARK_RECON_MP5 = 1, SPARK_RECON_PLM = 2, SPARK_RECON_FOG = 3 };
+enum spark_riemann { SPARK_RIEMANN_HLLC = 0, SPARK_RIEMANN_HLL = 1 };
+/* plain FP64 form, or FP32 reconstruction/Riemann with FP64 accumulation */
+enum spark_form { SPARK_FORM_PLAIN_FP64 = 0, SPARK_FORM_FP32_FLUX = 1 };
+/* guard policy: telescoping stages over guard cells filled once per step
+ * (the reference's own scheme, Spark/Hydro.ini:19,37), or refilling the
+ * guards of a uniform block grid on the device between stages */
+enum spark_guard_mode { SPARK_GUARD_TELESCOPE = 0, SPARK_GUARD_REFILL = 1 };
+
+/* Runtime parameters (Hydro_init.F90 / Eos_init.F90 subset).
+ * Replaces the RuntimeParameters_get calls of Spark/Hydro_init.F90:60-104
+ * and Eos/EosMain/Gamma/Eos_init.F90:55-86. */
+typedef struct spark_params {
+  double gamma;        /* Eos/EosMain/Config:43                             */
+  double cfl;          /* Hydro/HydroMain/Config:26                         */
+  double cvisc;        /* Spark/Config:17                                   */
+  double smlrho;       /* smallest density                                  */
+  double smallp;       /* smallest pressure                                 */
+  double smalle;       /* smallest specific energy                          */
+  double smallx;       /* smallest mass fraction                            */
+  double eintSwitch;   /* Eos/EosMain/Config:39                             */
+  double tiny;         /* hy_tiny, Spark/Hydro_data.F90:85                  */
+  double gasConst;     /* ideal gas constant, PhysicalConstants_init.F90:186 */
+  double abar;         /* eos_singleSpeciesA                                */
+  double grav[3];      /* constant gravitational acceleration (source term) */
+  int ndim;            /* 1, 2 or 3                                         */
+  int nguard;          /* NGUARD                                            */
+  int nstencil;        /* NSTENCIL (3 for WENO5/MP5, 2 for PLM/FOG)         */
+  int rk_order;        /* 2 or 3 (Spark/Hydro.ini:2-48)                     */
+  int recon;           /* enum spark_recon                                  */
+  int riemann;         /* enum spark_riemann                                */
+  int use_flattening;  /* Spark/Config:42                                   */
+  int use_hybrid_riemann; /* Spark/Config:43                                */
+  int nscalars;        /* NMASS_SCALARS, 0..SPARK_MAX_SCALARS               */
+  int use_gravity;     /* apply grav[] as a source in the update            */
+  int flux_correct;    /* accumulate flux registers (Spark/Hydro.F90-mc:160) */
+  int form;            /* enum spark_form                                   */
+  int guard_mode;      /* enum spark_guard_mode                             */
+  int nunk_vars;       /* NUNK_VARS of the host unk array                   */
+  /* 0-based unk slot of dens eint ener gamc game pres shok temp velx vely
+   * velz, then the mass scalars (the *_VAR map of Simulation.h minus 1) */
+  int var_index[SPARK_NPROP + SPARK_MAX_SCALARS];
+} spark_params;
+
+/* One LEAF block (tileDesc%limits, %deltas, %level). */
+typedef struct spark_block_desc {
+  int nx, ny, nz;      /* interior cells (1 for unused dimensions) */
+  int level;           /* refinement level (1 = coarsest)          */
+  double dx, dy, dz;   /* cell widths                              */
+} spark_block_desc;
+
+typedef struct spark_batch spark_batch;
+
+/* Reference defaults (Spark/Config, Eos/EosMain/Config, Grid Configs). */
+int spark_params_default(spark_params* p);
+
+/* Select the device and create the library's stream.
+ * Replaces: the device set-up implicit in Spark/Hydro_init.F90:24-179. */
+int spark_init(int device);
+int spark_finalize(void);
+
+/* Device storage for a set of LEAF blocks, alive until destroy.
+ * Replaces: allocate_scr/deallocate_scr, Spark/Hydro_funcs.F90-mc:337-422
+ * (the reference allocates scratch per block; here once per batch). */
+int spark_batch_create(const spark_params* p, int nblocks, const spark_block_desc* blocks, spark_batch** out);
+int spark_batch_destroy(spark_batch* b);
+
+/* Host unk blocks (guard-celled box, nunk_vars fastest) -> device.
+ * Replaces: tileDesc%getDataPtr + saveState's read of Uin,
+ * Grid_tile.F90:157-267 and Spark/Hydro_funcs.F90-mc:512-541. */
+int spark_pack(spark_batch* b, const double* const* unk_blocks);
+/* Device -> host unk blocks; interior_only mirrors updateState
+ * (Spark/Hydro_funcs.F90-mc:573-574).  which = 0: solution, 1: the stage
+ * state of the last stage run (hy_starState). */
+int spark_unpack(spark_batch* b, double* const* unk_blocks, int interior_only, int which);
+
+/* Declare the batch a uniform nbx x nby x nbz block grid (blocks numbered
+ * x fastest) with OUTFLOW domain faces, enabling device guard fills.
+ * Replaces: Grid_fillGuardCells for one refinement level,
+ * Grid/GridMain/AMR/Paramesh4/Grid_fillGuardCells.F90:436-546 and
+ * Grid/GridBoundaryConditions/Grid_bcApplyToRegion.F90:526-532. */
+int spark_set_uniform_grid(spark_batch* b, int nbx, int nby, int nbz);
+int spark_fill_guardcells(spark_batch* b);
+
+/* CFL time step over all blocks.  minloc = (block, i, j, k, level), the
+ * indices 1-based in the guard-celled box as Fortran sees them.
+ * Replaces: Hydro_computeDt, Hydro_interface.F90:24-42 /
+ * Spark/Hydro_computeDt.F90:27-150 (per tile) plus the MIN over tiles of
+ * Driver/DriverMain/Driver_computeDt.F90:306. */
+int spark_compute_dt(spark_batch* b, double* dt, int minloc[5]);
+
+/* One full RK step of every block (shockDetect, saveState, the stages,
+ * updateState; flux registers when flux_correct).
+ * Replaces: Hydro(timeEndAdv, dt, dtOld, sweepOrder),
+ * Hydro_interface.F90:61-66 / Spark/Hydro.F90-mc:24-202. */
+int spark_hydro_step(spark_batch* b, double dt, double dtOld);
+
+/* Stage-level entry points (parity testing, custom drivers).
+ * step_begin = shockDetect + saveState (Spark/Hydro.F90-mc:129-137);
+ * stage = one pass of the stage loop body (Spark/Hydro.F90-mc:144-192)
+ * with ngoff extra guard layers; step_end = updateState. */
+int spark_step_begin(spark_batch* b);
+int spark_stage(spark_batch* b, int stage, double dt, int ngoff);
+int spark_step_end(spark_batch* b);
+
+/* nsteps device-resident steps: dt (CFL, on the device) -> guard fill (if a
+ * uniform grid was declared) -> step.  dts (optional, nsteps) receives the
+ * dt used at each step.  No host synchronisation between steps.
+ * Replaces: the Hydro / Driver_computeDt pair of the step loop,
+ * Driver/DriverMain/Unsplit/Driver_evolveAll.F90:136-275. */
+int spark_advance(spark_batch* b, int nsteps, double* dts);
+
+/* Flux registers (interior faces, weighted sum over stages).
+ * get: device -> host fluxBufX/Y/Z(NFLUXES, ...) per block, Fortran order;
+ *      replaces the hy_fluxBuf[XYZ] hand-off of Grid_putFluxData_block,
+ *      Spark/Hydro.F90-mc:197-198.
+ * set: host -> device correction data (restricted fine - coarse on the
+ *      coarse side of fine/coarse faces, zero elsewhere), as
+ *      Grid_getFluxCorrData_block returns it (Paramesh4/Grid_getFluxCorrData_block.F90:182-216).
+ * correct: hy_rk_correctFluxes on every block (Spark/hy_rk_correctFluxes.F90:46-338). */
+int spark_get_flux_registers(spark_batch* b, double* const* fbx, double* const* fby, double* const* fbz);
+int spark_set_flux_corrections(spark_batch* b, const double* const* fbx, const double* const* fby,
+                               const double* const* fbz);
+int spark_flux_correct(spark_batch* b, double dt);
+
+int spark_synchronize(spark_batch* b);
+/* counters[0] = faces whose Riemann solve met a negative a^2 */
+int spark_get_counters(spark_batch* b, long long* counters, int n);
+/* Copy the last error message of this thread into buf (NUL-terminated). */
+int spark_last_error(char* buf, int len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,222 @@
+"""ctypes wrapper of the CPU restatement (libspark_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+the cpu_baseline leg of bench.py, always as the checker.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libspark_oracle.so"
+
+RECON = {"weno5": 0, "mp5": 1, "plm": 2, "fog": 3}
+RIEMANN = {"hllc": 0, "hll": 1}
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [
+        ("gamma", ctypes.c_double), ("cfl", ctypes.c_double), ("cvisc", ctypes.c_double),
+        ("smlrho", ctypes.c_double), ("smallp", ctypes.c_double), ("smalle", ctypes.c_double),
+        ("smallx", ctypes.c_double), ("eintSwitch", ctypes.c_double), ("tiny", ctypes.c_double),
+        ("gasConst", ctypes.c_double), ("abar", ctypes.c_double), ("grav", ctypes.c_double * 3),
+        ("ndim", ctypes.c_int), ("nguard", ctypes.c_int), ("nstencil", ctypes.c_int), ("rk_order", ctypes.c_int),
+        ("recon", ctypes.c_int), ("riemann", ctypes.c_int), ("use_flattening", ctypes.c_int),
+        ("use_hybrid_riemann", ctypes.c_int), ("nscalars", ctypes.c_int), ("use_gravity", ctypes.c_int),
+    ]
+
+
+class OrcBlock(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double)]
+
+
+_lib = None
+_D = ctypes.POINTER(ctypes.c_double)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
+        L = ctypes.CDLL(str(LIB_PATH))
+        vp = ctypes.c_void_p
+        P = ctypes.POINTER(OrcParams)
+        B = ctypes.POINTER(OrcBlock)
+        pp = ctypes.POINTER(ctypes.c_void_p)
+        L.orc_prim2con.argtypes = [_D, _D]
+        L.orc_prim2flx.argtypes = [ctypes.c_int, _D, _D]
+        L.orc_riemann_hllc.argtypes = [P, ctypes.c_int, _D, _D, ctypes.c_int, _D]
+        L.orc_riemann_hllc.restype = ctypes.c_int
+        L.orc_riemann_hll.argtypes = [P, ctypes.c_int, _D, _D, _D]
+        L.orc_riemann_hll.restype = ctypes.c_int
+        L.orc_recon.argtypes = [P, ctypes.c_int, _D, ctypes.c_int, ctypes.c_int, ctypes.c_double, _D, _D]
+        L.orc_eos_dens_ei_cell.argtypes = [P, _D, ctypes.c_int]
+        L.orc_flattening.argtypes = [P, B, _D, ctypes.POINTER(ctypes.c_int), _D]
+        L.orc_get_face_flux.argtypes = [P, B, _D, _D, ctypes.POINTER(ctypes.c_int), _D, _D, _D,
+                                        ctypes.POINTER(ctypes.c_int)]
+        L.orc_update_soln.argtypes = [P, B, _D, _D, ctypes.POINTER(ctypes.c_int), ctypes.c_double, _D, _D, _D, _D]
+        L.orc_eos_wrapped.argtypes = [P, B, _D, ctypes.POINTER(ctypes.c_int)]
+        L.orc_shock_detect.argtypes = [P, B, _D]
+        L.orc_compute_dt.argtypes = [P, B, _D, ctypes.POINTER(ctypes.c_int)]
+        L.orc_compute_dt.restype = ctypes.c_double
+        L.orc_stage.argtypes = [P, B, _D, _D, ctypes.c_int, ctypes.c_int, ctypes.c_double, _D, _D, _D,
+                                ctypes.POINTER(ctypes.c_int)]
+        L.orc_hydro_block.argtypes = [P, B, _D, ctypes.c_double, _D, _D, _D]
+        L.orc_hydro_block.restype = ctypes.c_int
+        L.orc_correct_fluxes.argtypes = [P, B, _D, ctypes.c_double, _D, _D, _D]
+        L.orc_ug_fill_guards.argtypes = [P, B, ctypes.c_int, ctypes.c_int, ctypes.c_int, pp, ctypes.c_int]
+        L.orc_ug_advance.argtypes = [P, B, ctypes.c_int, ctypes.c_int, ctypes.c_int, pp, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, _D]
+        L.orc_ug_advance.restype = ctypes.c_int
+        L.orc_batch_advance.argtypes = [P, B, ctypes.c_int, pp, ctypes.c_int, ctypes.c_int, _D]
+        L.orc_batch_advance.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def params(cfg: dict) -> OrcParams:
+    p = OrcParams()
+    for name, _ in OrcParams._fields_:
+        if name == "grav":
+            g = cfg.get("grav", (0.0, 0.0, 0.0))
+            for i in range(3):
+                p.grav[i] = float(g[i])
+            continue
+        v = cfg[name]
+        if name == "recon" and isinstance(v, str):
+            v = RECON[v]
+        if name == "riemann" and isinstance(v, str):
+            v = RIEMANN[v]
+        setattr(p, name, v)
+    return p
+
+
+def block(b: dict) -> OrcBlock:
+    return OrcBlock(int(b["nx"]), int(b["ny"]), int(b["nz"]), float(b["dx"]), float(b["dy"]), float(b["dz"]))
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_D)
+
+
+def ptr_array(arrays):
+    out = (ctypes.c_void_p * len(arrays))()
+    for i, a in enumerate(arrays):
+        assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+        out[i] = a.ctypes.data
+    return out
+
+
+def hydro_block(cfg, b, U, dt, fb=None):
+    P, B = params(cfg), block(b)
+    if fb is None:
+        return lib().orc_hydro_block(ctypes.byref(P), ctypes.byref(B), dptr(U), dt, None, None, None)
+    return lib().orc_hydro_block(ctypes.byref(P), ctypes.byref(B), dptr(U), dt, dptr(fb[0]), dptr(fb[1]),
+                                 dptr(fb[2]))
+
+
+def compute_dt(cfg, b, U):
+    P, B = params(cfg), block(b)
+    loc = (ctypes.c_int * 3)()
+    dt = lib().orc_compute_dt(ctypes.byref(P), ctypes.byref(B), dptr(U), loc)
+    return dt, list(loc)
+
+
+def ug_advance(cfg, b, ug, arrays, nsteps, refill=False, nthreads=0):
+    P, B = params(cfg), block(b)
+    dts = np.zeros(max(1, nsteps))
+    nerr = lib().orc_ug_advance(ctypes.byref(P), ctypes.byref(B), ug[0], ug[1], ug[2], ptr_array(arrays), nsteps,
+                                int(refill), nthreads, dptr(dts))
+    return dts[:nsteps], nerr
+
+
+def ug_fill(cfg, b, ug, arrays):
+    P, B = params(cfg), block(b)
+    nunk = 11 + cfg["nscalars"]
+    lib().orc_ug_fill_guards(ctypes.byref(P), ctypes.byref(B), ug[0], ug[1], ug[2], ptr_array(arrays), nunk)
+
+
+def batch_advance(cfg, blocks, arrays, nsteps, nthreads=0):
+    P = params(cfg)
+    bl = (OrcBlock * len(blocks))(*[block(b) for b in blocks])
+    dts = np.zeros(max(1, nsteps))
+    nerr = lib().orc_batch_advance(ctypes.byref(P), bl, len(blocks), ptr_array(arrays), nsteps, nthreads, dptr(dts))
+    return dts[:nsteps], nerr
+
+
+def stage(cfg, b, U0, S, stage_no, ngoff, dt, fb=None):
+    P, B = params(cfg), block(b)
+    nerr = ctypes.c_int(0)
+    if fb is None:
+        lib().orc_stage(ctypes.byref(P), ctypes.byref(B), dptr(U0), dptr(S), stage_no, ngoff, dt, None, None, None,
+                        ctypes.byref(nerr))
+    else:
+        lib().orc_stage(ctypes.byref(P), ctypes.byref(B), dptr(U0), dptr(S), stage_no, ngoff, dt, dptr(fb[0]),
+                        dptr(fb[1]), dptr(fb[2]), ctypes.byref(nerr))
+    return nerr.value
+
+
+def shock_detect(cfg, b, U):
+    P, B = params(cfg), block(b)
+    lib().orc_shock_detect(ctypes.byref(P), ctypes.byref(B), dptr(U))
+
+
+def correct_fluxes(cfg, b, U, dt, fb):
+    P, B = params(cfg), block(b)
+    lib().orc_correct_fluxes(ctypes.byref(P), ctypes.byref(B), dptr(U), dt, dptr(fb[0]), dptr(fb[1]), dptr(fb[2]))
+
+
+# ---- single-function entry points (closed-form tests) ----------------------
+def prim2con(V):
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    U = np.zeros(5)
+    lib().orc_prim2con(dptr(V), dptr(U))
+    return U
+
+
+def prim2flx(d, V):
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    F = np.zeros(5)
+    lib().orc_prim2flx(d, dptr(V), dptr(F))
+    return F
+
+
+def hllc(cfg, d, VL, VR, in_shock=False):
+    P = params(cfg)
+    F = np.zeros(5)
+    VL = np.ascontiguousarray(VL, dtype=np.float64)
+    VR = np.ascontiguousarray(VR, dtype=np.float64)
+    ierr = lib().orc_riemann_hllc(ctypes.byref(P), d, dptr(VL), dptr(VR), int(in_shock), dptr(F))
+    return F, ierr
+
+
+def hll(cfg, d, VL, VR):
+    P = params(cfg)
+    F = np.zeros(5)
+    VL = np.ascontiguousarray(VL, dtype=np.float64)
+    VR = np.ascontiguousarray(VR, dtype=np.float64)
+    ierr = lib().orc_riemann_hll(ctypes.byref(P), d, dptr(VL), dptr(VR), dptr(F))
+    return F, ierr
+
+
+def recon(cfg, pen, i1, flat=1.0):
+    """pen: (nrecon, plen) array; returns (uPlus, uMinus) at cell i1."""
+    P = params(cfg)
+    pen = np.ascontiguousarray(pen, dtype=np.float64)
+    nr, plen = pen.shape
+    uP, uM = np.zeros(nr), np.zeros(nr)
+    lib().orc_recon(ctypes.byref(P), nr, dptr(pen), i1, plen, flat, dptr(uP), dptr(uM))
+    return uP, uM
+
+
+def eos_cell(cfg, v, full=True):
+    P = params(cfg)
+    v = np.ascontiguousarray(v, dtype=np.float64).copy()
+    lib().orc_eos_dens_ei_cell(ctypes.byref(P), dptr(v), int(full))
+    return v

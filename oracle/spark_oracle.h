@@ -1,0 +1,121 @@
+/*
+ * spark_oracle.h -- CPU restatement of the Flash-X Spark unsplit-hydro RK
+ * stage (Flash-X/source/physics/Hydro/HydroMain/Spark).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the HIP path is
+ * compared against.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it; no product code links it.
+ *
+ * Block data use the reference unk layout: unk(var, i, j, k), variable
+ * index fastest (Paramesh_for_FlashX/.../physicaldata.F90:80), 0-based here.
+ * Variable order is the alphabetical order setup.py gives Spark + Gamma
+ * (dens eint ener gamc game pres shok temp velx vely velz), then the mass
+ * scalars.  Face fluxes use (flux, i, j, k) with the face i the low face of
+ * cell i, as hy_saveFluxes stores them (Spark/Inhost/hy_local.ini:118-127).
+ */
+#ifndef SPARK_ORACLE_H
+#define SPARK_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  ORC_DENS = 0, ORC_EINT = 1, ORC_ENER = 2, ORC_GAMC = 3, ORC_GAME = 4,
+  ORC_PRES = 5, ORC_SHOK = 6, ORC_TEMP = 7, ORC_VELX = 8, ORC_VELY = 9,
+  ORC_VELZ = 10, ORC_NPROP = 11
+};
+
+enum { ORC_RECON_WENO5 = 0, ORC_RECON_MP5 = 1, ORC_RECON_PLM = 2, ORC_RECON_FOG = 3 };
+enum { ORC_RIEMANN_HLLC = 0, ORC_RIEMANN_HLL = 1 };
+
+typedef struct orc_params {
+  double gamma;        /* Eos/EosMain/Config:43            */
+  double cfl;          /* Hydro/HydroMain/Config:26        */
+  double cvisc;        /* Spark/Config:17                  */
+  double smlrho, smallp, smalle, smallx; /* Grid Config floors */
+  double eintSwitch;   /* Eos/EosMain/Config:39            */
+  double tiny;         /* hy_tiny, Spark/Hydro_data.F90:85 */
+  double gasConst;     /* PhysicalConstants_init.F90:186   */
+  double abar;         /* eos_singleSpeciesA               */
+  double grav[3];      /* constant gravity (source term only) */
+  int ndim, nguard, nstencil, rk_order;
+  int recon, riemann;
+  int use_flattening, use_hybrid_riemann;
+  int nscalars;
+  int use_gravity;
+} orc_params;
+
+/* One block: interior sizes (1 for unused dims) and its deltas. */
+typedef struct orc_block {
+  int nx, ny, nz;
+  double dx, dy, dz;
+} orc_block;
+
+/* helpers */
+int orc_nunk(const orc_params* P);
+int orc_nflux(const orc_params* P);
+void orc_gc_dims(const orc_params* P, const orc_block* b, int* NX, int* NY, int* NZ);
+
+/* Individual reference functions (exposed for unit tests). */
+void orc_prim2con(const double* V, double* U);                /* prim2con.F90:34-62 */
+void orc_prim2flx(int dir, const double* V, double* F);       /* prim2flx.F90:36-118 */
+int  orc_riemann_hllc(const orc_params* P, int dir, const double* VL, const double* VR,
+                      int inShock, double* F);                /* riemann.F90-mc:42-282 */
+int  orc_riemann_hll(const orc_params* P, int dir, const double* VL, const double* VR,
+                     double* F);                               /* tvd2/riemann.F90:63-141 */
+void orc_recon(const orc_params* P, int nrecon, const double* pen, int i1, int plen,
+               double flat, double* uP, double* uM);           /* reconstruct.F90-mc:20-104 */
+void orc_eos_dens_ei_cell(const orc_params* P, double* v, int full);  /* Eos_wrapped path */
+
+/* Block-level kernels. lim[d][0..1] are 0-based GC-box limits. */
+void orc_flattening(const orc_params* P, const orc_block* b, const double* S,
+                    const int lim[3][2], double* flat3d);
+void orc_get_face_flux(const orc_params* P, const orc_block* b, const double* S,
+                       const double* flat3d, const int lim[3][2],
+                       double* flx, double* fly, double* flz, int* nerr);
+void orc_update_soln(const orc_params* P, const orc_block* b, const double* U0, double* S,
+                     const int lim[3][2], double dt, const double coeffs[3],
+                     const double* flx, const double* fly, const double* flz);
+void orc_eos_wrapped(const orc_params* P, const orc_block* b, double* S, const int lim[3][2]);
+void orc_shock_detect(const orc_params* P, const orc_block* b, double* U);
+double orc_compute_dt(const orc_params* P, const orc_block* b, const double* U, int minloc[3]);
+
+/* RK tables (Spark/Hydro.ini:2-48, with the RK3 constants defined). */
+void orc_rk_tables(const orc_params* P, int* last_stage, double coeff[3][3],
+                   int limits_array[3], double weights[3]);
+
+/* One stage on one block: ngoff extra layers, optional flux-buffer sum. */
+void orc_stage(const orc_params* P, const orc_block* b, const double* U0, double* S,
+               int stage, int ngoff, double dt,
+               double* fbx, double* fby, double* fbz, int* nerr);
+
+/* Whole Hydro() body for one block, telescoping the stages
+ * (Spark/Hydro.F90-mc:113-201).  U is updated in place (interior). */
+int orc_hydro_block(const orc_params* P, const orc_block* b, double* U, double dt,
+                    double* fbx, double* fby, double* fbz);
+
+/* Flux correction on one block (hy_rk_correctFluxes.F90:46-338).
+ * fb[xyz] are (restricted fine - coarse) on coarse-side f/c faces, 0 elsewhere. */
+void orc_correct_fluxes(const orc_params* P, const orc_block* b, double* U, double dt,
+                        const double* fbx, const double* fby, const double* fbz);
+
+/* Uniform block grid (one refinement level) helpers: guard fill by
+ * same-level neighbour copy with OUTFLOW at the domain faces, and a full
+ * multi-step driver in either parity (telescoping) or refill mode. */
+void orc_ug_fill_guards(const orc_params* P, const orc_block* b, int nbx, int nby, int nbz,
+                        double* const* blocks, int nvar);
+int orc_ug_advance(const orc_params* P, const orc_block* b, int nbx, int nby, int nbz,
+                   double* const* blocks, int nsteps, int refill, int nthreads,
+                   double* dts_out);
+double orc_ug_compute_dt(const orc_params* P, const orc_block* b, int nblocks,
+                         double* const* blocks);
+
+/* Independent blocks (frozen guards): nsteps of orc_hydro_block with a
+ * shared CFL dt recomputed each step.  blocks[i] each has its own shape. */
+int orc_batch_advance(const orc_params* P, const orc_block* bl, int nblocks,
+                      double* const* blocks, int nsteps, int nthreads, double* dts_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
