@@ -1,0 +1,144 @@
+#!/usr/bin/env python3
+"""Benchmark of the Spark RK stage on one MI355X.
+
+Default workload: 3D Sedov blast, uniform 8^3 blocks of 16^3 interior
+cells (128^3 zones), 6 guard cells refilled on the device between stages,
+WENO5 + HLLC, RK3, CFL 0.8, gamma 1.4, plain FP64 form.  A step is one
+full RK3 step of all blocks (CFL dt on the device, guard fill, 3 stages).
+Inputs are resident in HBM before the timed region; the timed region is
+K device-resident steps bracketed by device synchronisations.
+
+Prints one JSON line (see the metric/roofline/cpu_baseline fields).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "flash-x_amd"))
+
+METRIC = "zone-updates/s per RK3 step: plain FP64 vs batched FP32-flux; % HBM roofline"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    return ap.parse_args()
+
+
+def cpu_baseline(target_s: float):
+    """The oracle (CPU restatement, OpenMP over blocks) on the same workload,
+    bounded to about target_s seconds of wall time."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle
+    from sparkhip import problems
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    prob = problems.sedov3d(nb=8, n=16, nguard=6, guard_mode="refill")
+    t0 = time.perf_counter()
+    pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, prob.arrays, 1, refill=True, nthreads=threads)
+    one = time.perf_counter() - t0
+    nsteps = int(max(1, min(30, (target_s - one) / max(one, 1e-3))))
+    t0 = time.perf_counter()
+    pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, prob.arrays, nsteps, refill=True, nthreads=threads)
+    el = time.perf_counter() - t0
+    return {
+        "value": prob.zones * nsteps / el,
+        "unit": "zone-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"3D Sedov 128^3 (8^3 blocks of 16^3, 6 guards, refill), {nsteps} RK3 steps after 1 untimed, "
+                  f"{el:.1f} s wall, OpenMP over blocks",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the block set is a single-GPU "
+              f"shard and no multi-GPU path is built", file=sys.stderr)
+        sys.exit(2)
+    from sparkhip import capi, problems
+
+    prob = problems.sedov3d(nb=8, n=16, nguard=6, guard_mode="refill")
+    capi.init(0)
+    b = capi.Batch(prob.cfg, prob.blocks)
+    b.pack(prob.arrays)
+    b.set_uniform_grid(*prob.ug)
+    if args.warmup:
+        b.advance(args.warmup)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.advance(args.steps)
+    b.synchronize()
+    el = time.perf_counter() - t0
+    value = prob.zones * args.steps / el
+
+    # per-kernel HIP-event timing over a separate pass (same stream)
+    b.profile(True)
+    b.advance(args.profile_steps)
+    stats = b.profile_read()
+    b.profile(False)
+    b.close()
+    tot = sum(s["ms"] for s in stats)
+    flux = [s for s in stats if s["name"].startswith("k_flux")]
+    dom = max(stats, key=lambda s: s["ms"])
+    fl_ms = sum(s["ms"] for s in flux)
+    fl_bytes = sum(s["bytes"] for s in flux)
+    fl_flops = sum(s["flops"] for s in flux)
+    fl_launch = sum(s["launches"] for s in flux)
+    achieved = fl_bytes / (fl_ms * 1e-3) / 1e9
+    roofline = {
+        "bound": "hbm",
+        "kernel": "k_flux (x, y, z sweeps: WENO5-Z + HLLC face fluxes)",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "avg_launch_ms": round(fl_ms / max(1, fl_launch), 4),
+        "alg_bytes_per_launch": round(fl_bytes / max(1, fl_launch)),
+        "valu_fp64": {"achieved_tflops": round(fl_flops / (fl_ms * 1e-3) / 1e12, 2), "peak_tflops": FP64_PEAK_TFS,
+                      "frac": round(fl_flops / (fl_ms * 1e-3) / 1e12 / FP64_PEAK_TFS, 4)},
+    }
+    step_bytes = 384.0 * prob.zones  # algorithmic FP64 state traffic per zone-update
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "3D Sedov blast, 8^3 blocks of 16^3 (128^3 zones), 6 guard cells refilled on the device "
+                        "between stages, WENO5-Z + HLLC, RK3, CFL 0.8, gamma 1.4, plain FP64 form",
+            "zones": prob.zones, "blocks": len(prob.blocks), "form": "plain_fp64",
+        },
+        "roofline": roofline,
+        "path_hbm_frac_algorithmic": round(value * step_bytes / prob.zones / 1e9 / HBM_PEAK_GBS, 5),
+        "kernel_share": {s["name"]: round(s["ms"] / tot, 4) for s in sorted(stats, key=lambda s: -s["ms"])},
+        "dominant_kernel": dom["name"],
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

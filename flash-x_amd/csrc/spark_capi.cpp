@@ -87,6 +87,7 @@ struct spark_batch {
   bool has_ug = false;
   bool step_open = false;
   int64_t max_cells = 0;
+  spark::Prof prof;
   int last_stage = 3;
   double coeff[3][3];
   int limits_array[3];
@@ -130,6 +131,7 @@ PlainLaunch make_launch(spark_batch* b, const ShapeClass& c) {
   L.add_flux = 0;
   L.dtp = b->dtp;
   L.err = b->err;
+  L.prof = b->prof.on ? &b->prof : nullptr;
   return L;
 }
 
@@ -164,7 +166,9 @@ int run_step_begin(spark_batch* b) {
     PlainLaunch L = make_launch(b, c);
     HIPCHK(spark::plain_shock_detect(L, b->U, b->flat));
   }
+  HIPCHK(b->prof.begin(g_stream, "saveState_copy", 16.0 * b->pool, 0.0));
   HIPCHK(hipMemcpyAsync(b->S, b->U, sizeof(double) * b->pool, hipMemcpyDeviceToDevice, g_stream));
+  HIPCHK(b->prof.end(g_stream));
   b->step_open = true;
   return SPARK_OK;
 }
@@ -203,6 +207,10 @@ int run_full_step(spark_batch* b) {
 
 void free_batch(spark_batch* b) {
   if (!b) return;
+  for (auto& r : b->prof.recs) {
+    if (r.e0) (void)hipEventDestroy(r.e0);
+    if (r.e1) (void)hipEventDestroy(r.e1);
+  }
   auto F = [](void* p) { if (p) (void)hipFree(p); };
   F(b->meta_d);
   for (auto& c : b->classes) F(c.ids_d);
@@ -635,6 +643,49 @@ int spark_flux_correct(spark_batch* b, double dt) {
     PlainLaunch L = make_launch(b, c);
     HIPCHK(spark::plain_correct(L, b->U));
   }
+  return SPARK_OK;
+}
+
+static void prof_clear(spark_batch* b) {
+  for (auto& r : b->prof.recs) {
+    if (r.e0) (void)hipEventDestroy(r.e0);
+    if (r.e1) (void)hipEventDestroy(r.e1);
+  }
+  b->prof.recs.clear();
+}
+
+int spark_profile(spark_batch* b, int enable) {
+  if (!b) return fail(SPARK_ERR_ARG, "spark_profile: null batch");
+  HIPCHK(hipStreamSynchronize(g_stream));
+  prof_clear(b);
+  b->prof.on = enable != 0;
+  return SPARK_OK;
+}
+
+int spark_profile_read(spark_batch* b, spark_kernel_stat* out, int max, int* n) {
+  if (!b || !out || !n || max < 0) return fail(SPARK_ERR_ARG, "spark_profile_read: bad argument");
+  HIPCHK(hipStreamSynchronize(g_stream));
+  std::vector<spark_kernel_stat> acc;
+  for (auto& r : b->prof.recs) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, r.e0, r.e1));
+    spark_kernel_stat* k = nullptr;
+    for (auto& a : acc)
+      if (strncmp(a.name, r.name, sizeof(a.name)) == 0) k = &a;
+    if (!k) {
+      spark_kernel_stat z;
+      memset(&z, 0, sizeof(z));
+      snprintf(z.name, sizeof(z.name), "%s", r.name);
+      acc.push_back(z);
+      k = &acc.back();
+    }
+    k->launches += 1;
+    k->total_ms += ms;
+    k->bytes += r.bytes;
+    k->flops += r.flops;
+  }
+  *n = (int)acc.size();
+  for (int i = 0; i < (int)acc.size() && i < max; i++) out[i] = acc[i];
   return SPARK_OK;
 }
 

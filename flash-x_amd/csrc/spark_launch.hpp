@@ -3,9 +3,37 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <vector>
+
 #include "spark_physics.hpp"
 
 namespace spark {
+
+// Optional per-launch timing: HIP events recorded on the launch stream
+// around each kernel, with the launch's algorithmic bytes and flops.
+struct ProfRec {
+  const char* name;
+  hipEvent_t e0, e1;
+  double bytes, flops;
+};
+struct Prof {
+  bool on = false;
+  std::vector<ProfRec> recs;
+  hipError_t begin(hipStream_t st, const char* name, double bytes, double flops) {
+    if (!on) return hipSuccess;
+    ProfRec r{name, nullptr, nullptr, bytes, flops};
+    hipError_t e = hipEventCreate(&r.e0);
+    if (e == hipSuccess) e = hipEventCreate(&r.e1);
+    if (e == hipSuccess) e = hipEventRecord(r.e0, st);
+    recs.push_back(r);
+    return e;
+  }
+  hipError_t end(hipStream_t st) {
+    if (!on || recs.empty()) return hipSuccess;
+    return hipEventRecord(recs.back().e1, st);
+  }
+};
 
 // Everything one plain-form launch sequence needs for a list of blocks that
 // share one interior shape.
@@ -25,6 +53,7 @@ struct PlainLaunch {
   int add_flux;           // 0: overwrite (stage 1), 1: accumulate
   const double* dtp;      // device scalar holding dt
   int* err;               // device counter of negative a^2 events
+  Prof* prof;             // per-launch timing (nullptr: off)
 };
 
 hipError_t plain_stage(const PlainLaunch& L, int ngoff, double c1, double c2, double c3, int final_stage);

@@ -647,8 +647,23 @@ static hipError_t launch_flux_t(const PlainLaunch& L, int ngoff) {
   dim3 grid(chunks * nt2, L.nblk);
   dim3 block(lanes * Lr);
   const size_t shm = sizeof(double) * (size_t)(NRM + 6) * block.x;
+  if (L.prof) {
+    // algorithmic traffic: every state value of the stencil box read once
+    // (dens velx vely velz pres gamc eint shok, scalars, flat), every face
+    // flux written once; flops: 117 per variable per reconstructed cell
+    // (WENO5-Z) + 175 per face (HLLC, viscosity, floors)
+    const double cells_in = (double)(Lr + 2 * ReconRadius<RECON>::R) * nt1 * nt2;
+    const double recon_cells = (double)Lr * nt1 * nt2;
+    const double faces = (double)(Lr - 1) * nt1 * nt2;
+    const int ns = L.P.ns;
+    const double bytes = 8.0 * L.nblk * (cells_in * (8 + ns + (RECON == RECON_WENO5 && L.P.use_flat)) + faces * (5 + ns));
+    const double flops = (double)L.nblk * (recon_cells * (7 + ns) * 117.0 + faces * (175.0 + ns));
+    static const char* names[3] = {"k_flux_x", "k_flux_y", "k_flux_z"};
+    L.prof->begin(L.stream, names[DIR], bytes, flops);
+  }
   hipLaunchKernelGGL((k_flux<DIR, RECON, RIEMANN, NSM>), grid, block, shm, L.stream, L.P, L.meta, L.blist, L.S,
                      L.flat, L.fl[DIR], ngoff, lanes, L.err);
+  if (L.prof) L.prof->end(L.stream);
   return hipGetLastError();
 }
 
@@ -678,9 +693,17 @@ hipError_t plain_stage(const PlainLaunch& L, int ngoff, double c1, double c2, do
   for (int d = 0; d < 3; d++) {
     if (d < L.P.ndim) { box *= n[d] + 2 * ngoff; boxf *= n[d] + 2 * ngoff + 2; }
   }
+  const int nsv = L.P.ns;
   if (L.P.use_flat && L.P.recon == RECON_WENO5) {
+    if (L.prof) {
+      int bx = 1, bxr = 1;
+      for (int d = 0; d < L.P.ndim; d++) { bx *= n[d] + 2 * ngoff + 2; bxr *= n[d] + 2 * ngoff + 6; }
+      L.prof->begin(L.stream, "k_flatten", 8.0 * L.nblk * ((double)bxr * (2 + L.P.ndim) + bx),
+                    (double)L.nblk * bx * 25.0 * L.P.ndim);
+    }
     hipLaunchKernelGGL(k_flatten, dim3(grid_x_for(boxf, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta, L.blist,
                        L.S, L.flat, ngoff);
+    if (L.prof) L.prof->end(L.stream);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   for (int dir = 0; dir < L.P.ndim; dir++) {
@@ -700,37 +723,56 @@ hipError_t plain_stage(const PlainLaunch& L, int ngoff, double c1, double c2, do
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
   }
+  if (L.prof) {
+    // reads U0 and S (5+ns each) and one face flux per face of the box,
+    // writes dens vel(3) pres gamc eint ener (+temp game) and the scalars
+    double faces = 0.;
+    for (int d = 0; d < L.P.ndim; d++) faces += (double)box / (n[d] + 2 * ngoff) * (n[d] + 2 * ngoff + 1);
+    const double bytes = 8.0 * L.nblk * ((double)box * (2 * (5 + nsv) + 8 + nsv + 2 * final_stage) + faces * (5 + nsv));
+    L.prof->begin(L.stream, "k_update", bytes, (double)L.nblk * box * (60.0 + 10.0 * L.P.ndim * (5 + nsv)));
+  }
   if (L.P.ns == 0)
     hipLaunchKernelGGL(k_update<0>, dim3(grid_x_for(box, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta, L.blist,
                        L.U0, L.S, L.fl[0], L.fl[1], L.fl[2], L.dtp, c1, c2, c3, ngoff, final_stage);
   else
     hipLaunchKernelGGL(k_update<8>, dim3(grid_x_for(box, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta, L.blist,
                        L.U0, L.S, L.fl[0], L.fl[1], L.fl[2], L.dtp, c1, c2, c3, ngoff, final_stage);
+  if (L.prof) L.prof->end(L.stream);
   return hipGetLastError();
 }
 
 hipError_t plain_copy_interior(const PlainLaunch& L, const double* S, double* U, int nvar) {
   int box = L.nx * (L.P.ndim > 1 ? L.ny : 1) * (L.P.ndim > 2 ? L.nz : 1);
+  if (L.prof) L.prof->begin(L.stream, "k_copy_interior", 16.0 * L.nblk * (double)box * nvar, 0.0);
   hipLaunchKernelGGL(k_copy_interior, dim3(grid_x_for(box, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta,
                      L.blist, S, U, nvar);
+  if (L.prof) L.prof->end(L.stream);
   return hipGetLastError();
 }
 
 hipError_t plain_shock_detect(const PlainLaunch& L, double* U, double* scratch) {
   const int NX = L.nx + 2 * L.P.ng, NY = L.P.ndim > 1 ? L.ny + 2 * L.P.ng : 1,
             NZ = L.P.ndim > 2 ? L.nz + 2 * L.P.ng : 1;
+  const double nc = (double)NX * NY * NZ * L.nblk;
+  if (L.prof) L.prof->begin(L.stream, "k_shock_vc", 8.0 * nc * 5, nc * 6);
   hipLaunchKernelGGL(k_shock_vc, dim3(grid_x_for(NX * NY * NZ, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta,
                      L.blist, U, scratch);
+  if (L.prof) L.prof->end(L.stream);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  if (L.prof) L.prof->begin(L.stream, "k_shock", 8.0 * nc * 6, nc * 70);
   hipLaunchKernelGGL(k_shock, dim3(grid_x_for(NX * NY * NZ, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta,
                      L.blist, U, scratch);
+  if (L.prof) L.prof->end(L.stream);
   return hipGetLastError();
 }
 
 hipError_t plain_dt_part(const PlainLaunch& L, const double* U, void* parts, int chunks) {
+  const double nc = (double)L.nx * (L.P.ndim > 1 ? L.ny : 1) * (L.P.ndim > 2 ? L.nz : 1) * L.nblk;
+  if (L.prof) L.prof->begin(L.stream, "k_dt_part", 8.0 * nc * (3 + L.P.ndim), nc * (6 + 4 * L.P.ndim));
   hipLaunchKernelGGL(k_dt_part, dim3(chunks, L.nblk), dim3(256), 0, L.stream, L.P, L.meta, L.blist, U,
                      (DtPart*)parts, chunks);
+  if (L.prof) L.prof->end(L.stream);
   return hipGetLastError();
 }
 
@@ -755,8 +797,11 @@ size_t plain_dt_part_size() { return sizeof(DtPart); }
 hipError_t plain_ug_fill(const PlainLaunch& L, int nbx, int nby, int nbz, double* U, int nvar) {
   const int NX = L.nx + 2 * L.P.ng, NY = L.P.ndim > 1 ? L.ny + 2 * L.P.ng : 1,
             NZ = L.P.ndim > 2 ? L.nz + 2 * L.P.ng : 1;
+  const int nin = L.nx * (L.P.ndim > 1 ? L.ny : 1) * (L.P.ndim > 2 ? L.nz : 1);
+  if (L.prof) L.prof->begin(L.stream, "k_ug_fill", 16.0 * L.nblk * (double)(NX * NY * NZ - nin) * nvar, 0.0);
   hipLaunchKernelGGL(k_ug_fill, dim3(grid_x_for(NX * NY * NZ, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta,
                      nbx, nby, nbz, U, nvar);
+  if (L.prof) L.prof->end(L.stream);
   return hipGetLastError();
 }
 

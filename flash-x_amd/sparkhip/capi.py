@@ -51,6 +51,11 @@ class SparkBlockDesc(ctypes.Structure):
                 ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double)]
 
 
+class SparkKernelStat(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_longlong), ("total_ms", ctypes.c_double),
+                ("bytes", ctypes.c_double), ("flops", ctypes.c_double)]
+
+
 class SparkError(RuntimeError):
     pass
 
@@ -90,6 +95,8 @@ def lib():
             "spark_synchronize": [vp],
             "spark_get_counters": [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.c_int],
             "spark_last_error": [ctypes.c_char_p, ctypes.c_int],
+            "spark_profile": [vp, ctypes.c_int],
+            "spark_profile_read": [vp, ctypes.POINTER(SparkKernelStat), ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -106,7 +113,7 @@ def exported_symbols():
         "spark_pack", "spark_unpack", "spark_set_uniform_grid", "spark_fill_guardcells", "spark_compute_dt",
         "spark_hydro_step", "spark_step_begin", "spark_stage", "spark_step_end", "spark_advance",
         "spark_get_flux_registers", "spark_set_flux_corrections", "spark_flux_correct", "spark_synchronize",
-        "spark_get_counters", "spark_last_error",
+        "spark_get_counters", "spark_last_error", "spark_profile", "spark_profile_read",
     ]
 
 
@@ -243,6 +250,16 @@ class Batch:
     def set_flux_corrections(self, fbx, fby, fbz):
         check(lib().spark_set_flux_corrections(self.h, _ptrs(fbx), _ptrs(fby), _ptrs(fbz)),
               "spark_set_flux_corrections")
+
+    def profile(self, enable: bool):
+        check(lib().spark_profile(self.h, int(enable)), "spark_profile")
+
+    def profile_read(self):
+        stats = (SparkKernelStat * 64)()
+        n = ctypes.c_int()
+        check(lib().spark_profile_read(self.h, stats, 64, ctypes.byref(n)), "spark_profile_read")
+        return [dict(name=stats[i].name.decode(), launches=stats[i].launches, ms=stats[i].total_ms,
+                     bytes=stats[i].bytes, flops=stats[i].flops) for i in range(min(n.value, 64))]
 
     def flux_correct(self, dt):
         check(lib().spark_flux_correct(self.h, float(dt)), "spark_flux_correct")

@@ -158,6 +158,22 @@ int spark_set_flux_corrections(spark_batch* b, const double* const* fbx, const d
                                const double* const* fbz);
 int spark_flux_correct(spark_batch* b, double dt);
 
+/* Per-kernel timing of the library's own launches: HIP events recorded on
+ * the library stream around every launch while enabled (enabling clears
+ * earlier records).  read() returns one entry per kernel name with the
+ * launch count, summed event time and summed algorithmic bytes and flops
+ * of those launches.  No counterpart in the reference (its Timers_start /
+ * Timers_stop labels, Spark/Hydro.F90-mc:76-191, time host regions). */
+typedef struct spark_kernel_stat {
+  char name[32];
+  long long launches;
+  double total_ms;
+  double bytes;
+  double flops;
+} spark_kernel_stat;
+int spark_profile(spark_batch* b, int enable);
+int spark_profile_read(spark_batch* b, spark_kernel_stat* stats, int max, int* n);
+
 int spark_synchronize(spark_batch* b);
 /* counters[0] = faces whose Riemann solve met a negative a^2 */
 int spark_get_counters(spark_batch* b, long long* counters, int n);
