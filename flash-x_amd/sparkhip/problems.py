@@ -124,7 +124,7 @@ def sedov3d(nb=8, n=16, nguard=6, gamma=1.4, E=1.0, r_cells=3.5, p_amb=1e-5, rho
     idx = (np.arange(N) + 0.5) * dx - 0.5
     r2 = idx[:, None, None] ** 2 + idx[None, :, None] ** 2 + idx[None, None, :] ** 2
     n_in = int(np.count_nonzero(r2 < r0 * r0))
-    p_in = (gamma - 1.0) * E / (n_in * dx ** 3)
+    p_in = (gamma - 1.0) * E / (n_in * dx ** 3) if n_in else p_amb
     blocks, arrays = [], []
     nunk = SPARK_NPROP + cfg["nscalars"]
     for bk in range(nb):

@@ -1,0 +1,229 @@
+"""Pins of the CPU restatement's pointwise functions by closed forms,
+invariants and hand-worked cases (no GPU)."""
+import numpy as np
+import pytest
+
+import pyoracle
+from sparkhip import problems
+
+CFG = problems.base_cfg(ndim=3, nguard=10)
+G = CFG["gamma"]
+
+
+def prim(d, u, v, w, p, g=G):
+    """pencil state (dens, velx, vely, velz, pres, gamc, rhoe)"""
+    return np.array([d, u, v, w, p, g, p / (g - 1.0)])
+
+
+def euler_flux(V, d):
+    r, u, v, w, p, g, re = V
+    vel = (u, v, w)
+    E = 0.5 * r * (u * u + v * v + w * w) + re
+    F = np.array([r * vel[d], r * vel[d] * u, r * vel[d] * v, r * vel[d] * w, (E + p) * vel[d]])
+    F[1 + d] += p
+    return F
+
+
+# --- prim2con / prim2flx ----------------------------------------------------
+def test_prim2con_closed_form():
+    V = prim(2.0, 0.5, -1.5, 3.0, 4.0)
+    U = pyoracle.prim2con(V)
+    E = 0.5 * 2.0 * (0.25 + 2.25 + 9.0) + 4.0 / 0.4
+    np.testing.assert_allclose(U, [2.0, 1.0, -3.0, 6.0, E], rtol=1e-15)
+
+
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_prim2flx_closed_form(d):
+    V = prim(1.3, 0.7, -0.2, 0.9, 2.5)
+    np.testing.assert_allclose(pyoracle.prim2flx(d, V), euler_flux(V, d), rtol=1e-15)
+
+
+# --- HLLC ---------------------------------------------------------------------
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_hllc_consistency(d):
+    """F(V, V) is the physical flux."""
+    V = prim(1.1, 0.3, -0.4, 0.2, 0.9)
+    F, ierr = pyoracle.hllc(CFG, d, V, V)
+    assert ierr == 0
+    np.testing.assert_allclose(F, euler_flux(V, d), rtol=1e-13, atol=1e-14)
+
+
+def test_hllc_supersonic_upwinding():
+    VL = prim(1.0, 5.0, 0.1, 0.0, 1.0)
+    VR = prim(0.5, 4.0, -0.3, 0.2, 0.4)
+    F, _ = pyoracle.hllc(CFG, 0, VL, VR)
+    assert np.array_equal(F, pyoracle.prim2flx(0, VL))
+    VL2 = prim(1.0, -5.0, 0.1, 0.0, 1.0)
+    VR2 = prim(0.5, -6.0, -0.3, 0.2, 0.4)
+    F2, _ = pyoracle.hllc(CFG, 0, VL2, VR2)
+    assert np.array_equal(F2, pyoracle.prim2flx(0, VR2))
+
+
+@pytest.mark.parametrize("u", [0.4, -0.4])
+def test_hllc_resolves_contact(u):
+    """Isolated contact (same u, p): HLLC returns the upwind physical flux."""
+    VL = prim(1.0, u, 0.2, -0.1, 1.0)
+    VR = prim(0.125, u, 0.7, 0.3, 1.0)
+    F, _ = pyoracle.hllc(CFG, 0, VL, VR)
+    up = VL if u > 0 else VR
+    np.testing.assert_allclose(F, euler_flux(up, 0), rtol=1e-12, atol=1e-13)
+
+
+def test_hllc_direction_symmetry():
+    VL = prim(1.0, 0.3, -0.5, 0.2, 1.0)
+    VR = prim(0.4, -0.1, 0.6, -0.3, 0.3)
+    Fx, _ = pyoracle.hllc(CFG, 0, VL, VR)
+    sw = lambda V: V[[0, 2, 1, 3, 4, 5, 6]]
+    Fy, _ = pyoracle.hllc(CFG, 1, sw(VL), sw(VR))
+    np.testing.assert_array_equal(Fx[[0, 2, 1, 3, 4]], Fy)
+
+
+def test_hllc_vacuum_and_negative_sound_speed():
+    VL = prim(1e-40, 0.0, 0, 0, 1.0)
+    F, ierr = pyoracle.hllc(CFG, 0, VL, prim(1.0, 0, 0, 0, 1.0))
+    assert ierr == 0 and np.all(F == 0.0)
+    bad = prim(1.0, 0, 0, 0, 1.0)
+    bad[5] = -1.4
+    _, ierr = pyoracle.hllc(CFG, 0, bad, prim(1.0, 0, 0, 0, 1.0))
+    assert ierr == 1
+
+
+def test_hllc_hybrid_is_hlle_in_shocks():
+    cfg = dict(CFG, use_hybrid_riemann=1)
+    VL = prim(1.0, 0.5, 0.0, 0.0, 2.0)
+    VR = prim(0.3, -0.5, 0.0, 0.0, 0.5)
+    F, _ = pyoracle.hllc(cfg, 0, VL, VR, in_shock=True)
+    cL, cR = np.sqrt(G * 2.0 / 1.0), np.sqrt(G * 0.5 / 0.3)
+    SL, SR = min(0.5 - cL, -0.5 - cR), max(0.5 + cL, -0.5 + cR)
+    UL, UR = pyoracle.prim2con(VL), pyoracle.prim2con(VR)
+    FL, FR = euler_flux(VL, 0), euler_flux(VR, 0)
+    np.testing.assert_allclose(F, (SR * FL - SL * FR + SR * SL * (UR - UL)) / (SR - SL), rtol=1e-13)
+    F2, _ = pyoracle.hllc(cfg, 0, VL, VR, in_shock=False)
+    assert not np.allclose(F, F2)
+
+
+def test_hll_consistency_and_upwinding():
+    V = prim(0.8, -0.2, 0.1, 0.3, 0.6)
+    F, ierr = pyoracle.hll(CFG, 2, V, V)
+    assert ierr == 0
+    np.testing.assert_allclose(F, euler_flux(V, 2), rtol=1e-13, atol=1e-14)
+    VL = prim(1.0, 5.0, 0.0, 0.0, 1.0)
+    F, _ = pyoracle.hll(CFG, 0, VL, prim(0.5, 4.5, 0, 0, 0.5))
+    assert np.array_equal(F, pyoracle.prim2flx(0, VL))
+
+
+# --- reconstruction -------------------------------------------------------------
+def pencil(vals, nrec=7):
+    return np.tile(np.asarray(vals, dtype=float), (nrec, 1))
+
+
+@pytest.mark.parametrize("recon", ["weno5", "mp5"])
+def test_linear_data_exact(recon):
+    cfg = dict(CFG, recon=recon)
+    a, b = 0.7, 0.35
+    x = a + b * np.arange(7)
+    uP, uM = pyoracle.recon(cfg, pencil(x), 3)
+    np.testing.assert_allclose(uP, a + b * 3.5, rtol=2e-15)
+    np.testing.assert_allclose(uM, a + b * 2.5, rtol=2e-15)
+
+
+def test_weno5_constant_and_flat_zero():
+    uP, uM = pyoracle.recon(CFG, pencil([2.5] * 7), 3)
+    np.testing.assert_allclose(uP, 2.5, rtol=4e-16)
+    np.testing.assert_allclose(uM, 2.5, rtol=4e-16)
+    x = np.sin(np.arange(7.0))
+    uP, uM = pyoracle.recon(CFG, pencil(x), 3, flat=0.0)
+    assert np.all(uP == x[3]) and np.all(uM == x[3])
+
+
+def test_weno5_fifth_order_on_smooth_data():
+    """Face-value error of sin(x) cell data at h and h/2: order close to 5."""
+    errs = []
+    for h in (0.2, 0.1):
+        xc = 1.0 + h * (np.arange(7) - 3)
+        # cell averages of sin over [x - h/2, x + h/2]
+        avg = (np.cos(xc - h / 2) - np.cos(xc + h / 2)) / h
+        uP, _ = pyoracle.recon(CFG, pencil(avg, 1), 3)
+        errs.append(abs(uP[0] - np.sin(1.0 + h / 2)))
+    order = np.log2(errs[0] / errs[1])
+    assert order > 4.5, (errs, order)
+
+
+def test_weno5_extremum_falls_back_to_first_order():
+    u = [0.0, 1.0, 3.0, 4.0, 3.0, 1.0, 0.0]
+    uP, uM = pyoracle.recon(CFG, pencil(u), 3)
+    assert np.all(uP == 4.0) and np.all(uM == 4.0)
+
+
+@pytest.mark.parametrize("recon", ["weno5", "mp5", "plm"])
+def test_step_reconstruction_is_bounded(recon):
+    cfg = dict(CFG, recon=recon)
+    u = [1.0, 1.0, 1.0, 1.0, 0.1, 0.1, 0.1]
+    for i in (2, 3, 4):
+        uP, uM = pyoracle.recon(cfg, pencil(u), i)
+        assert np.all((uP >= 0.1 - 1e-14) & (uP <= 1.0 + 1e-14))
+        assert np.all((uM >= 0.1 - 1e-14) & (uM <= 1.0 + 1e-14))
+
+
+def test_mp5_unlimited_on_smooth_monotone_data():
+    u = np.array([4, 9, 16, 25, 36, 49, 64], dtype=float)  # i^2, monotone
+    uP, uM = pyoracle.recon(dict(CFG, recon="mp5"), pencil(u), 3)
+    np.testing.assert_allclose(uP, (2 * 9 - 13 * 16 + 47 * 25 + 27 * 36 - 3 * 49) / 60.0, rtol=1e-15)
+    np.testing.assert_allclose(uM, (-3 * 9 + 27 * 16 + 47 * 25 - 13 * 36 + 2 * 49) / 60.0, rtol=1e-15)
+
+
+def test_plm_minmod_hand_cases():
+    cfg = dict(CFG, recon="plm")
+    uP, uM = pyoracle.recon(cfg, pencil([0, 1.0, 2.0, 4.0, 0, 0, 0]), 2)
+    assert np.all(uP == 2.5) and np.all(uM == 1.5)
+    uP, uM = pyoracle.recon(cfg, pencil([0, 1.0, 2.0, 1.0, 0, 0, 0]), 2)
+    assert np.all(uP == 2.0) and np.all(uM == 2.0)
+    uP, uM = pyoracle.recon(cfg, pencil([0, 4.0, 2.0, 1.0, 0, 0, 0]), 2)
+    assert np.all(uP == 1.5) and np.all(uM == 2.5)
+
+
+def test_fog_copies_cell_value():
+    u = np.random.default_rng(1).normal(size=7)
+    uP, uM = pyoracle.recon(dict(CFG, recon="fog"), pencil(u), 3)
+    assert np.all(uP == u[3]) and np.all(uM == u[3])
+
+
+# --- Gamma EOS ---------------------------------------------------------------
+def state(d=1.2, e=2.0, vx=0.3, vy=-0.4, vz=0.5, ns=0):
+    v = np.zeros(11 + ns)
+    v[0], v[1], v[8], v[9], v[10] = d, e, vx, vy, vz
+    v[2] = e + 0.5 * (vx * vx + vy * vy + vz * vz)
+    v[3] = G
+    return v
+
+
+def test_eos_dens_ei_closed_form():
+    v = pyoracle.eos_cell(CFG, state())
+    np.testing.assert_allclose(v[5], (G - 1.0) * 1.2 * 2.0, rtol=4e-16)
+    np.testing.assert_allclose(v[4], G, rtol=4e-16)          # game = p/(e rho) + 1
+    np.testing.assert_allclose(v[7], 2.0 * (G - 1.0) / CFG["gasConst"], rtol=4e-16)
+    assert v[3] == G
+    np.testing.assert_allclose(v[2], v[1] + 0.5 * (0.09 + 0.16 + 0.25), rtol=1e-15)
+
+
+def test_eos_round_trip_pres_to_eint():
+    """MODE_DENS_PRES -> MODE_DENS_EI returns the pressure (cf. the reference EOS unit test)."""
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        d, p = rng.uniform(0.1, 10), rng.uniform(0.1, 10)
+        e = p / ((G - 1.0) * d)
+        v = pyoracle.eos_cell(CFG, state(d=d, e=e))
+        np.testing.assert_allclose(v[5], p, rtol=1e-15)
+
+
+def test_eos_internal_energy_source_and_floor():
+    v = state(e=2.0)
+    v[2] = v[2] + 1.0  # E - ekin = 3 > smalle: eint taken from E
+    out = pyoracle.eos_cell(CFG, v)
+    np.testing.assert_allclose(out[1], 3.0, rtol=1e-15)
+    out = pyoracle.eos_cell(dict(CFG, eintSwitch=1e6), v)   # switch keeps the stored eint
+    assert out[1] == 2.0
+    v = state(e=1e-30)
+    v[2] = 0.5 * (0.09 + 0.16 + 0.25) - 1.0                  # E - ekin < 0
+    out = pyoracle.eos_cell(CFG, v)
+    assert out[1] == CFG["smalle"]

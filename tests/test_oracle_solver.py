@@ -1,0 +1,245 @@
+"""Pins of the CPU restatement's block-level stage and whole-solver behaviour:
+RK tables (exact polynomial on a linear problem), flux-register weights,
+flattening / shock-detector special cases, CFL dt closed form, flux
+correction signs, conservation, symmetry and the Sod exact solution."""
+import numpy as np
+import pytest
+
+import pyoracle
+import exact_riemann
+from sparkhip import problems
+from sparkhip.capi import VI
+
+G = 1.4
+
+
+def one_d_problem(nblk, n, ng, x0=0.0, dx=None, fields=None, **kw):
+    """1D uniform block grid over [0, 1] (or cells of width dx from x0)."""
+    cfg = problems.base_cfg(ndim=1, nguard=ng, **kw)
+    N = nblk * n
+    dx = dx or 1.0 / N
+    blocks, arrays = [], []
+    for b in range(nblk):
+        blk = dict(nx=n, ny=1, nz=1, level=1, dx=dx, dy=1.0, dz=1.0)
+        x = x0 + (b * n + np.arange(n + 2 * ng) - ng + 0.5) * dx
+        d, u, p = fields(x)
+        U = np.zeros((1, 1, n + 2 * ng, 11 + cfg["nscalars"]))
+        z = np.zeros_like(x)
+        problems.fill_thermo(U, d, p, u, z, z, cfg["gamma"])
+        blocks.append(blk)
+        arrays.append(U)
+    return problems.Problem("1d", cfg, blocks, arrays, ug=(nblk, 1, 1))
+
+
+def interior(prob, arr, var):
+    g = prob.cfg["nguard"]
+    nd = prob.cfg["ndim"]
+    sl = [slice(g, -g) if d < nd else slice(None) for d in range(3)][::-1]
+    return arr[tuple(sl) + (VI[var] if isinstance(var, str) else var,)]
+
+
+# --- RK tables: exact polynomial on a linear problem --------------------------
+@pytest.mark.parametrize("rk", [2, 3])
+def test_rk_stages_give_taylor_polynomial(rk):
+    """Supersonic uniform flow (u = 5, p = 1) with FOG reconstruction: HLLC
+    is the upwind flux and the mass equation is the linear upwind operator
+    L r_i = -(u/dx)(r_i - r_{i-1}); SSP-RK2/3 must give sum_k (dt L)^k / k!."""
+    n, ng = 32, 6
+    fields = lambda x: (1.0 + 0.2 * np.sin(2 * np.pi * x), np.full_like(x, 5.0), np.ones_like(x))
+    prob = one_d_problem(1, n, ng, fields=fields, recon="fog", nstencil=2, rk_order=rk)
+    U = prob.arrays[0].copy()
+    r0 = U[0, 0, :, VI["dens"]].copy()
+    dt, _ = pyoracle.compute_dt(prob.cfg, prob.blocks[0], U)
+    pyoracle.hydro_block(prob.cfg, prob.blocks[0], U, dt)
+    c = 5.0 * dt / prob.blocks[0]["dx"]
+    D = lambda r: r - np.concatenate(([np.nan], r[:-1]))      # (I - S) r
+    terms = [r0, -c * D(r0), c * c / 2 * D(D(r0))]
+    if rk == 3:
+        terms.append(-c ** 3 / 6 * D(D(D(r0))))
+    expect = sum(terms)
+    got = U[0, 0, :, VI["dens"]]
+    np.testing.assert_allclose(got[ng:-ng], expect[ng:-ng], rtol=1e-13)
+    # velocity and pressure stay uniform
+    np.testing.assert_allclose(U[0, 0, ng:-ng, VI["velx"]], 5.0, rtol=1e-13)
+    np.testing.assert_allclose(U[0, 0, ng:-ng, VI["pres"]], 1.0, rtol=1e-12)
+
+
+def test_flux_register_weights_reproduce_the_step():
+    """With fluxBuf = sum_s w_s F_s (w = 1/6, 1/6, 2/3), the RK3 step equals
+    U - dt/dx (fbuf(i+1) - fbuf(i)) for the conserved mass."""
+    n, ng = 16, 10
+    fields = lambda x: (1.0 + 0.3 * np.exp(-((x - 0.5) / 0.1) ** 2), 0.2 + 0.1 * np.sin(6 * x), 1.0 + 0.5 * x)
+    prob = one_d_problem(1, n, ng, fields=fields)
+    U = prob.arrays[0].copy()
+    r0 = U[0, 0, ng:-ng, VI["dens"]].copy()
+    dt, _ = pyoracle.compute_dt(prob.cfg, prob.blocks[0], U)
+    fb = [np.zeros((n + 1) * 5), np.zeros(n * 5), np.zeros(n * 5)]
+    pyoracle.hydro_block(prob.cfg, prob.blocks[0], U, dt, fb=fb)
+    F = fb[0].reshape(n + 1, 5)[:, 0]
+    dx = prob.blocks[0]["dx"]
+    np.testing.assert_allclose(U[0, 0, ng:-ng, VI["dens"]], r0 - dt / dx * (F[1:] - F[:-1]), rtol=1e-13)
+
+
+# --- flattening / shock detector / dt ----------------------------------------------
+def block3(n=4, ng=4, **kw):
+    cfg = problems.base_cfg(ndim=3, nguard=ng, rk_order=2, **kw)
+    b = dict(nx=n, ny=n, nz=n, level=1, dx=0.1, dy=0.2, dz=0.4)
+    shape = (n + 2 * ng,) * 3
+    return cfg, b, shape
+
+
+def run_flattening(cfg, b, U):
+    import ctypes
+    P, B = pyoracle.params(cfg), pyoracle.block(b)
+    g = cfg["nguard"]
+    lim = (ctypes.c_int * 6)(g, g + b["nx"] - 1, g, g + b["ny"] - 1, g, g + b["nz"] - 1)
+    flat = np.full(U.shape[:3], -1.0)
+    pyoracle.lib().orc_flattening(ctypes.byref(P), ctypes.byref(B), pyoracle.dptr(U), lim, pyoracle.dptr(flat))
+    return flat
+
+
+def test_flattening_special_cases():
+    cfg, b, shape = block3()
+    k, j, i = np.meshgrid(*(np.arange(s) for s in shape), indexing="ij")
+    g = cfg["nguard"]
+    # linear pressure: beta = 1/2 -> flat = 1
+    U = np.zeros(shape + (11,))
+    problems.fill_thermo(U, np.ones(shape), 1.0 + 0.01 * i, -0.01 * i, 0 * i, 0 * i, G)
+    f = run_flattening(cfg, b, U)
+    assert np.all(f[g - 1:-g + 1, g - 1:-g + 1, g - 1:-g + 1] == 1.0)
+    # a pressure jump 10 -> 1 between i = 5 and 6 with converging flow: beta = 1
+    # gives f~ = 0, so flat = min(1, (0.75 - Z)/0.5) with Z = 9/(gamma p_i)
+    p = np.where(i < 6, 10.0, 1.0)
+    u = np.where(i < 6, 1.0, -1.0)
+    problems.fill_thermo(U, np.ones(shape), p, u, 0 * i, 0 * i, G)
+    f = run_flattening(cfg, b, U)
+    np.testing.assert_allclose(f[g:-g, g:-g, 5], (0.75 - 9.0 / (G * 10.0)) / 0.5, rtol=1e-15)
+    # at i = 6 the compression test looks at u(7) < u(6): false, so f~ = 1
+    assert np.all(f[g:-g, g:-g, 6] == 1.0)
+    assert np.all(f[g:-g, g:-g, 3] == 1.0) and np.all(f[g:-g, g:-g, 8] == 1.0)  # beta = 0 away from the jump
+    # the same jump with diverging flow is not flattened
+    problems.fill_thermo(U, np.ones(shape), p, -u, 0 * i, 0 * i, G)
+    f = run_flattening(cfg, b, U)
+    assert np.all(f[g - 1:-g + 1, g - 1:-g + 1, g - 1:-g + 1] == 1.0)
+
+
+def test_shock_detector_special_cases():
+    cfg, b, shape = block3()
+    k, j, i = np.meshgrid(*(np.arange(s) for s in shape), indexing="ij")
+    U = np.zeros(shape + (11,))
+    problems.fill_thermo(U, np.ones(shape), np.ones(shape), 0 * i + 0.3, 0 * i, 0 * i, G)
+    pyoracle.shock_detect(cfg, b, U)
+    assert np.all(U[..., VI["shok"]] == 0.0)
+    p = np.where(i < 6, 10.0, 1.0)
+    u = np.where(i < 6, 2.0, -2.0)
+    problems.fill_thermo(U, np.ones(shape), p, u, 0 * i, 0 * i, G)
+    pyoracle.shock_detect(cfg, b, U)
+    s = U[..., VI["shok"]]
+    assert np.all(s[1:-1, 1:-1, 5:7] == 1.0) and np.all(s[:, :, :4] == 0.0) and np.all(s[:, :, 8:] == 0.0)
+
+
+def test_cfl_dt_closed_form():
+    cfg, b, shape = block3()
+    U = np.zeros(shape + (11,))
+    one = np.ones(shape)
+    problems.fill_thermo(U, 2.0 * one, 3.0 * one, 0.5 * one, -1.0 * one, 0.25 * one, G)
+    dt, loc = pyoracle.compute_dt(cfg, b, U)
+    c = np.sqrt(G * 3.0 / 2.0)
+    expect = cfg["cfl"] / max((0.5 + c) / 0.1, (1.0 + c) / 0.2, (0.25 + c) / 0.4)
+    np.testing.assert_allclose(dt, expect, rtol=1e-15)
+    assert loc == [4, 4, 4]   # first interior cell in (k, j, i) order
+
+
+def test_flux_correction_signs_and_closed_form():
+    cfg, b, shape = block3()
+    U = np.zeros(shape + (11,))
+    one = np.ones(shape)
+    problems.fill_thermo(U, 1.5 * one, 2.0 * one, 0.0 * one, 0.0 * one, 0.0 * one, G)
+    V = U.copy()
+    n, nf = 4, 5
+    fbx = np.zeros((n, n, n + 1, nf))
+    fbx[1, 2, 0, 0] = 0.3      # low x face of interior cell (i=0, j=2, k=1)
+    fbx[1, 2, n, 0] = 0.2      # high x face of interior cell (i=3, j=2, k=1)
+    fby, fbz = np.zeros(n * (n + 1) * n * nf), np.zeros(n * n * (n + 1) * nf)
+    dt = 0.01
+    pyoracle.correct_fluxes(cfg, b, V, dt, [fbx.ravel(), fby, fbz])
+    g = cfg["nguard"]
+    dens = V[..., VI["dens"]]
+    np.testing.assert_allclose(dens[g + 1, g + 2, g + 0], 1.5 + dt / 0.1 * 0.3, rtol=1e-14)  # low: -(-buf)
+    np.testing.assert_allclose(dens[g + 1, g + 2, g + 3], 1.5 - dt / 0.1 * 0.2, rtol=1e-14)  # high: -(+buf)
+    untouched = np.ones(shape, bool)
+    untouched[g + 1, g + 2, g + 0] = untouched[g + 1, g + 2, g + 3] = False
+    np.testing.assert_allclose(dens[untouched], 1.5, rtol=1e-15)
+
+
+# --- whole-solver invariants -----------------------------------------------------
+def test_uniform_state_is_a_fixed_point():
+    prob = problems.sedov3d(nb=1, n=8, nguard=10, p_amb=1.0, E=0.0, r_cells=0.0)
+    for a in prob.arrays:
+        a[..., VI["velx"]] = 0.3
+        a[..., VI["ener"]] += 0.045
+    ref = prob.arrays[0].copy()
+    pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, prob.arrays, 3)
+    for v in ("dens", "velx", "pres", "ener"):
+        np.testing.assert_allclose(interior(prob, prob.arrays[0], v), interior(prob, ref, v), rtol=1e-14)
+
+
+def test_mass_and_energy_conserved_before_waves_reach_the_boundary():
+    prob2 = problems.sedov3d(nb=2, n=16, nguard=10, p_amb=0.1, E=0.05, r_cells=3.5)
+    vol = prob2.blocks[0]["dx"] ** 3
+    tot = lambda p: (sum(interior(p, a, "dens").sum() for a in p.arrays) * vol,
+                     sum((interior(p, a, "dens") * interior(p, a, "ener")).sum() for a in p.arrays) * vol)
+    m0, e0 = tot(prob2)
+    pyoracle.ug_advance(prob2.cfg, prob2.blocks[0], prob2.ug, prob2.arrays, 5)
+    m1, e1 = tot(prob2)
+    np.testing.assert_allclose(m1, m0, rtol=1e-13)
+    np.testing.assert_allclose(e1, e0, rtol=1e-13)
+
+
+def test_diagonal_sod_is_mirror_symmetric():
+    prob = problems.sod2d(nbx=2, nby=2, n=16, nguard=10)
+    pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, prob.arrays, 8)
+    g = 10
+    full = lambda v: np.block([[interior(prob, prob.arrays[bj * 2 + bi], v)[0] for bi in range(2)] for bj in range(2)])
+    d = full("dens")
+    # the problem is symmetric under the reflection (x, y) -> (y, x)
+    np.testing.assert_allclose(d, d.T, rtol=1e-13)
+    np.testing.assert_allclose(full("velx"), full("vely").T, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(full("pres"), full("pres").T, rtol=1e-13)
+
+
+def test_refill_and_telescope_agree_when_boundary_flow_is_uniform():
+    a = problems.sedov3d(nb=2, n=16, nguard=6, guard_mode="refill")
+    b = problems.sedov3d(nb=2, n=16, nguard=10)
+    pyoracle.ug_advance(a.cfg, a.blocks[0], a.ug, a.arrays, 3, refill=True)
+    pyoracle.ug_advance(b.cfg, b.blocks[0], b.ug, b.arrays, 3, refill=False)
+    for x, y in zip(a.arrays, b.arrays):
+        for v in ("dens", "velx", "pres", "ener", "eint"):
+            assert np.array_equal(interior(a, x, v), interior(b, y, v))
+
+
+# --- Sod shock tube against the exact Riemann solution ---------------------------
+def test_exact_riemann_reference_values():
+    """Toro (2009) table 4.2, test 1 star state."""
+    p, u = exact_riemann.star_state(1.0, 0.0, 1.0, 0.125, 0.0, 0.1, 1.4)
+    np.testing.assert_allclose([p, u], [0.30313, 0.92745], atol=5e-6)
+
+
+def sod_l1(N):
+    n = 16
+    fields = lambda x: (np.where(x < 0.5, 1.0, 0.125), np.zeros_like(x), np.where(x < 0.5, 1.0, 0.1))
+    prob = one_d_problem(N // n, n, 10, fields=fields)
+    t = 0.0
+    while t < 0.2:
+        dts, _ = pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, prob.arrays, 1)
+        t += dts[0]
+    x = (np.arange(N) + 0.5) / N
+    rho = np.concatenate([interior(prob, a, "dens").ravel() for a in prob.arrays])
+    ex = exact_riemann.sample(x, t, 0.5, 1.0, 0.0, 1.0, 0.125, 0.0, 0.1, 1.4)
+    return np.mean(np.abs(rho - ex[0]))
+
+
+def test_sod_converges_to_exact_solution():
+    e128, e256 = sod_l1(128), sod_l1(256)
+    assert e256 < 6e-3, e256
+    assert e128 / e256 > 1.4, (e128, e256)
