@@ -124,7 +124,8 @@ PlainLaunch make_launch(spark_batch* b, const ShapeClass& c) {
   L.nblk = (int)c.ids.size();
   L.nx = c.nx; L.ny = c.ny; L.nz = c.nz;
   L.U0 = b->U;
-  L.S = b->S;
+  L.Sin = b->S;
+  L.Sout = b->S;
   L.flat = b->flat;
   for (int d = 0; d < 3; d++) { L.fl[d] = b->fl[d]; L.fb[d] = b->p.flux_correct ? b->fb[d] : nullptr; }
   L.weight = 0.;
@@ -153,42 +154,63 @@ int run_dt(spark_batch* b, int hist_slot) {
   return SPARK_OK;
 }
 
-int run_fill(spark_batch* b, double* pool) {
+// Guard fill of a uniform block grid.  full: every variable over every
+// guard layer (Grid_fillGuardCells semantics).  Otherwise only what the
+// stages read: dens eint ener gamc pres vel(3) and the scalars, over the
+// guard layers the next stage's stencils reach.
+int run_fill(spark_batch* b, double* pool, bool full, int depth) {
   PlainLaunch L = make_launch(b, b->classes[0]);
   L.blist = nullptr;
   L.nblk = b->nblk;
-  HIPCHK(spark::plain_ug_fill(L, b->ug[0], b->ug[1], b->ug[2], pool, b->nvar));
+  spark::VarList vl;
+  vl.n = 0;
+  if (full) {
+    for (int v = 0; v < b->nvar; v++) vl.v[vl.n++] = v;
+    depth = b->p.nguard;
+  } else {
+    const int need[8] = {spark::V_DENS, spark::V_EINT, spark::V_ENER, spark::V_GAMC,
+                         spark::V_PRES, spark::V_VELX, spark::V_VELY, spark::V_VELZ};
+    for (int v : need) vl.v[vl.n++] = v;
+    for (int s = 0; s < b->p.nscalars; s++) vl.v[vl.n++] = spark::V_NPROP + s;
+  }
+  HIPCHK(spark::plain_ug_fill(L, b->ug[0], b->ug[1], b->ug[2], pool, vl, depth));
   return SPARK_OK;
 }
 
+// first-stage guard reach: stencil extent of stage 1 beyond the interior
+int stage1_reach(const spark_batch* b) {
+  return (b->p.guard_mode == SPARK_GUARD_REFILL ? 0 : b->limits_array[0]) + b->p.nstencil;
+}
+
 int run_step_begin(spark_batch* b) {
+  // SHOK over the cells stage 1 reads (the reference computes NGUARD-1
+  // layers; the outer layers are never read before the next refill)
+  const int band = (b->p.guard_mode == SPARK_GUARD_REFILL ? 0 : b->limits_array[0]) + 1;
   for (auto& c : b->classes) {
     PlainLaunch L = make_launch(b, c);
-    HIPCHK(spark::plain_shock_detect(L, b->U, b->flat));
+    HIPCHK(spark::plain_shock_detect(L, b->U, b->flat, band));
   }
-  HIPCHK(b->prof.begin(g_stream, "saveState_copy", 16.0 * b->pool, 0.0));
-  HIPCHK(hipMemcpyAsync(b->S, b->U, sizeof(double) * b->pool, hipMemcpyDeviceToDevice, g_stream));
-  HIPCHK(b->prof.end(g_stream));
   b->step_open = true;
   return SPARK_OK;
 }
 
+// saveState's copy is folded away: stage 1 reads the solution U directly
+// (hy_starState == Uin at stage 1) and the last stage writes U directly
+// (updateState copies exactly the last stage's region, the interior).
 int run_stage(spark_batch* b, int stage, int ngoff) {
   const double* co = b->coeff[stage - 1];
   for (auto& c : b->classes) {
     PlainLaunch L = make_launch(b, c);
     L.weight = b->weights[stage - 1];
     L.add_flux = stage > 1;
+    L.Sin = (stage == 1) ? b->U : b->S;
+    L.Sout = (stage == b->last_stage) ? b->U : b->S;
     HIPCHK(spark::plain_stage(L, ngoff, co[0], co[1], co[2], stage == b->last_stage));
   }
   return SPARK_OK;
 }
 
 int run_step_end(spark_batch* b) {
-  for (auto& c : b->classes) {
-    PlainLaunch L = make_launch(b, c);
-    HIPCHK(spark::plain_copy_interior(L, b->S, b->U, b->nvar));
-  }
   b->step_open = false;
   return SPARK_OK;
 }
@@ -200,7 +222,7 @@ int run_full_step(spark_batch* b) {
   for (int s = 1; s <= b->last_stage; s++) {
     if ((rc = run_stage(b, s, refill ? 0 : b->limits_array[s - 1]))) return rc;
     if (refill && s < b->last_stage)
-      if ((rc = run_fill(b, b->S))) return rc;
+      if ((rc = run_fill(b, b->S, false, b->p.nstencil))) return rc;
   }
   return run_step_end(b);
 }
@@ -505,7 +527,7 @@ int spark_set_uniform_grid(spark_batch* b, int nbx, int nby, int nbz) {
 int spark_fill_guardcells(spark_batch* b) {
   if (!b) return fail(SPARK_ERR_ARG, "spark_fill_guardcells: null batch");
   if (!b->has_ug) return fail(SPARK_ERR_STATE, "spark_fill_guardcells: no uniform grid declared");
-  return run_fill(b, b->U);
+  return run_fill(b, b->U, true, b->p.nguard);
 }
 
 int spark_compute_dt(spark_batch* b, double* dt, int minloc[5]) {
@@ -573,7 +595,7 @@ int spark_advance(spark_batch* b, int nsteps, double* dts) {
   for (int s = 0; s < nsteps; s++) {
     int rc = run_dt(b, s);
     if (rc) return rc;
-    if (b->has_ug && (rc = run_fill(b, b->U))) return rc;
+    if (b->has_ug && (rc = run_fill(b, b->U, false, stage1_reach(b)))) return rc;
     if ((rc = run_full_step(b))) return rc;
   }
   if (dts && nsteps > 0) {
@@ -658,6 +680,7 @@ int spark_profile(spark_batch* b, int enable) {
   if (!b) return fail(SPARK_ERR_ARG, "spark_profile: null batch");
   HIPCHK(hipStreamSynchronize(g_stream));
   prof_clear(b);
+  b->prof.err = hipSuccess;
   b->prof.on = enable != 0;
   return SPARK_OK;
 }
@@ -665,6 +688,7 @@ int spark_profile(spark_batch* b, int enable) {
 int spark_profile_read(spark_batch* b, spark_kernel_stat* out, int max, int* n) {
   if (!b || !out || !n || max < 0) return fail(SPARK_ERR_ARG, "spark_profile_read: bad argument");
   HIPCHK(hipStreamSynchronize(g_stream));
+  HIPCHK(b->prof.err);
   std::vector<spark_kernel_stat> acc;
   for (auto& r : b->prof.recs) {
     float ms = 0.f;

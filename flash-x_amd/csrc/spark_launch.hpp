@@ -19,19 +19,21 @@ struct ProfRec {
 };
 struct Prof {
   bool on = false;
+  hipError_t err = hipSuccess;  // first event error, reported when read
   std::vector<ProfRec> recs;
-  hipError_t begin(hipStream_t st, const char* name, double bytes, double flops) {
-    if (!on) return hipSuccess;
+  void begin(hipStream_t st, const char* name, double bytes, double flops) {
+    if (!on) return;
     ProfRec r{name, nullptr, nullptr, bytes, flops};
     hipError_t e = hipEventCreate(&r.e0);
     if (e == hipSuccess) e = hipEventCreate(&r.e1);
     if (e == hipSuccess) e = hipEventRecord(r.e0, st);
+    if (e != hipSuccess && err == hipSuccess) err = e;
     recs.push_back(r);
-    return e;
   }
-  hipError_t end(hipStream_t st) {
-    if (!on || recs.empty()) return hipSuccess;
-    return hipEventRecord(recs.back().e1, st);
+  void end(hipStream_t st) {
+    if (!on || recs.empty()) return;
+    hipError_t e = hipEventRecord(recs.back().e1, st);
+    if (e != hipSuccess && err == hipSuccess) err = e;
   }
 };
 
@@ -44,8 +46,9 @@ struct PlainLaunch {
   const int* blist;       // device list of block ids in this shape class
   int nblk;               // length of blist
   int nx, ny, nz;         // interior shape of the class
-  const double* U0;       // step input (hy_tmpState role)
-  double* S;              // stage state (hy_starState role)
+  const double* U0;       // step input (hy_tmpState role); also holds SHOK
+  const double* Sin;      // stage input state (U at stage 1, S after)
+  double* Sout;           // stage output state (U at the last stage, S before)
   double* flat;           // flattening coefficients / scratch (cell pool)
   double* fl[3];          // face-flux pools
   double* fb[3];          // flux-register pools (nullptr when not correcting)
@@ -57,14 +60,18 @@ struct PlainLaunch {
 };
 
 hipError_t plain_stage(const PlainLaunch& L, int ngoff, double c1, double c2, double c3, int final_stage);
-hipError_t plain_copy_interior(const PlainLaunch& L, const double* S, double* U, int nvar);
-hipError_t plain_shock_detect(const PlainLaunch& L, double* U, double* scratch);
+hipError_t plain_shock_detect(const PlainLaunch& L, double* U, double* scratch, int band);
 hipError_t plain_dt_part(const PlainLaunch& L, const double* U, void* parts, int chunks);
 hipError_t plain_dt_final(hipStream_t st, const DevParams& P, const void* parts, int nblk_total, int chunks,
                           double* dt_out, long long* loc_out, double* dt_hist, int hist_slot);
 hipError_t plain_set_scalar(hipStream_t st, double* p, double v);
 size_t plain_dt_part_size();
-hipError_t plain_ug_fill(const PlainLaunch& L, int nbx, int nby, int nbz, double* U, int nvar);
+// variables a guard fill copies (canonical indices)
+struct VarList {
+  int n;
+  int v[24];
+};
+hipError_t plain_ug_fill(const PlainLaunch& L, int nbx, int nby, int nbz, double* U, const VarList& vars, int depth);
 hipError_t plain_correct(const PlainLaunch& L, double* U);
 hipError_t plain_aos_to_soa(hipStream_t st, const BlockMeta* meta, int nblk, int maxcells, const double* aos,
                             const int64_t* aos_off, int nunk, const int* var_index, int nvar, double* U);

@@ -77,8 +77,8 @@ __global__ void __launch_bounds__(256) k_flatten(DevParams P, const BlockMeta* _
 template <int DIR, int RECON, int RIEMANN, int NSM>
 __global__ void __launch_bounds__(1024) k_flux(DevParams P, const BlockMeta* __restrict__ meta,
                                                const int* __restrict__ blist, const double* __restrict__ S,
-                                               const double* __restrict__ flat, double* __restrict__ fl, int ngoff,
-                                               int lanes, int* __restrict__ err) {
+                                               const double* __restrict__ Ushok, const double* __restrict__ flat,
+                                               double* __restrict__ fl, int ngoff, int lanes, int* __restrict__ err) {
   constexpr int R = ReconRadius<RECON>::R;
   constexpr int NRM = R_NUM + NSM;
   constexpr int NFM = F_NUM + NSM;
@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(1024) k_flux(DevParams P, const BlockMeta* __r
     umr[R_RHOE] = um;
     sm[(NRM + 4) * NT + tid] = u[2];
     sm[(NRM + 0) * NT + tid] = ud[2];
-    sm[(NRM + 5) * NT + tid] = sp[V_SHOK * cs + c];
+    sm[(NRM + 5) * NT + tid] = Ushok[m.base + V_SHOK * cs + c];
 #pragma unroll
     for (int v = 0; v < NSM; v++) {
       if (v < ns) {
@@ -207,8 +207,8 @@ __global__ void __launch_bounds__(1024) k_flux(DevParams P, const BlockMeta* __r
 // Gamma EOS in MODE_DENS_EI (Eos_wrapped over the same limits).
 template <int NSM>
 __global__ void __launch_bounds__(256) k_update(DevParams P, const BlockMeta* __restrict__ meta,
-                                                const int* __restrict__ blist, const double* __restrict__ U0,
-                                                double* __restrict__ S, const double* __restrict__ flx,
+                                                const int* __restrict__ blist, const double* U0,
+                                                const double* Sin, double* Sout, const double* __restrict__ flx,
                                                 const double* __restrict__ fly, const double* __restrict__ flz,
                                                 const double* __restrict__ dtp, double c1, double c2, double c3,
                                                 int ngoff, int final_stage) {
@@ -221,8 +221,11 @@ __global__ void __launch_bounds__(256) k_update(DevParams P, const BlockMeta* __
   const double dt = *dtp;
   const double dx = m.dx, dy = m.dy, dz = m.dz;
   const int64_t cs = m.cstride, fs = m.fstride;
+  // U0, Sin and Sout may alias (stage 1 reads U twice; the last stage
+  // writes U): every thread reads and writes only its own cell
   const double* u0 = U0 + m.base;
-  double* us = S + m.base;
+  const double* ui = Sin + m.base;
+  double* us = Sout + m.base;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
     const int i = L.lo[0] + t % ex, j = L.lo[1] + (t / ex) % ey, k = L.lo[2] + t / (ex * ey);
     const int64_t c = cix(m, i, j, k);
@@ -233,17 +236,17 @@ __global__ void __launch_bounds__(256) k_update(DevParams P, const BlockMeta* __
     A[F_YMOM] = d0 * u0[V_VELY * cs + c];
     A[F_ZMOM] = d0 * u0[V_VELZ * cs + c];
     A[F_ENER] = d0 * u0[V_ENER * cs + c];
-    const double ds = us[V_DENS * cs + c];
+    const double ds = ui[V_DENS * cs + c];
     B[F_MASS] = ds;
-    B[F_XMOM] = ds * us[V_VELX * cs + c];
-    B[F_YMOM] = ds * us[V_VELY * cs + c];
-    B[F_ZMOM] = ds * us[V_VELZ * cs + c];
-    B[F_ENER] = ds * us[V_ENER * cs + c];
+    B[F_XMOM] = ds * ui[V_VELX * cs + c];
+    B[F_YMOM] = ds * ui[V_VELY * cs + c];
+    B[F_ZMOM] = ds * ui[V_VELZ * cs + c];
+    B[F_ENER] = ds * ui[V_ENER * cs + c];
 #pragma unroll
     for (int v = 0; v < NSM; v++) {
       if (v < ns) {
         A[F_NUM + v] = u0[(V_NPROP + v) * cs + c] * d0;
-        B[F_NUM + v] = us[(V_NPROP + v) * cs + c] * ds;
+        B[F_NUM + v] = ui[(V_NPROP + v) * cs + c] * ds;
       }
     }
     double Sg[NFM];
@@ -294,43 +297,31 @@ __global__ void __launch_bounds__(256) k_update(DevParams P, const BlockMeta* __
 }
 
 // ---------------------------------------------------------------------------
-// updateState (Hydro_funcs.F90-mc:573-574): interior of S -> U, all vars
-__global__ void __launch_bounds__(256) k_copy_interior(DevParams P, const BlockMeta* __restrict__ meta,
-                                                       const int* __restrict__ blist, const double* __restrict__ S,
-                                                       double* __restrict__ U, int nvar) {
-  const BlockMeta m = meta[blist[blockIdx.y]];
-  Lims L = stage_lims(P, m, 0);
-  const int ex = L.hi[0] - L.lo[0] + 1, ey = L.hi[1] - L.lo[1] + 1, ez = L.hi[2] - L.lo[2] + 1;
-  const int total = ex * ey * ez;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-    const int i = L.lo[0] + t % ex, j = L.lo[1] + (t / ex) % ey, k = L.lo[2] + t / (ex * ey);
-    const int64_t c = m.base + cix(m, i, j, k);
-    for (int v = 0; v < nvar; v++) U[c + v * m.cstride] = S[c + v * m.cstride];
-  }
-}
-
-// ---------------------------------------------------------------------------
 // shockDetect (Hydro_funcs.F90-mc:600-706): pass 1 zeroes SHOK and stores
 // the sound speed over the guard-celled box, pass 2 flags shocks over
 // interior +/- (NGUARD-1).
 __global__ void __launch_bounds__(256) k_shock_vc(DevParams P, const BlockMeta* __restrict__ meta,
                                                   const int* __restrict__ blist, double* __restrict__ U,
-                                                  double* __restrict__ vc) {
+                                                  double* __restrict__ vc, int band) {
   const BlockMeta m = meta[blist[blockIdx.y]];
-  const int total = m.NX * m.NY * m.NZ;
+  Lims L = stage_lims(P, m, band + 1);
+  const int ex = L.hi[0] - L.lo[0] + 1, ey = L.hi[1] - L.lo[1] + 1, ez = L.hi[2] - L.lo[2] + 1;
+  const int total = ex * ey * ez;
   double* up = U + m.base;
   const int64_t cs = m.cstride;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-    up[V_SHOK * cs + t] = 0.;
-    vc[m.obase + t] = Num<double>::sqr(up[V_GAMC * cs + t] * up[V_PRES * cs + t] / fmax2(up[V_DENS * cs + t], P.tiny));
+    const int i = L.lo[0] + t % ex, j = L.lo[1] + (t / ex) % ey, k = L.lo[2] + t / (ex * ey);
+    const int64_t c = cix(m, i, j, k);
+    up[V_SHOK * cs + c] = 0.;
+    vc[m.obase + c] = Num<double>::sqr(up[V_GAMC * cs + c] * up[V_PRES * cs + c] / fmax2(up[V_DENS * cs + c], P.tiny));
   }
 }
 
 __global__ void __launch_bounds__(256) k_shock(DevParams P, const BlockMeta* __restrict__ meta,
                                                const int* __restrict__ blist, double* __restrict__ U,
-                                               const double* __restrict__ vc) {
+                                               const double* __restrict__ vc, int band) {
   const BlockMeta m = meta[blist[blockIdx.y]];
-  Lims L = stage_lims(P, m, P.ng - 1);
+  Lims L = stage_lims(P, m, band);
   const int ex = L.hi[0] - L.lo[0] + 1, ey = L.hi[1] - L.lo[1] + 1, ez = L.hi[2] - L.lo[2] + 1;
   const int total = ex * ey * ez;
   double* up = U + m.base;
@@ -463,16 +454,19 @@ __global__ void __launch_bounds__(256) k_dt_final(DevParams P, const DtPart* __r
 // cell takes the value at its global index clamped into the domain.
 // Blocks are numbered x-fastest over an nb[0] x nb[1] x nb[2] grid.
 __global__ void __launch_bounds__(256) k_ug_fill(DevParams P, const BlockMeta* __restrict__ meta, int nbx,
-                                                 int nby, int nbz, double* __restrict__ U, int nvar) {
+                                                 int nby, int nbz, double* __restrict__ U, VarList vars, int depth) {
   const int bid = blockIdx.y;
   const BlockMeta m = meta[bid];
-  const int total = m.NX * m.NY * m.NZ;
   const int n[3] = {m.nx, (P.ndim > 1) ? m.ny : 1, (P.ndim > 2) ? m.nz : 1};
   const int gg[3] = {P.ng, (P.ndim > 1) ? P.ng : 0, (P.ndim > 2) ? P.ng : 0};
+  const int dd[3] = {depth, (P.ndim > 1) ? depth : 0, (P.ndim > 2) ? depth : 0};
+  const int ext[3] = {n[0] + 2 * dd[0], n[1] + 2 * dd[1], n[2] + 2 * dd[2]};
+  const int total = ext[0] * ext[1] * ext[2];
   const int nb[3] = {nbx, nby, nbz};
   const int bi[3] = {bid % nbx, (bid / nbx) % nby, bid / (nbx * nby)};
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-    const int loc[3] = {t % m.NX, (t / m.NX) % m.NY, t / (m.NX * m.NY)};
+    const int loc[3] = {gg[0] - dd[0] + t % ext[0], gg[1] - dd[1] + (t / ext[0]) % ext[1],
+                        gg[2] - dd[2] + t / (ext[0] * ext[1])};
     bool inside = true;
     int sb[3], sl[3];
 #pragma unroll
@@ -487,8 +481,8 @@ __global__ void __launch_bounds__(256) k_ug_fill(DevParams P, const BlockMeta* _
     if (inside) continue;
     const BlockMeta ms = meta[sb[0] + nbx * (sb[1] + nby * sb[2])];
     const int64_t src = ms.base + cix(ms, sl[0], sl[1], sl[2]);
-    const int64_t dst = m.base + t;
-    for (int v = 0; v < nvar; v++) U[dst + v * m.cstride] = U[src + v * ms.cstride];
+    const int64_t dst = m.base + cix(m, loc[0], loc[1], loc[2]);
+    for (int q = 0; q < vars.n; q++) U[dst + vars.v[q] * m.cstride] = U[src + vars.v[q] * ms.cstride];
   }
 }
 
@@ -661,8 +655,8 @@ static hipError_t launch_flux_t(const PlainLaunch& L, int ngoff) {
     static const char* names[3] = {"k_flux_x", "k_flux_y", "k_flux_z"};
     L.prof->begin(L.stream, names[DIR], bytes, flops);
   }
-  hipLaunchKernelGGL((k_flux<DIR, RECON, RIEMANN, NSM>), grid, block, shm, L.stream, L.P, L.meta, L.blist, L.S,
-                     L.flat, L.fl[DIR], ngoff, lanes, L.err);
+  hipLaunchKernelGGL((k_flux<DIR, RECON, RIEMANN, NSM>), grid, block, shm, L.stream, L.P, L.meta, L.blist, L.Sin,
+                     L.U0, L.flat, L.fl[DIR], ngoff, lanes, L.err);
   if (L.prof) L.prof->end(L.stream);
   return hipGetLastError();
 }
@@ -702,7 +696,7 @@ hipError_t plain_stage(const PlainLaunch& L, int ngoff, double c1, double c2, do
                     (double)L.nblk * bx * 25.0 * L.P.ndim);
     }
     hipLaunchKernelGGL(k_flatten, dim3(grid_x_for(boxf, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta, L.blist,
-                       L.S, L.flat, ngoff);
+                       L.Sin, L.flat, ngoff);
     if (L.prof) L.prof->end(L.stream);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
@@ -733,36 +727,30 @@ hipError_t plain_stage(const PlainLaunch& L, int ngoff, double c1, double c2, do
   }
   if (L.P.ns == 0)
     hipLaunchKernelGGL(k_update<0>, dim3(grid_x_for(box, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta, L.blist,
-                       L.U0, L.S, L.fl[0], L.fl[1], L.fl[2], L.dtp, c1, c2, c3, ngoff, final_stage);
+                       L.U0, L.Sin, L.Sout, L.fl[0], L.fl[1], L.fl[2], L.dtp, c1, c2, c3, ngoff, final_stage);
   else
     hipLaunchKernelGGL(k_update<8>, dim3(grid_x_for(box, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta, L.blist,
-                       L.U0, L.S, L.fl[0], L.fl[1], L.fl[2], L.dtp, c1, c2, c3, ngoff, final_stage);
+                       L.U0, L.Sin, L.Sout, L.fl[0], L.fl[1], L.fl[2], L.dtp, c1, c2, c3, ngoff, final_stage);
   if (L.prof) L.prof->end(L.stream);
   return hipGetLastError();
 }
 
-hipError_t plain_copy_interior(const PlainLaunch& L, const double* S, double* U, int nvar) {
-  int box = L.nx * (L.P.ndim > 1 ? L.ny : 1) * (L.P.ndim > 2 ? L.nz : 1);
-  if (L.prof) L.prof->begin(L.stream, "k_copy_interior", 16.0 * L.nblk * (double)box * nvar, 0.0);
-  hipLaunchKernelGGL(k_copy_interior, dim3(grid_x_for(box, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta,
-                     L.blist, S, U, nvar);
-  if (L.prof) L.prof->end(L.stream);
-  return hipGetLastError();
-}
-
-hipError_t plain_shock_detect(const PlainLaunch& L, double* U, double* scratch) {
-  const int NX = L.nx + 2 * L.P.ng, NY = L.P.ndim > 1 ? L.ny + 2 * L.P.ng : 1,
-            NZ = L.P.ndim > 2 ? L.nz + 2 * L.P.ng : 1;
-  const double nc = (double)NX * NY * NZ * L.nblk;
-  if (L.prof) L.prof->begin(L.stream, "k_shock_vc", 8.0 * nc * 5, nc * 6);
-  hipLaunchKernelGGL(k_shock_vc, dim3(grid_x_for(NX * NY * NZ, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta,
-                     L.blist, U, scratch);
+// band: SHOK is computed over interior +/- band (the reference uses
+// NGUARD-1; only the cells the stage-1 face fluxes read are needed).
+hipError_t plain_shock_detect(const PlainLaunch& L, double* U, double* scratch, int band) {
+  const int n[3] = {L.nx, L.P.ndim > 1 ? L.ny : 1, L.P.ndim > 2 ? L.nz : 1};
+  int b1 = 1, b0 = 1;
+  for (int d = 0; d < L.P.ndim; d++) { b1 *= n[d] + 2 * band + 2; b0 *= n[d] + 2 * band; }
+  const double nc1 = (double)b1 * L.nblk, nc0 = (double)b0 * L.nblk;
+  if (L.prof) L.prof->begin(L.stream, "k_shock_vc", 8.0 * nc1 * 5, nc1 * 6);
+  hipLaunchKernelGGL(k_shock_vc, dim3(grid_x_for(b1, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta,
+                     L.blist, U, scratch, band);
   if (L.prof) L.prof->end(L.stream);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  if (L.prof) L.prof->begin(L.stream, "k_shock", 8.0 * nc * 6, nc * 70);
-  hipLaunchKernelGGL(k_shock, dim3(grid_x_for(NX * NY * NZ, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta,
-                     L.blist, U, scratch);
+  if (L.prof) L.prof->begin(L.stream, "k_shock", 8.0 * (nc1 * 5 + nc0), nc0 * 70);
+  hipLaunchKernelGGL(k_shock, dim3(grid_x_for(b0, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta,
+                     L.blist, U, scratch, band);
   if (L.prof) L.prof->end(L.stream);
   return hipGetLastError();
 }
@@ -794,13 +782,16 @@ hipError_t plain_set_scalar(hipStream_t st, double* p, double v) {
 
 size_t plain_dt_part_size() { return sizeof(DtPart); }
 
-hipError_t plain_ug_fill(const PlainLaunch& L, int nbx, int nby, int nbz, double* U, int nvar) {
-  const int NX = L.nx + 2 * L.P.ng, NY = L.P.ndim > 1 ? L.ny + 2 * L.P.ng : 1,
-            NZ = L.P.ndim > 2 ? L.nz + 2 * L.P.ng : 1;
-  const int nin = L.nx * (L.P.ndim > 1 ? L.ny : 1) * (L.P.ndim > 2 ? L.nz : 1);
-  if (L.prof) L.prof->begin(L.stream, "k_ug_fill", 16.0 * L.nblk * (double)(NX * NY * NZ - nin) * nvar, 0.0);
-  hipLaunchKernelGGL(k_ug_fill, dim3(grid_x_for(NX * NY * NZ, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta,
-                     nbx, nby, nbz, U, nvar);
+hipError_t plain_ug_fill(const PlainLaunch& L, int nbx, int nby, int nbz, double* U, const VarList& vars, int depth) {
+  const int n[3] = {L.nx, L.P.ndim > 1 ? L.ny : 1, L.P.ndim > 2 ? L.nz : 1};
+  int box = 1, nin = 1;
+  for (int d = 0; d < 3; d++) {
+    box *= n[d] + (d < L.P.ndim ? 2 * depth : 0);
+    nin *= n[d];
+  }
+  if (L.prof) L.prof->begin(L.stream, "k_ug_fill", 16.0 * L.nblk * (double)(box - nin) * vars.n, 0.0);
+  hipLaunchKernelGGL(k_ug_fill, dim3(grid_x_for(box, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta,
+                     nbx, nby, nbz, U, vars, depth);
   if (L.prof) L.prof->end(L.stream);
   return hipGetLastError();
 }
