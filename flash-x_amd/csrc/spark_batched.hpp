@@ -1,0 +1,53 @@
+// spark_batched.hpp -- host/device interface of the batched FP32-flux form.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "spark_physics.hpp"
+
+namespace spark {
+
+constexpr int FUSED_THREADS = 256;    // workgroup size of the fused stage kernel
+constexpr int FUSED_XCDS = 8;         // MI355X: 8 XCDs, workgroups dispatched round-robin
+constexpr int FUSED_MAX_PLANE = 512;  // (tx + 6) * (ty + 6) bound of the LDS plane
+
+// one work item: a (tx x ty) column tile of one block's stage region,
+// marched over z0..z1 (guard-celled box coordinates)
+struct FusedItem {
+  int blk;
+  int x0, y0, tx, ty;
+  int z0, z1;
+  int xhi, yhi;  // last cell of the stage region in x / y (flux-register ownership)
+  int pad[3];
+};
+
+struct FusedArgs {
+  DevParams P;
+  const BlockMeta* meta;
+  const FusedItem* items;  // grouped by queue
+  const int* qoff;         // [FUSED_XCDS] first item of each queue
+  const int* qlen;         // [FUSED_XCDS] items in each queue
+  int* counters;           // [FUSED_XCDS] queue heads (reset before every launch)
+  const double* U0;        // step input (conserved U0 of the update)
+  const double* Sin;       // stage input state
+  double* Sout;            // stage output state (never equal to Sin)
+  double* fb[3];           // flux registers (nullptr when not correcting)
+  double weight;
+  int add_flux;
+  const double* dtp;
+  double c1, c2, c3;
+  int final_stage;
+  int* err;
+};
+
+// the fused stage kernel in FP32 (_f32) and FP64 (_f64) arithmetic
+size_t fused_lds_bytes_f32(int ns, int max_plane);
+hipError_t fused_reset_f32(hipStream_t st, int* counters);
+hipError_t fused_stage_f32(const FusedArgs& A, int grid, size_t lds, hipStream_t st);
+hipError_t fused_occupancy_f32(int ns, size_t lds, int* blocks_per_cu);
+size_t fused_lds_bytes_f64(int ns, int max_plane);
+hipError_t fused_reset_f64(hipStream_t st, int* counters);
+hipError_t fused_stage_f64(const FusedArgs& A, int grid, size_t lds, hipStream_t st);
+hipError_t fused_occupancy_f64(int ns, size_t lds, int* blocks_per_cu);
+
+}  // namespace spark

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "../../include/spark_hip.h"
+#include "spark_batched.hpp"
 #include "spark_launch.hpp"
 
 using spark::BlockMeta;
@@ -88,6 +89,32 @@ struct spark_batch {
   bool step_open = false;
   int64_t max_cells = 0;
   spark::Prof prof;
+  // batched FP32-flux form: ping-pong stage buffers S (stage 1) and S2
+  bool fused = false;
+  bool fused64 = false;  // batched form in FP64 arithmetic
+  double* S2 = nullptr;
+  int cus = 0;            // compute units of the device
+  size_t lds_limit = 0;   // LDS bytes a workgroup may allocate
+  size_t stage_lds[3] = {0, 0, 0};
+  int stage_grid[3] = {0, 0, 0};
+  struct StagePlan {
+    std::vector<spark::FusedItem> items;
+    spark::FusedItem* items_d = nullptr;
+    int* qoff_d = nullptr;
+    int* qlen_d = nullptr;
+  };
+  StagePlan plan[3];
+  int* counters = nullptr;  // FUSED_XCDS per stage
+  std::map<std::pair<int, int>, std::pair<int, int>> tiles;  // (class, ngoff) -> (tx, ty)
+  bool planned = false;
+  bool autotune = true;
+  int* tune_err = nullptr;    // Riemann-error counter of tuning launches
+  double* tune_dt = nullptr;  // dt = 0 of tuning launches (the batch's dt stays untouched)
+  // graph of one device-resident step (dt, guard fill, stages)
+  bool use_graph = true;
+  int* hist_slot = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
   int last_stage = 3;
   double coeff[3][3];
   int limits_array[3];
@@ -150,7 +177,7 @@ int run_dt(spark_batch* b, int hist_slot) {
     HIPCHK(spark::plain_dt_part(L, b->U, b->dtparts, b->dtchunks));
   }
   HIPCHK(spark::plain_dt_final(g_stream, b->dp, b->dtparts, b->nblk, b->dtchunks, b->dtp, b->loc,
-                               hist_slot >= 0 ? b->dt_hist : nullptr, hist_slot < 0 ? 0 : hist_slot));
+                               hist_slot >= 0 ? b->dt_hist : nullptr, b->hist_slot));
   return SPARK_OK;
 }
 
@@ -194,17 +221,100 @@ int run_step_begin(spark_batch* b) {
   return SPARK_OK;
 }
 
+// stage input/output buffers.  Plain form: U -> S -> S (in place, each
+// thread owns its cell) -> U.  Fused form: U -> S -> S2 -> U (tiles read
+// neighbours' cells, so input and output never alias).
+const double* stage_in(const spark_batch* b, int stage) {
+  if (stage == 1) return b->U;
+  if (!b->fused) return b->S;
+  return stage == 2 ? b->S : b->S2;
+}
+double* stage_out(spark_batch* b, int stage) {
+  if (stage == b->last_stage) return b->U;
+  if (!b->fused) return b->S;
+  return stage == 1 ? b->S : b->S2;
+}
+
+int stage_ngoff(const spark_batch* b, int stage) {
+  return b->p.guard_mode == SPARK_GUARD_REFILL ? 0 : b->limits_array[stage - 1];
+}
+
+int fused_plan(spark_batch* b);
+int fused_tune(spark_batch* b);
+
+size_t fused_lds(const spark_batch* b, int max_plane) {
+  return b->fused64 ? spark::fused_lds_bytes_f64(b->p.nscalars, max_plane)
+                    : spark::fused_lds_bytes_f32(b->p.nscalars, max_plane);
+}
+
+// persistent grid for a given LDS size: resident workgroups of the whole
+// device, a multiple of the XCD count
+int fused_grid_for(const spark_batch* b, size_t lds) {
+  int occ = 0;
+  hipError_t e = b->fused64 ? spark::fused_occupancy_f64(b->p.nscalars, lds, &occ)
+                            : spark::fused_occupancy_f32(b->p.nscalars, lds, &occ);
+  if (e != hipSuccess || occ < 1) occ = 1;
+  return std::max(spark::FUSED_XCDS, (b->cus * occ) / spark::FUSED_XCDS * spark::FUSED_XCDS);
+}
+
+hipError_t fused_launch(const spark_batch* b, const spark::FusedArgs& A, int grid, size_t lds) {
+  hipError_t e = b->fused64 ? spark::fused_reset_f64(g_stream, A.counters) : spark::fused_reset_f32(g_stream, A.counters);
+  if (e != hipSuccess) return e;
+  return b->fused64 ? spark::fused_stage_f64(A, grid, lds, g_stream) : spark::fused_stage_f32(A, grid, lds, g_stream);
+}
+
+spark::FusedArgs fused_args(spark_batch* b, int stage) {
+  spark::FusedArgs A;
+  memset(&A, 0, sizeof(A));
+  A.P = b->dp;
+  A.meta = b->meta_d;
+  A.items = b->plan[stage - 1].items_d;
+  A.qoff = b->plan[stage - 1].qoff_d;
+  A.qlen = b->plan[stage - 1].qlen_d;
+  A.counters = b->counters + (stage - 1) * spark::FUSED_XCDS;
+  A.U0 = b->U;
+  A.Sin = stage_in(b, stage);
+  A.Sout = stage_out(b, stage);
+  for (int d = 0; d < 3; d++) A.fb[d] = b->p.flux_correct ? b->fb[d] : nullptr;
+  A.weight = b->weights[stage - 1];
+  A.add_flux = stage > 1;
+  A.dtp = b->dtp;
+  A.c1 = b->coeff[stage - 1][0];
+  A.c2 = b->coeff[stage - 1][1];
+  A.c3 = b->coeff[stage - 1][2];
+  A.final_stage = stage == b->last_stage;
+  A.err = b->err;
+  return A;
+}
+
 // saveState's copy is folded away: stage 1 reads the solution U directly
 // (hy_starState == Uin at stage 1) and the last stage writes U directly
 // (updateState copies exactly the last stage's region, the interior).
 int run_stage(spark_batch* b, int stage, int ngoff) {
   const double* co = b->coeff[stage - 1];
+  if (b->fused) {
+    if (ngoff != stage_ngoff(b, stage))
+      return fail(SPARK_ERR_ARG, "batched form: stage %d runs with ngoff %d", stage, stage_ngoff(b, stage));
+    int rc;
+    if (!b->planned && (rc = fused_plan(b))) return rc;
+    spark::FusedArgs A = fused_args(b, stage);
+    const auto& pl = b->plan[stage - 1];
+    double cells = 0.;
+    for (const auto& it : pl.items) cells += (double)it.tx * it.ty * (it.z1 - it.z0 + 1);
+    // algorithmic traffic: read U0 and the stage input (5+ns conserved-side
+    // fields each), write the stage output (8+ns); flops as the plain form
+    b->prof.begin(g_stream, "k_fused_stage", cells * 8.0 * (2 * (5 + b->p.nscalars) + 8 + b->p.nscalars),
+                  cells * (3.0 * ((7 + b->p.nscalars) * 117.0 + 175.0) + 150.0));
+    HIPCHK(fused_launch(b, A, b->stage_grid[stage - 1], b->stage_lds[stage - 1]));
+    b->prof.end(g_stream);
+    return SPARK_OK;
+  }
   for (auto& c : b->classes) {
     PlainLaunch L = make_launch(b, c);
     L.weight = b->weights[stage - 1];
     L.add_flux = stage > 1;
-    L.Sin = (stage == 1) ? b->U : b->S;
-    L.Sout = (stage == b->last_stage) ? b->U : b->S;
+    L.Sin = stage_in(b, stage);
+    L.Sout = stage_out(b, stage);
     HIPCHK(spark::plain_stage(L, ngoff, co[0], co[1], co[2], stage == b->last_stage));
   }
   return SPARK_OK;
@@ -222,13 +332,25 @@ int run_full_step(spark_batch* b) {
   for (int s = 1; s <= b->last_stage; s++) {
     if ((rc = run_stage(b, s, refill ? 0 : b->limits_array[s - 1]))) return rc;
     if (refill && s < b->last_stage)
-      if ((rc = run_fill(b, b->S, false, b->p.nstencil))) return rc;
+      if ((rc = run_fill(b, stage_out(b, s), false, b->p.nstencil))) return rc;
   }
   return run_step_end(b);
 }
 
 void free_batch(spark_batch* b) {
   if (!b) return;
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (auto& pl : b->plan) {
+    if (pl.items_d) (void)hipFree(pl.items_d);
+    if (pl.qoff_d) (void)hipFree(pl.qoff_d);
+    if (pl.qlen_d) (void)hipFree(pl.qlen_d);
+  }
+  if (b->S2) (void)hipFree(b->S2);
+  if (b->counters) (void)hipFree(b->counters);
+  if (b->tune_err) (void)hipFree(b->tune_err);
+  if (b->tune_dt) (void)hipFree(b->tune_dt);
+  if (b->hist_slot) (void)hipFree(b->hist_slot);
   for (auto& r : b->prof.recs) {
     if (r.e0) (void)hipEventDestroy(r.e0);
     if (r.e1) (void)hipEventDestroy(r.e1);
@@ -240,6 +362,180 @@ void free_batch(spark_batch* b) {
   for (int d = 0; d < 3; d++) { F(b->fl[d]); F(b->fb[d]); }
   F(b->dtp); F(b->loc); F(b->dt_hist); F(b->dtparts); F(b->err);
   delete b;
+}
+
+// ---------------------------------------------------------------------------
+// batched form: work items and tile autotuning
+
+// candidate (tx, ty) tiles of an E x E region: near-equal splits that fit
+// the workgroup ((tx+2)(ty+2) <= threads) and the LDS plane
+std::vector<std::pair<int, int>> tile_candidates(const spark_batch* b, int ex, int ey) {
+  std::vector<std::pair<int, int>> out;
+  std::vector<int> xs, ys;
+  for (int parts = 1; parts <= 8; parts++) {
+    xs.push_back((ex + parts - 1) / parts);
+    ys.push_back((ey + parts - 1) / parts);
+  }
+  std::sort(xs.begin(), xs.end());
+  xs.erase(std::unique(xs.begin(), xs.end()), xs.end());
+  std::sort(ys.begin(), ys.end());
+  ys.erase(std::unique(ys.begin(), ys.end()), ys.end());
+  for (int tx : xs)
+    for (int ty : ys) {
+      if (tx < 2 || ty < 2) continue;
+      if ((tx + 2) * (ty + 2) > spark::FUSED_THREADS) continue;
+      if ((tx + 6) * (ty + 6) > spark::FUSED_MAX_PLANE) continue;
+      if (fused_lds(b, (tx + 6) * (ty + 6)) > b->lds_limit) continue;
+      out.push_back({tx, ty});
+    }
+  // rank by workgroup passes (tiles per region), then by idle lanes
+  auto tiles = [&](const std::pair<int, int>& t) {
+    return ((ex + t.first - 1) / t.first) * ((ey + t.second - 1) / t.second);
+  };
+  std::sort(out.begin(), out.end(), [&](const std::pair<int, int>& a, const std::pair<int, int>& c) {
+    const int ta = tiles(a), tc = tiles(c);
+    if (ta != tc) return ta < tc;
+    return (a.first + 2) * (a.second + 2) < (c.first + 2) * (c.second + 2);
+  });
+  if (out.size() > 6) out.resize(6);
+  return out;
+}
+
+void tile_items(const spark_batch* b, int blk, int ngoff, int tx, int ty, std::vector<spark::FusedItem>& out) {
+  const BlockMeta& m = b->meta_h[blk];
+  const int g = b->p.nguard;
+  const int xlo = g - ngoff, xhi = g + m.nx - 1 + ngoff;
+  const int ylo = g - ngoff, yhi = g + m.ny - 1 + ngoff;
+  const int zlo = g - ngoff, zhi = g + m.nz - 1 + ngoff;
+  for (int y0 = ylo; y0 <= yhi; y0 += ty)
+    for (int x0 = xlo; x0 <= xhi; x0 += tx) {
+      spark::FusedItem it;
+      memset(&it, 0, sizeof(it));
+      it.blk = blk;
+      it.x0 = x0; it.y0 = y0;
+      it.tx = std::min(tx, xhi - x0 + 1);
+      it.ty = std::min(ty, yhi - y0 + 1);
+      it.z0 = zlo; it.z1 = zhi;
+      it.xhi = xhi; it.yhi = yhi;
+      out.push_back(it);
+    }
+}
+
+// build the per-XCD queues of one stage from the chosen tiles.  A block's
+// items stay in one queue; blocks go to the least-loaded queue, largest first.
+int build_stage_plan(spark_batch* b, int stage, const std::vector<int>* only_blocks) {
+  const int ngoff = stage_ngoff(b, stage);
+  std::vector<int> blocks;
+  if (only_blocks) blocks = *only_blocks;
+  else for (int i = 0; i < b->nblk; i++) blocks.push_back(i);
+  std::vector<std::pair<double, int>> cost;
+  std::map<int, int> cls_of;
+  for (size_t ci = 0; ci < b->classes.size(); ci++)
+    for (int id : b->classes[ci].ids) cls_of[id] = (int)ci;
+  for (int id : blocks) {
+    const BlockMeta& m = b->meta_h[id];
+    cost.push_back({(double)(m.nx + 2 * ngoff) * (m.ny + 2 * ngoff) * (m.nz + 2 * ngoff + 2), id});
+  }
+  std::sort(cost.begin(), cost.end(), [](const std::pair<double, int>& a, const std::pair<double, int>& c) {
+    return a.first > c.first || (a.first == c.first && a.second < c.second);
+  });
+  std::vector<std::vector<spark::FusedItem>> q(spark::FUSED_XCDS);
+  std::vector<double> load(spark::FUSED_XCDS, 0.);
+  for (auto& pr : cost) {
+    const int id = pr.second;
+    const auto key = std::make_pair(cls_of[id], ngoff);
+    auto tt = b->tiles.find(key);
+    if (tt == b->tiles.end()) return fail(SPARK_ERR_STATE, "no tile chosen for class %d ngoff %d", key.first, ngoff);
+    const int qi = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    tile_items(b, id, ngoff, tt->second.first, tt->second.second, q[qi]);
+    load[qi] += pr.first;
+  }
+  auto& pl = b->plan[stage - 1];
+  pl.items.clear();
+  int qoff[spark::FUSED_XCDS], qlen[spark::FUSED_XCDS];
+  for (int i = 0; i < spark::FUSED_XCDS; i++) {
+    qoff[i] = (int)pl.items.size();
+    qlen[i] = (int)q[i].size();
+    pl.items.insert(pl.items.end(), q[i].begin(), q[i].end());
+  }
+  if (pl.items_d) (void)hipFree(pl.items_d);
+  if (!pl.qoff_d) HIPCHK(hipMalloc((void**)&pl.qoff_d, sizeof(int) * spark::FUSED_XCDS));
+  if (!pl.qlen_d) HIPCHK(hipMalloc((void**)&pl.qlen_d, sizeof(int) * spark::FUSED_XCDS));
+  HIPCHK(hipMalloc((void**)&pl.items_d, sizeof(spark::FusedItem) * std::max<size_t>(1, pl.items.size())));
+  HIPCHK(hipMemcpy(pl.items_d, pl.items.data(), sizeof(spark::FusedItem) * pl.items.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(pl.qoff_d, qoff, sizeof(qoff), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(pl.qlen_d, qlen, sizeof(qlen), hipMemcpyHostToDevice));
+  int max_plane = 1;
+  for (const auto& it : pl.items) max_plane = std::max(max_plane, (it.tx + 6) * (it.ty + 6));
+  b->stage_lds[stage - 1] = fused_lds(b, max_plane);
+  b->stage_grid[stage - 1] = fused_grid_for(b, b->stage_lds[stage - 1]);
+  return SPARK_OK;
+}
+
+// Autotuner: for every (shape class, stage region) time the candidate
+// tiles on (up to 64 of) the batch's own blocks and keep the fastest.
+// Runs once, on the first step, with the real data packed.
+int fused_tune(spark_batch* b) {
+  std::vector<int> ngoffs;
+  for (int s = 1; s <= b->last_stage; s++)
+    if (std::find(ngoffs.begin(), ngoffs.end(), stage_ngoff(b, s)) == ngoffs.end()) ngoffs.push_back(stage_ngoff(b, s));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  for (size_t ci = 0; ci < b->classes.size(); ci++) {
+    const ShapeClass& c = b->classes[ci];
+    std::vector<int> sample(c.ids.begin(), c.ids.begin() + std::min<size_t>(64, c.ids.size()));
+    for (int ngoff : ngoffs) {
+      auto cands = tile_candidates(b, c.nx + 2 * ngoff, c.ny + 2 * ngoff);
+      if (cands.empty()) return fail(SPARK_ERR_ARG, "no tile fits a %d x %d region", c.nx + 2 * ngoff, c.ny + 2 * ngoff);
+      const auto key = std::make_pair((int)ci, ngoff);
+      if (!b->autotune || cands.size() == 1) {
+        b->tiles[key] = cands[0];
+        continue;
+      }
+      // the stage that uses this region, for its coefficients
+      int stage = 1;
+      for (int s = 1; s <= b->last_stage; s++)
+        if (stage_ngoff(b, s) == ngoff) { stage = s; break; }
+      float best = 1e30f;
+      std::pair<int, int> pick = cands[0];
+      for (auto& cand : cands) {
+        b->tiles[key] = cand;
+        int rc = build_stage_plan(b, stage, &sample);
+        if (rc) return rc;
+        spark::FusedArgs A = fused_args(b, stage);
+        A.Sin = b->U;                     // read the packed solution,
+        A.Sout = b->S2 ? b->S2 : b->S;    // write a scratch stage buffer
+        for (int d = 0; d < 3; d++) A.fb[d] = nullptr;
+        A.err = b->tune_err;
+        A.dtp = b->tune_dt;
+        float t_best = 1e30f;
+        for (int rep = 0; rep < 3; rep++) {
+          HIPCHK(hipEventRecord(e0, g_stream));
+          HIPCHK(fused_launch(b, A, b->stage_grid[stage - 1], b->stage_lds[stage - 1]));
+          HIPCHK(hipEventRecord(e1, g_stream));
+          HIPCHK(hipEventSynchronize(e1));
+          float ms = 0.f;
+          HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+          if (rep > 0) t_best = std::min(t_best, ms);
+        }
+        if (t_best < best) { best = t_best; pick = cand; }
+      }
+      b->tiles[key] = pick;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return SPARK_OK;
+}
+
+int fused_plan(spark_batch* b) {
+  int rc;
+  if ((rc = fused_tune(b))) return rc;
+  for (int s = 1; s <= b->last_stage; s++)
+    if ((rc = build_stage_plan(b, s, nullptr))) return rc;
+  b->planned = true;
+  return SPARK_OK;
 }
 
 }  // namespace
@@ -312,7 +608,7 @@ int spark_batch_create(const spark_params* p, int nblocks, const spark_block_des
   if (p->rk_order != 2 && p->rk_order != 3) return fail(SPARK_ERR_ARG, "rk_order = %d", p->rk_order);
   if (p->recon < 0 || p->recon > 3) return fail(SPARK_ERR_ARG, "recon = %d", p->recon);
   if (p->riemann < 0 || p->riemann > 1) return fail(SPARK_ERR_ARG, "riemann = %d", p->riemann);
-  if (p->form != SPARK_FORM_PLAIN_FP64 && p->form != SPARK_FORM_FP32_FLUX)
+  if (p->form < SPARK_FORM_PLAIN_FP64 || p->form > SPARK_FORM_BATCHED_FP64)
     return fail(SPARK_ERR_ARG, "form = %d", p->form);
   const int radius = (p->recon == SPARK_RECON_PLM) ? 1 : (p->recon == SPARK_RECON_FOG) ? 0 : 2;
   if (p->nstencil < radius + 1) return fail(SPARK_ERR_ARG, "nstencil = %d too small for the reconstruction", p->nstencil);
@@ -323,8 +619,11 @@ int spark_batch_create(const spark_params* p, int nblocks, const spark_block_des
   for (int v = 0; v < nvar; v++)
     if (p->var_index[v] < 0 || p->var_index[v] >= p->nunk_vars)
       return fail(SPARK_ERR_ARG, "var_index[%d] = %d out of range", v, p->var_index[v]);
-  if (p->form == SPARK_FORM_FP32_FLUX)
-    return fail(SPARK_ERR_ARG, "form SPARK_FORM_FP32_FLUX is served by spark_batched_* entry points");
+  if (p->form == SPARK_FORM_FP32_FLUX || p->form == SPARK_FORM_BATCHED_FP64) {
+    if (p->ndim != 3) return fail(SPARK_ERR_ARG, "the batched form is 3D only (ndim = %d)", p->ndim);
+    if (p->use_hybrid_riemann)
+      return fail(SPARK_ERR_ARG, "the batched form does not implement use_hybrid_riemann");
+  }
 
   spark_batch* b = new spark_batch();
   b->p = *p;
@@ -417,6 +716,15 @@ int spark_batch_create(const spark_params* p, int nblocks, const spark_block_des
   ALLOC(b->dt_hist, sizeof(double) * b->hist_cap);
   ALLOC(b->dtparts, spark::plain_dt_part_size() * (size_t)nblocks * b->dtchunks);
   ALLOC(b->err, sizeof(int));
+  ALLOC(b->hist_slot, sizeof(int));
+  if (p->form == SPARK_FORM_FP32_FLUX || p->form == SPARK_FORM_BATCHED_FP64) {
+    b->fused = true;
+    b->fused64 = p->form == SPARK_FORM_BATCHED_FP64;
+    ALLOC(b->S2, sizeof(double) * b->pool);
+    ALLOC(b->counters, sizeof(int) * 3 * spark::FUSED_XCDS);
+    ALLOC(b->tune_err, sizeof(int));
+    ALLOC(b->tune_dt, sizeof(double));
+  }
 #undef ALLOC
   HIPCHK(hipMemcpyAsync(b->meta_d, b->meta_h.data(), sizeof(BlockMeta) * nblocks, hipMemcpyHostToDevice, g_stream));
   for (auto& c : b->classes)
@@ -428,6 +736,22 @@ int spark_batch_create(const spark_params* p, int nblocks, const spark_block_des
   if (p->flux_correct)
     for (int q = 0; q < 3; q++) HIPCHK(hipMemsetAsync(b->fb[q], 0, sizeof(double) * b->bpool, g_stream));
   HIPCHK(hipMemsetAsync(b->err, 0, sizeof(int), g_stream));
+  HIPCHK(hipMemsetAsync(b->hist_slot, 0, sizeof(int), g_stream));
+  if (b->fused) {
+    HIPCHK(hipMemsetAsync(b->S2, 0, sizeof(double) * b->pool, g_stream));
+    HIPCHK(hipMemsetAsync(b->counters, 0, sizeof(int) * 3 * spark::FUSED_XCDS, g_stream));
+    HIPCHK(hipMemsetAsync(b->tune_err, 0, sizeof(int), g_stream));
+    HIPCHK(hipMemsetAsync(b->tune_dt, 0, sizeof(double), g_stream));
+    int lim = 0;
+    HIPCHK(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, g_device));
+    HIPCHK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, g_device));
+    b->lds_limit = (size_t)lim;
+    if (fused_lds(b, 8 * 8) > b->lds_limit) {
+      const size_t need = fused_lds(b, 64), have = b->lds_limit;
+      free_batch(b);
+      return fail(SPARK_ERR_ARG, "batched form: %zu bytes of LDS needed, the device allows %zu", need, have);
+    }
+  }
   HIPCHK(hipMemsetAsync(b->loc, 0, sizeof(long long) * 2, g_stream));
   HIPCHK(hipStreamSynchronize(g_stream));
   *out = b;
@@ -581,27 +905,95 @@ int spark_step_end(spark_batch* b) {
   return run_step_end(b);
 }
 
+static int one_step(spark_batch* b) {
+  int rc = run_dt(b, 0);
+  if (rc) return rc;
+  if (b->has_ug && (rc = run_fill(b, b->U, false, stage1_reach(b)))) return rc;
+  return run_full_step(b);
+}
+
+static void drop_graph(spark_batch* b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+}
+
 int spark_advance(spark_batch* b, int nsteps, double* dts) {
   if (!b) return fail(SPARK_ERR_ARG, "spark_advance: null batch");
   if (nsteps < 0) return fail(SPARK_ERR_ARG, "nsteps = %d", nsteps);
   if (b->p.guard_mode == SPARK_GUARD_REFILL && !b->has_ug)
     return fail(SPARK_ERR_STATE, "refill guard mode needs spark_set_uniform_grid");
   if (nsteps > b->hist_cap) {
+    drop_graph(b);
     (void)hipFree(b->dt_hist);
     b->dt_hist = nullptr;
     HIPCHK(hipMalloc((void**)&b->dt_hist, sizeof(double) * nsteps));
     b->hist_cap = nsteps;
   }
+  int rc;
+  if (b->fused && !b->planned && (rc = fused_plan(b))) return rc;
+  HIPCHK(spark::plain_set_int(g_stream, b->hist_slot, 0));
+  const bool graph = b->use_graph && !b->prof.on && nsteps > 0;
+  if (graph && b->gexec == nullptr) {
+    // capture one device-resident step; every parameter of it lives on the
+    // device (dt, the dt-history slot, the queue heads), so replays chain
+    HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+    rc = one_step(b);
+    hipGraph_t gr = nullptr;
+    hipError_t e = hipStreamEndCapture(g_stream, &gr);
+    if (rc) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(SPARK_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    b->graph = gr;
+    HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+  }
   for (int s = 0; s < nsteps; s++) {
-    int rc = run_dt(b, s);
-    if (rc) return rc;
-    if (b->has_ug && (rc = run_fill(b, b->U, false, stage1_reach(b)))) return rc;
-    if ((rc = run_full_step(b))) return rc;
+    if (graph) {
+      HIPCHK(hipGraphLaunch(b->gexec, g_stream));
+    } else if ((rc = one_step(b))) {
+      return rc;
+    }
   }
   if (dts && nsteps > 0) {
     HIPCHK(hipMemcpyAsync(dts, b->dt_hist, sizeof(double) * nsteps, hipMemcpyDeviceToHost, g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
   }
+  return SPARK_OK;
+}
+
+int spark_set_option(spark_batch* b, int option, int value) {
+  if (!b) return fail(SPARK_ERR_ARG, "spark_set_option: null batch");
+  switch (option) {
+    case SPARK_OPT_GRAPH:
+      b->use_graph = value != 0;
+      if (!b->use_graph) drop_graph(b);
+      return SPARK_OK;
+    case SPARK_OPT_AUTOTUNE:
+      b->autotune = value != 0;
+      return SPARK_OK;
+    default:
+      return fail(SPARK_ERR_ARG, "spark_set_option: unknown option %d", option);
+  }
+}
+
+int spark_batched_tiles(spark_batch* b, int* out, int max_rows, int* nrows) {
+  if (!b || !out || !nrows) return fail(SPARK_ERR_ARG, "spark_batched_tiles: bad argument");
+  if (!b->fused) return fail(SPARK_ERR_STATE, "spark_batched_tiles: not a batched FP32-flux batch");
+  int rc;
+  if (!b->planned && (rc = fused_plan(b))) return rc;
+  int n = 0;
+  for (const auto& kv : b->tiles) {
+    if (n < max_rows) {
+      const ShapeClass& c = b->classes[kv.first.first];
+      int* r = out + 6 * n;
+      r[0] = c.nx; r[1] = c.ny; r[2] = c.nz; r[3] = kv.first.second; r[4] = kv.second.first; r[5] = kv.second.second;
+    }
+    n++;
+  }
+  *nrows = n;
   return SPARK_OK;
 }
 

@@ -63,8 +63,9 @@ hipError_t plain_stage(const PlainLaunch& L, int ngoff, double c1, double c2, do
 hipError_t plain_shock_detect(const PlainLaunch& L, double* U, double* scratch, int band);
 hipError_t plain_dt_part(const PlainLaunch& L, const double* U, void* parts, int chunks);
 hipError_t plain_dt_final(hipStream_t st, const DevParams& P, const void* parts, int nblk_total, int chunks,
-                          double* dt_out, long long* loc_out, double* dt_hist, int hist_slot);
+                          double* dt_out, long long* loc_out, double* dt_hist, int* hist_slot);
 hipError_t plain_set_scalar(hipStream_t st, double* p, double v);
+hipError_t plain_set_int(hipStream_t st, int* p, int v);
 size_t plain_dt_part_size();
 // variables a guard fill copies (canonical indices)
 struct VarList {

@@ -410,7 +410,7 @@ __global__ void __launch_bounds__(256) k_dt_part(DevParams P, const BlockMeta* _
 __global__ void __launch_bounds__(256) k_dt_final(DevParams P, const DtPart* __restrict__ parts, int nblocks,
                                                   int chunks, double* __restrict__ dt_out,
                                                   long long* __restrict__ loc_out, double* __restrict__ dt_hist,
-                                                  int hist_slot) {
+                                                  int* __restrict__ hist_slot) {
   __shared__ double sdt[256];
   __shared__ int sb[256];
   __shared__ long long sc[256];
@@ -440,7 +440,13 @@ __global__ void __launch_bounds__(256) k_dt_final(DevParams P, const DtPart* __r
   }
   if (threadIdx.x == 0) {
     dt_out[0] = sdt[0];
-    if (dt_hist) dt_hist[hist_slot] = sdt[0];
+    if (dt_hist) {
+      // the slot counter lives on the device so that a captured step graph
+      // records each replay's dt in the next slot
+      const int h = *hist_slot;
+      dt_hist[h] = sdt[0];
+      *hist_slot = h + 1;
+    }
     if (sb[0] < nblocks) {
       loc_out[0] = sb[0];
       loc_out[1] = sc[0];
@@ -765,7 +771,7 @@ hipError_t plain_dt_part(const PlainLaunch& L, const double* U, void* parts, int
 }
 
 hipError_t plain_dt_final(hipStream_t st, const DevParams& P, const void* parts, int nblk_total, int chunks,
-                          double* dt_out, long long* loc_out, double* dt_hist, int hist_slot) {
+                          double* dt_out, long long* loc_out, double* dt_hist, int* hist_slot) {
   hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, st, P, (const DtPart*)parts, nblk_total, chunks, dt_out,
                      loc_out, dt_hist, hist_slot);
   return hipGetLastError();
@@ -773,6 +779,15 @@ hipError_t plain_dt_final(hipStream_t st, const DevParams& P, const void* parts,
 
 __global__ void k_set_scalar(double* p, double v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = v;
+}
+
+__global__ void k_set_int(int* p, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = v;
+}
+
+hipError_t plain_set_int(hipStream_t st, int* p, int v) {
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, st, p, v);
+  return hipGetLastError();
 }
 
 hipError_t plain_set_scalar(hipStream_t st, double* p, double v) {

@@ -22,7 +22,7 @@ SPARK_MAX_SCALARS = 8
 
 RECON = {"weno5": 0, "mp5": 1, "plm": 2, "fog": 3}
 RIEMANN = {"hllc": 0, "hll": 1}
-FORM = {"plain_fp64": 0, "fp32_flux": 1}
+FORM = {"plain_fp64": 0, "fp32_flux": 1, "batched_fp64": 2}
 GUARD = {"telescope": 0, "refill": 1}
 
 # canonical variable order of the device state (alphabetical unk order of
@@ -96,6 +96,8 @@ def lib():
             "spark_get_counters": [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.c_int],
             "spark_last_error": [ctypes.c_char_p, ctypes.c_int],
             "spark_profile": [vp, ctypes.c_int],
+            "spark_set_option": [vp, ctypes.c_int, ctypes.c_int],
+            "spark_batched_tiles": [vp, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
             "spark_profile_read": [vp, ctypes.POINTER(SparkKernelStat), ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
         }
         for name, args in sig.items():
@@ -113,7 +115,8 @@ def exported_symbols():
         "spark_pack", "spark_unpack", "spark_set_uniform_grid", "spark_fill_guardcells", "spark_compute_dt",
         "spark_hydro_step", "spark_step_begin", "spark_stage", "spark_step_end", "spark_advance",
         "spark_get_flux_registers", "spark_set_flux_corrections", "spark_flux_correct", "spark_synchronize",
-        "spark_get_counters", "spark_last_error", "spark_profile", "spark_profile_read",
+        "spark_get_counters", "spark_last_error", "spark_profile", "spark_profile_read", "spark_set_option",
+        "spark_batched_tiles",
     ]
 
 
@@ -250,6 +253,16 @@ class Batch:
     def set_flux_corrections(self, fbx, fby, fbz):
         check(lib().spark_set_flux_corrections(self.h, _ptrs(fbx), _ptrs(fby), _ptrs(fbz)),
               "spark_set_flux_corrections")
+
+    def set_option(self, option: str, value: int):
+        code = {"graph": 1, "autotune": 2}[option]
+        check(lib().spark_set_option(self.h, code, int(value)), "spark_set_option")
+
+    def tiles(self):
+        rows = (ctypes.c_int * (6 * 64))()
+        n = ctypes.c_int()
+        check(lib().spark_batched_tiles(self.h, rows, 64, ctypes.byref(n)), "spark_batched_tiles")
+        return [tuple(rows[6 * i:6 * i + 6]) for i in range(min(n.value, 64))]
 
     def profile(self, enable: bool):
         check(lib().spark_profile(self.h, int(enable)), "spark_profile")

@@ -35,8 +35,11 @@ extern "C" {
 
 enum spark_recon { SPARK_RECON_WENO5 = 0, SPARK_RECON_MP5 = 1, SPARK_RECON_PLM = 2, SPARK_RECON_FOG = 3 };
 enum spark_riemann { SPARK_RIEMANN_HLLC = 0, SPARK_RIEMANN_HLL = 1 };
-/* plain FP64 form, or FP32 reconstruction/Riemann with FP64 accumulation */
-enum spark_form { SPARK_FORM_PLAIN_FP64 = 0, SPARK_FORM_FP32_FLUX = 1 };
+/* plain FP64 form (one launch per kernel per stage, materialised face
+ * fluxes); batched form (one fused persistent launch per stage over blocks
+ * of any size) with FP32 reconstruction/Riemann arithmetic and FP64
+ * accumulation, or in FP64 throughout */
+enum spark_form { SPARK_FORM_PLAIN_FP64 = 0, SPARK_FORM_FP32_FLUX = 1, SPARK_FORM_BATCHED_FP64 = 2 };
 /* guard policy: telescoping stages over guard cells filled once per step
  * (the reference's own scheme, Spark/Hydro.ini:19,37), or refilling the
  * guards of a uniform block grid on the device between stages */
@@ -173,6 +176,17 @@ typedef struct spark_kernel_stat {
 } spark_kernel_stat;
 int spark_profile(spark_batch* b, int enable);
 int spark_profile_read(spark_batch* b, spark_kernel_stat* stats, int max, int* n);
+
+/* Options.  SPARK_OPT_GRAPH (default 1): spark_advance captures one step
+ * into a hipGraph and replays it.  SPARK_OPT_AUTOTUNE (default 1): the
+ * batched FP32-flux form times candidate tile shapes per block shape and
+ * stage region on the first step and keeps the fastest.  No counterpart
+ * in the reference. */
+#define SPARK_OPT_GRAPH 1
+#define SPARK_OPT_AUTOTUNE 2
+int spark_set_option(spark_batch* b, int option, int value);
+/* Tiles chosen by the batched form: rows of (nx, ny, nz, ngoff, tx, ty). */
+int spark_batched_tiles(spark_batch* b, int* rows, int max_rows, int* nrows);
 
 int spark_synchronize(spark_batch* b);
 /* counters[0] = faces whose Riemann solve met a negative a^2 */
