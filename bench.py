@@ -72,16 +72,22 @@ def main():
 
     prob = problems.sedov3d(nb=8, n=16, nguard=6, guard_mode="refill")
     capi.init(0)
-    b = capi.Batch(prob.cfg, prob.blocks)
-    b.pack(prob.arrays)
-    b.set_uniform_grid(*prob.ug)
-    if args.warmup:
-        b.advance(args.warmup)
-    b.synchronize()
-    t0 = time.perf_counter()
-    b.advance(args.steps)
-    b.synchronize()
-    el = time.perf_counter() - t0
+
+    def timed(form):
+        b = capi.Batch(dict(prob.cfg, form=form), prob.blocks)
+        b.pack(prob.arrays)
+        b.set_uniform_grid(*prob.ug)
+        if args.warmup:
+            b.advance(args.warmup)   # the batched forms tune their tiles here
+        b.synchronize()
+        t0 = time.perf_counter()
+        b.advance(args.steps)
+        b.synchronize()
+        el = time.perf_counter() - t0
+        return b, el
+
+    # headline: the plain FP64 form
+    b, el = timed("plain_fp64")
     value = prob.zones * args.steps / el
 
     # per-kernel HIP-event timing over a separate pass (same stream)
@@ -90,6 +96,26 @@ def main():
     stats = b.profile_read()
     b.profile(False)
     b.close()
+
+    forms = {"plain_fp64": {"zone_updates_per_s": round(value, 1), "ms_per_step": round(el / args.steps * 1e3, 4)}}
+    for form in ("batched_fp64", "fp32_flux"):
+        bf, elf = timed(form)
+        bf.profile(True)
+        bf.advance(args.profile_steps)
+        st = bf.profile_read()
+        bf.profile(False)
+        tiles = bf.tiles()
+        bf.close()
+        fk = [x for x in st if x["name"] == "k_fused_stage"]
+        entry = {"zone_updates_per_s": round(prob.zones * args.steps / elf, 1),
+                 "ms_per_step": round(elf / args.steps * 1e3, 4),
+                 "speedup_vs_plain_fp64": round(el / elf, 3),
+                 "tiles": [list(t) for t in tiles]}
+        if fk:
+            k = fk[0]
+            entry["fused_stage_avg_ms"] = round(k["ms"] / k["launches"], 4)
+            entry["fused_stage_valu_tflops"] = round(k["flops"] / (k["ms"] * 1e-3) / 1e12, 2)
+        forms[form] = entry
     tot = sum(s["ms"] for s in stats)
     flux = [s for s in stats if s["name"].startswith("k_flux")]
     dom = max(stats, key=lambda s: s["ms"])
@@ -134,6 +160,7 @@ def main():
         "path_hbm_frac_algorithmic": round(value * step_bytes / prob.zones / 1e9 / HBM_PEAK_GBS, 5),
         "kernel_share": {s["name"]: round(s["ms"] / tot, 4) for s in sorted(stats, key=lambda s: -s["ms"])},
         "dominant_kernel": dom["name"],
+        "forms": forms,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
