@@ -33,7 +33,24 @@ def parse():
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--forms", default="batched_fp64,fp32_flux",
+                    help="comma list of the other forms timed beside plain FP64 (empty: none)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (counter calibration) and exit")
     return ap.parse_args()
+
+
+def calibrate_copy(mib: int):
+    """A plain device copy of known size: its FETCH_SIZE / WRITE_SIZE counter
+    readings give the scale of those counters for the traffic figure."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes": n * 8, "launches": 3}), flush=True)
 
 
 def cpu_baseline(target_s: float):
@@ -68,6 +85,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the block set is a single-GPU "
               f"shard and no multi-GPU path is built", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     from sparkhip import capi, problems
 
     prob = problems.sedov3d(nb=8, n=16, nguard=6, guard_mode="refill")
@@ -98,7 +118,7 @@ def main():
     b.close()
 
     forms = {"plain_fp64": {"zone_updates_per_s": round(value, 1), "ms_per_step": round(el / args.steps * 1e3, 4)}}
-    for form in ("batched_fp64", "fp32_flux"):
+    for form in [f for f in args.forms.split(",") if f]:
         bf, elf = timed(form)
         bf.profile(True)
         bf.advance(args.profile_steps)

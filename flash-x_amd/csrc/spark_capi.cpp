@@ -866,11 +866,11 @@ int spark_compute_dt(spark_batch* b, double* dt, int minloc[5]) {
     const int bid = (int)loc[0];
     const BlockMeta& m = b->meta_h[bid < 0 || bid >= b->nblk ? 0 : bid];
     const long long c = loc[1];
-    minloc[0] = bid;
-    minloc[1] = (int)(c % m.NX) + 1;
-    minloc[2] = (int)((c / m.NX) % m.NY) + 1;
-    minloc[3] = (int)(c / ((long long)m.NX * m.NY)) + 1;
-    minloc[4] = m.level;
+    minloc[0] = (int)(c % m.NX) + 1;
+    minloc[1] = (int)((c / m.NX) % m.NY) + 1;
+    minloc[2] = (int)(c / ((long long)m.NX * m.NY)) + 1;
+    minloc[3] = m.level;
+    minloc[4] = bid;
   }
   return SPARK_OK;
 }
@@ -881,6 +881,8 @@ int spark_hydro_step(spark_batch* b, double dt, double dtOld) {
   if (b->p.guard_mode == SPARK_GUARD_REFILL && !b->has_ug)
     return fail(SPARK_ERR_STATE, "refill guard mode needs spark_set_uniform_grid");
   HIPCHK(spark::plain_set_scalar(g_stream, b->dtp, dt));
+  int rc;
+  if (b->has_ug && (rc = run_fill(b, b->U, false, stage1_reach(b)))) return rc;
   return run_full_step(b);
 }
 

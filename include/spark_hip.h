@@ -120,15 +120,18 @@ int spark_unpack(spark_batch* b, double* const* unk_blocks, int interior_only, i
 int spark_set_uniform_grid(spark_batch* b, int nbx, int nby, int nbz);
 int spark_fill_guardcells(spark_batch* b);
 
-/* CFL time step over all blocks.  minloc = (block, i, j, k, level), the
- * indices 1-based in the guard-celled box as Fortran sees them.
+/* CFL time step over all blocks.  minloc = (i, j, k, level, block): the
+ * cell indices 1-based in the guard-celled box as Fortran sees them, as
+ * dtMinLoc(1:4) of the reference (Spark/Hydro_computeDt.F90:126-134); the
+ * fifth entry, the MPI rank there, is the 0-based batch index of the block.
  * Replaces: Hydro_computeDt, Hydro_interface.F90:24-42 /
  * Spark/Hydro_computeDt.F90:27-150 (per tile) plus the MIN over tiles of
  * Driver/DriverMain/Driver_computeDt.F90:306. */
 int spark_compute_dt(spark_batch* b, double* dt, int minloc[5]);
 
-/* One full RK step of every block (shockDetect, saveState, the stages,
- * updateState; flux registers when flux_correct).
+/* One full RK step of every block: the guard fill when a uniform grid was
+ * declared (Spark/Hydro.F90-mc:106), shockDetect, saveState, the stages,
+ * updateState; flux registers when flux_correct.
  * Replaces: Hydro(timeEndAdv, dt, dtOld, sweepOrder),
  * Hydro_interface.F90:61-66 / Spark/Hydro.F90-mc:24-202. */
 int spark_hydro_step(spark_batch* b, double dt, double dtOld);
