@@ -30,7 +30,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--profile-steps", type=int, default=10, help="steps of the per-kernel timing pass (>= 1)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--forms", default="batched_fp64,fp32_flux",
@@ -81,6 +81,7 @@ def cpu_baseline(target_s: float):
 
 def main():
     args = parse()
+    args.profile_steps = max(1, args.profile_steps)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the block set is a single-GPU "
               f"shard and no multi-GPU path is built", file=sys.stderr)

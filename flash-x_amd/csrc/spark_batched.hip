@@ -13,15 +13,24 @@
 // Work items are (block, x-y tile); a workgroup takes items from per-XCD
 // queues (an XCD's queue holds whole blocks, so the halos that tiles of one
 // block share stay in that XCD's L2) and steals from the other queues when
-// its own runs dry.  Inside an item the workgroup marches up z:
-//   * plane k (tile + 3 halo cells in x and y) is staged in LDS and every
-//     thread keeps a 5-deep z window of its own column in registers;
-//   * flattening, reconstruction, Riemann solve, artificial viscosity and
-//     scalar fluxes run in Real; x/y face states and fluxes go through LDS,
-//     the z face state/flux of the previous level stays in registers;
-//   * the flux differences are accumulated in FP64 in the reference order
-//     (x, y, z, source; hy_rk_updateSoln.F90-mc:257-265), then the update,
-//     floors and Gamma EOS (hy_rk_updateSoln.F90-mc:278-287, Eos_wrapped).
+// its own runs dry.  Inside an item the workgroup marches up z.  Each phase
+// of a level maps threads onto its own task space, so the lanes stay busy
+// although the task counts differ (tile tx x ty, C = tx*ty columns):
+//   P1  stage plane k (tile + 3 halo cells) in LDS; thread t < C owns
+//       column t and keeps a 5-deep z window of it in registers
+//   P2  flattening of the tile and its 1-cell ring (hy_rk_getFaceFlux.F90-mc:200-276)
+//   P3  reconstruction: z of the C columns (registers), x of the
+//       (tx+2) x ty and y of the tx x (ty+2) cells (LDS stencils)
+//   P4  Riemann solve + artificial viscosity: z faces of the columns, the
+//       (tx+1) x ty x faces and tx x (ty+1) y faces
+//   P5  FP64 update, floors and Gamma EOS of level k-1 (the z flux above it
+//       is known now), then the x/y flux differences of level k
+//   mass scalars follow in chunks of four through the same buffers
+//       (recon, upwinded fluxes, update), so the LDS and register footprint
+//       of the hydro part does not grow with the scalar count.
+// x and y task ranges are padded to whole wavefronts, so no wavefront mixes
+// the two directions.  The flux differences are accumulated in FP64 in the
+// reference order (x, y, z, source; hy_rk_updateSoln.F90-mc:257-265).
 // Stage input and output states are distinct buffers, so no tile reads a
 // cell another tile writes.
 #include <type_traits>
@@ -33,30 +42,108 @@
 #define SPARK_FUSED_REAL float
 #endif
 
+#define FUSED_NAME2(a, b) a##b
+#define FUSED_NAME(a, b) FUSED_NAME2(a, b)
+#define FUSED_SFX SPARK_FUSED_SUFFIX
+
 namespace spark {
 namespace {
 
 using Real = SPARK_FUSED_REAL;
 constexpr bool kF64 = std::is_same<Real, double>::value;
+constexpr int NT = FUSED_THREADS;
+// hydro slots staged per cell: dens velx vely velz pres [gamc] rhoe.  The
+// FP32 form takes gamc = gamma (the Gamma EOS sets every cell's gamc to
+// gamma), so it neither stages nor reconstructs it.
+constexpr int NH = kF64 ? 7 : 6;
+constexpr int CH = 4;  // mass scalars per chunk
+constexpr int SL_RHOE = NH - 1;
+
+__host__ __device__ __forceinline__ constexpr int hslot(int s) { return (!kF64 && s >= R_GAMC) ? s + 1 : s; }
+__host__ __device__ __forceinline__ constexpr int rup64(int n) { return (n + 63) & ~63; }
+
+// LDS carve-up of one work item, in Reals
+struct Lay {
+  int plane, flat, xm, xp, ym, yp, fx, fy, mx, my, total;
+};
+__host__ __device__ __forceinline__ Lay lds_layout(int tx, int ty, int nvt) {
+  Lay L;
+  const int psz = (tx + 6) * (ty + 6);
+  const int nrx = (tx + 2) * ty, nry = tx * (ty + 2), nfx = (tx + 1) * ty, nfy = tx * (ty + 1);
+  int o = 0;
+  L.plane = o; o += nvt * psz;
+  L.flat = o; o += (tx + 2) * (ty + 2);
+  L.xm = o; o += NH * nrx;
+  L.xp = o; o += NH * nrx;
+  L.ym = o; o += NH * nry;
+  L.yp = o; o += NH * nry;
+  L.fx = o; o += NH * nfx;  // flux rows: 5 hydro fluxes, or one chunk of scalars
+  L.fy = o; o += NH * nfy;
+  L.mx = o; o += nfx;       // mass flux of the x / y faces (scalar upwinding)
+  L.my = o; o += nfy;
+  L.total = o;
+  return L;
+}
+
+// division of small non-negative task indices by a per-item divisor
+// through one float multiply and a correction (no integer divide sequence)
+struct FDiv {
+  int d;
+  float inv;
+};
+__device__ __forceinline__ FDiv fdiv(int d) { return FDiv{d, 1.0f / (float)d}; }
+__device__ __forceinline__ int fquot(int w, const FDiv& f, int& r) {
+  int q = (int)((float)w * f.inv);
+  r = w - q * f.d;
+  if (r < 0) { q--; r += f.d; }
+  else if (r >= f.d) { q++; r -= f.d; }
+  return q;
+}
 
 __device__ __forceinline__ int64_t bcix(const BlockMeta& m, int i, int j, int k) {
   return (int64_t)i + (int64_t)m.NX * ((int64_t)j + (int64_t)m.NY * (int64_t)k);
 }
 
-// pencil slot v of a cell (R_ order: dens velx vely velz pres gamc rhoe,
-// then scalars).  The FP32 form takes gamc = gamma (the Gamma EOS sets
-// every cell's gamc to gamma); slots past the scalar count read as 0.
-__device__ __forceinline__ Real load_prim(const double* __restrict__ s, int64_t cs, int64_t c, int v, int ns,
-                                          double gamma) {
-  switch (v) {
+// pool offset of cell (i, j, k) of a block's guard-celled box.  With a
+// neighbour table (uniform grid, no guard refill) a cell outside the
+// interior is taken from the adjacent block, or, past the domain edge, from
+// the block's own boundary layer (outflow: the guard copies the first
+// interior layer, Grid_bcApplyToRegion.F90:526-532), one dimension at a
+// time as the guard fill clamps the global index.
+__device__ __forceinline__ int64_t cell_at(const BlockMeta& m, const int64_t* __restrict__ nbt, int g, int i, int j,
+                                           int k) {
+  if (nbt == nullptr) return m.base + bcix(m, i, j, k);
+  int ox = 0, oy = 0, oz = 0;
+  if (i < g) ox = -1; else if (i >= g + m.nx) ox = 1;
+  if (j < g) oy = -1; else if (j >= g + m.ny) oy = 1;
+  if (k < g) oz = -1; else if (k >= g + m.nz) oz = 1;
+  if ((ox | oy | oz) == 0) return m.base + bcix(m, i, j, k);
+  if (ox != 0) {
+    if (nbt[13 + ox] < 0) { i = ox < 0 ? g : g + m.nx - 1; ox = 0; }
+    else i -= ox * m.nx;
+  }
+  if (oy != 0) {
+    if (nbt[13 + 3 * oy] < 0) { j = oy < 0 ? g : g + m.ny - 1; oy = 0; }
+    else j -= oy * m.ny;
+  }
+  if (oz != 0) {
+    if (nbt[13 + 9 * oz] < 0) { k = oz < 0 ? g : g + m.nz - 1; oz = 0; }
+    else k -= oz * m.nz;
+  }
+  return nbt[13 + ox + 3 * oy + 9 * oz] + bcix(m, i, j, k);
+}
+
+// staged slot v of cell c: hydro slots in the pencil order, then scalars
+__device__ __forceinline__ Real load_slot(const double* __restrict__ s, int64_t cs, int64_t c, int v, double gamma) {
+  if (v >= NH) return (Real)s[(V_NPROP + v - NH) * cs + c];
+  switch (hslot(v)) {
     case R_DENS: return (Real)s[V_DENS * cs + c];
     case R_VELX: return (Real)s[V_VELX * cs + c];
     case R_VELY: return (Real)s[V_VELY * cs + c];
     case R_VELZ: return (Real)s[V_VELZ * cs + c];
     case R_PRES: return (Real)s[V_PRES * cs + c];
     case R_GAMC: return kF64 ? (Real)s[V_GAMC * cs + c] : (Real)gamma;
-    case R_RHOE: return (Real)(s[V_DENS * cs + c] * s[V_EINT * cs + c]);
-    default: return (v - R_NUM < ns) ? (Real)s[(V_NPROP + v - R_NUM) * cs + c] : Real(0.);
+    default: return (Real)(s[V_DENS * cs + c] * s[V_EINT * cs + c]);
   }
 }
 
@@ -76,12 +163,12 @@ struct RParams {
   Real gamma, cvisc, smlrho, smallp, tiny;
 };
 
-// face flux from the left/right face states (ensurePhysicalState,
-// Riemann solve, artificial viscosity, scalar upwinding).  c* hold the two
-// cells' centred dens, velx, vely, velz, rhoe.  (Hydro.ini:222-247,285-301)
-template <int DIR, int RIEMANN, int NSM>
-__device__ __forceinline__ void face_flux(const RParams& rp, int ns, Real* L, Real* R, const Real* cl, const Real* cr,
-                                          Real* F, int* err) {
+// hydro face flux from the left/right face states (ensurePhysicalState,
+// Riemann solve, artificial viscosity; Hydro.ini:222-235,285-301).  c*
+// hold the two cells' centred dens, velx, vely, velz, rhoe.
+template <int DIR, int RIEMANN>
+__device__ __forceinline__ void hydro_face(const RParams& rp, Real* L, Real* R, const Real* cl, const Real* cr, Real* F,
+                                           int* err) {
   L[R_DENS] = fmax2(rp.smlrho, L[R_DENS]);
   L[R_PRES] = fmax2(rp.smallp, L[R_PRES]);
   R[R_DENS] = fmax2(rp.smlrho, R[R_DENS]);
@@ -104,31 +191,54 @@ __device__ __forceinline__ void face_flux(const RParams& rp, int ns, Real* L, Re
   F[F_YMOM] = F[F_YMOM] + cvisc * (cl[0] * cl[2] - cr[0] * cr[2]);
   F[F_ZMOM] = F[F_ZMOM] + cvisc * (cl[0] * cl[3] - cr[0] * cr[3]);
   F[F_ENER] = F[F_ENER] + cvisc * (VenerLo - VenerHi);
+}
+
+// face states from the staged slots: L from a u+ buffer, R from a u- buffer
+__device__ __forceinline__ void face_states(const Real* __restrict__ bp, const Real* __restrict__ bm, int nr, int il,
+                                            int ir, Real* L, Real* R) {
 #pragma unroll
-  for (int s = 0; s < NSM; s++) {
-    if (s < ns) {
-      if (F[F_MASS] > Real(0.)) F[F_NUM + s] = L[R_NUM + s] * F[F_MASS];
-      else F[F_NUM + s] = R[R_NUM + s] * F[F_MASS];
-    }
+  for (int s = 0; s < NH; s++) {
+    L[hslot(s)] = bp[s * nr + il];
+    R[hslot(s)] = bm[s * nr + ir];
   }
+  if (!kF64) { L[R_GAMC] = Real(0.); R[R_GAMC] = Real(0.); }
+}
+
+__device__ __forceinline__ void centred(const Real* __restrict__ plane, int psz, int pc, Real* c) {
+  c[0] = plane[0 * psz + pc];
+  c[1] = plane[1 * psz + pc];
+  c[2] = plane[2 * psz + pc];
+  c[3] = plane[3 * psz + pc];
+  c[4] = plane[SL_RHOE * psz + pc];
+}
+
+// flux-register accumulation of one face (addFluxes, Hydro_funcs.F90-mc:175-201)
+__device__ __forceinline__ void reg_add(double* fb, int64_t o, int64_t tot, int n, double weight, int add, Real f) {
+  const double w = weight * (double)f;
+  fb[o + n * tot] = add ? fb[o + n * tot] + w : w;
 }
 
 template <int NSM, int RECON, int RIEMANN>
-__global__ void __launch_bounds__(FUSED_THREADS) k_fused(FusedArgs A) {
-  constexpr int NV = R_NUM + NSM;
-  constexpr int NF = F_NUM + NSM;
-  constexpr int NT = FUSED_THREADS;
+#ifndef FUSED_WAVES
+#define FUSED_WAVES 2
+#endif
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_WAVES))) FUSED_NAME(k_fused, FUSED_SFX)(FusedArgs A) {
+  constexpr int NFM = F_NUM + NSM;
+  constexpr int NSA = NSM > 0 ? NSM : 1;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   Real* lds = reinterpret_cast<Real*>(lds_raw);
   __shared__ int s_item;
+  __shared__ int64_t s_nb[27];
   const int tid = threadIdx.x;
   const DevParams& P = A.P;
   const RParams rp{(Real)P.gamma, (Real)P.cvisc, (Real)P.smlrho, (Real)P.smallp, (Real)P.tiny};
   const double dt = *A.dtp;
-  const int ns = P.ns;
+  const int ns = NSM > 0 ? P.ns : 0;
+  const int nvt = NH + ns;
+  const bool use_flat = RECON == RECON_WENO5 && P.use_flat;
+  const int g = P.ng;
   int q = blockIdx.x % FUSED_XCDS;  // the queue of this workgroup's XCD first
   int tried = 0;
-  Real* plane = lds;  // [NV][PW*PH]
   for (;;) {
     if (tid == 0) {
       int it = -1;
@@ -146,271 +256,496 @@ __global__ void __launch_bounds__(FUSED_THREADS) k_fused(FusedArgs A) {
     if (it < 0) break;
     const FusedItem I = A.items[it];
     const BlockMeta m = A.meta[I.blk];
-    const int W = I.tx + 2, H = I.ty + 2, PW = I.tx + 6, PH = I.ty + 6;
-    const int psz = PW * PH;
-    Real* xex = lds + NV * psz;  // [NV][NT] x face states u+, then x fluxes
-    Real* yex = xex + NV * NT;   // [NV][NT] y face states u+, then y fluxes
-    const bool active = tid < W * H;
-    const int px = tid % W, py = tid / W;
-    const int i = I.x0 - 1 + px, j = I.y0 - 1 + py;
-    const bool rowx = active && py >= 1 && py <= I.ty;  // x faces / x reconstruction rows
-    const bool coly = active && px >= 1 && px <= I.tx;  // y faces / y reconstruction columns
-    const bool inner = rowx && coly;                    // cells this item updates
+    const int tx = I.tx, ty = I.ty, PW = tx + 6, psz = PW * (ty + 6);
+    const int C = tx * ty, Cp = rup64(C);
+    const int nrx = (tx + 2) * ty, nry = tx * (ty + 2), nfx = (tx + 1) * ty, nfy = tx * (ty + 1);
+    const Lay L = lds_layout(tx, ty, nvt);
+    const FDiv d_tx = fdiv(tx), d_tx1 = fdiv(tx + 1), d_tx2 = fdiv(tx + 2), d_pw = fdiv(PW);
+    Real* plane = lds + L.plane;
+    Real* flatb = lds + L.flat;
+    Real* xm = lds + L.xm;
+    Real* xp = lds + L.xp;
+    Real* ym = lds + L.ym;
+    Real* yp = lds + L.yp;
+    Real* fxb = lds + L.fx;
+    Real* fyb = lds + L.fy;
+    Real* mxb = lds + L.mx;
+    Real* myb = lds + L.my;
     const int64_t cs = m.cstride;
     const int64_t sz = (int64_t)m.NX * m.NY;
     const double* sin = A.Sin + m.base;
+    const double* S = A.Sin;  // pool base: cell_at() offsets include the block's base
+    // the item's neighbour offsets, cached in LDS (read on every halo load)
+    const int64_t* nbt = nullptr;
+    if (A.nbr) {
+      if (tid < 27) s_nb[tid] = A.nbr[(int64_t)I.blk * 27 + tid];
+      __syncthreads();
+      nbt = s_nb;
+    }
     const double* u0p = A.U0 + m.base;
     double* sout = A.Sout + m.base;
     const double dtdx = dt / m.dx, dtdy = dt / m.dy, dtdz = dt / m.dz;
-    const int64_t colc = bcix(m, active ? i : I.x0, active ? j : I.y0, 0);
+    const bool col = tid < C;
+    int cx = 0, cy = 0;
+    if (col) cy = fquot(tid, d_tx, cx);
+    const int ci = I.x0 + cx, cj = I.y0 + cy;
+    const int64_t colc = bcix(m, ci, cj, 0);
+    const int pcc = (cy + 3) * PW + (cx + 3);  // the column's cell in the plane
+    const int fcc = (cy + 1) * (tx + 2) + (cx + 1);
+    const bool regs = A.fb[0] != nullptr;
+    const bool colin = ci >= g && ci < g + m.nx && cj >= g && cj < g + m.ny;
 
-    Real win[NV][5];  // z window k-2 .. k+2 of this thread's column
-    if (active) {
+    Real win[NH][5];    // z window k-2 .. k+2 of the column, hydro slots
+    Real wsc[NSA][5];   // same for the scalars
+    Real upz[NH], upzs[NSA];  // u+ of level k-1 (z faces)
+    Real Hprev[NFM];          // z flux at the low face of level k-1
+    double dUxy[NFM];         // -dt/dx dF - dt/dy dG of level k-1
+#pragma unroll
+    for (int v = 0; v < NH; v++) {
+      upz[v] = Real(0.);
+#pragma unroll
+      for (int w = 0; w < 5; w++) win[v][w] = Real(0.);
+    }
+#pragma unroll
+    for (int s = 0; s < NSA; s++) {
+      upzs[s] = Real(0.);
+#pragma unroll
+      for (int w = 0; w < 5; w++) wsc[s][w] = Real(0.);
+    }
+#pragma unroll
+    for (int n = 0; n < NFM; n++) { Hprev[n] = Real(0.); dUxy[n] = 0.; }
+    if (col) {
+      // levels z0-3 .. z0; the first pass (k = z0-1) shifts them down and
+      // loads z0+1
 #pragma unroll
       for (int w = 1; w < 5; w++) {
-        const int64_t c = colc + (int64_t)(I.z0 - 4 + w) * sz;
+        const int64_t c = cell_at(m, nbt, g, ci, cj, I.z0 - 4 + w);
 #pragma unroll
-        for (int v = 0; v < NV; v++) win[v][w] = load_prim(sin, cs, c, v, ns, P.gamma);
+        for (int v = 0; v < NH; v++) win[v][w] = load_slot(S, cs, c, v, P.gamma);
+#pragma unroll
+        for (int s = 0; s < NSM; s++)
+          if (s < ns) wsc[s][w] = (Real)S[(V_NPROP + s) * cs + c];
       }
     }
-    Real upz[NV];     // z face state u+ of level k-1
-    Real Hprev[NF];   // z flux at the low face of level k-1
-    double dUxy[NF];  // -dt/dx dF - dt/dy dG of level k-1
-#pragma unroll
-    for (int v = 0; v < NV; v++) upz[v] = Real(0.);
-#pragma unroll
-    for (int n = 0; n < NF; n++) { Hprev[n] = Real(0.); dUxy[n] = 0.; }
 
     for (int k = I.z0 - 1; k <= I.z1 + 1; k++) {
       const bool xyk = (k >= I.z0) && (k <= I.z1);
-      // advance the z window to k-2..k+2 and stage plane k
-      if (active) {
+      // ---- P1: advance the z window to k-2..k+2, stage plane k
+      if (col) {
+        const int64_t c = cell_at(m, nbt, g, ci, cj, k + 2);
 #pragma unroll
-        for (int v = 0; v < NV; v++) {
+        for (int v = 0; v < NH; v++) {
 #pragma unroll
           for (int w = 0; w < 4; w++) win[v][w] = win[v][w + 1];
+          win[v][4] = load_slot(S, cs, c, v, P.gamma);
+          plane[v * psz + pcc] = win[v][2];
         }
-        const int64_t c = colc + (int64_t)(k + 2) * sz;
 #pragma unroll
-        for (int v = 0; v < NV; v++) win[v][4] = load_prim(sin, cs, c, v, ns, P.gamma);
-        const int pc = (py + 2) * PW + (px + 2);
+        for (int s = 0; s < NSM; s++) {
+          if (s < ns) {
 #pragma unroll
-        for (int v = 0; v < NV; v++) plane[v * psz + pc] = win[v][2];
+            for (int w = 0; w < 4; w++) wsc[s][w] = wsc[s][w + 1];
+            wsc[s][4] = (Real)S[(V_NPROP + s) * cs + c];
+            plane[(NH + s) * psz + pcc] = wsc[s][2];
+          }
+        }
       }
-      for (int qq = tid; qq < psz; qq += NT) {
-        const int qx = qq % PW, qy = qq / PW;
-        if (qx >= 2 && qx < W + 2 && qy >= 2 && qy < H + 2) continue;
-        const int64_t c = bcix(m, I.x0 - 3 + qx, I.y0 - 3 + qy, k);
-#pragma unroll
-        for (int v = 0; v < NV; v++) plane[v * psz + qq] = load_prim(sin, cs, c, v, ns, P.gamma);
+      {
+        // the 3-cell halo ring: 3 full rows below and above, 3 + 3 cells in
+        // each of the ty middle rows
+        const int nhalo = 6 * PW + 6 * ty;
+        for (int u = tid; u < nhalo; u += NT) {
+          int qx, qy, h = u;
+          if (h < 3 * PW) { qy = fquot(h, d_pw, qx); }
+          else if ((h -= 3 * PW) < 3 * PW) { qy = ty + 3 + fquot(h, d_pw, qx); }
+          else {
+            h -= 3 * PW;
+            qy = 3 + h / 6;
+            const int cc = h % 6;
+            qx = cc < 3 ? cc : tx + cc;
+          }
+          const int64_t c = cell_at(m, nbt, g, I.x0 - 3 + qx, I.y0 - 3 + qy, k);
+          const int pc = qy * PW + qx;
+          for (int v = 0; v < nvt; v++) plane[v * psz + pc] = load_slot(S, cs, c, v, P.gamma);
+        }
       }
       __syncthreads();
 
-      const int pc = (py + 2) * PW + (px + 2);
-      Real flat = Real(1.);
-      if (RECON == RECON_WENO5 && P.use_flat && active) {
-        const Real* pp = plane + R_PRES * psz;
-        const Real g0 = plane[R_GAMC * psz + pc];
-        const Real fx = flat_dir(pp[pc - 2], pp[pc - 1], pp[pc], pp[pc + 1], pp[pc + 2], g0,
-                                 plane[R_VELX * psz + pc], plane[R_VELX * psz + pc + 1]);
-        const Real fy = flat_dir(pp[pc - 2 * PW], pp[pc - PW], pp[pc], pp[pc + PW], pp[pc + 2 * PW], g0,
-                                 plane[R_VELY * psz + pc], plane[R_VELY * psz + pc + PW]);
-        const Real fz = flat_dir(win[R_PRES][0], win[R_PRES][1], win[R_PRES][2], win[R_PRES][3], win[R_PRES][4], g0,
-                                 win[R_VELZ][2], win[R_VELZ][3]);
-        flat = fmin2(fmin2(fx, fy), fz);
-      }
-      // x and y reconstruction of level k from the plane
-      Real umx[NV], umy[NV];
-      if (xyk && rowx) {
-#pragma unroll
-        for (int v = 0; v < NV; v++) {
-          if (v < R_NUM + ns && (kF64 || v != R_GAMC)) {
-            const Real* r = plane + v * psz + pc;
-            Real u[5] = {r[-2], r[-1], r[0], r[1], r[2]};
-            Real up;
-            recon1<RECON, Real>(u, flat, up, umx[v]);
-            xex[v * NT + tid] = up;
+      // ---- P2: flattening of the tile cells (all levels) and of the ring
+      // around the tile (levels with x/y reconstruction)
+      if (use_flat) {
+        const int nflat = xyk ? (tx + 2) * (ty + 2) : C;
+        const Real* pp = plane + 4 * psz;
+        for (int u = tid; u < nflat; u += NT) {
+          int px, py;
+          const bool own = u < C;  // u == tid: the column's own cell
+          if (own) { py = fquot(u, d_tx, px) + 1; px += 1; }
+          else {
+            const int h = u - C;
+            if (h < tx + 2) { px = h; py = 0; }
+            else if (h < 2 * (tx + 2)) { px = h - (tx + 2); py = ty + 1; }
+            else { const int r = h - 2 * (tx + 2); py = 1 + r / 2; px = (r & 1) ? tx + 1 : 0; }
           }
-        }
-      }
-      if (xyk && coly) {
-#pragma unroll
-        for (int v = 0; v < NV; v++) {
-          if (v < R_NUM + ns && (kF64 || v != R_GAMC)) {
-            const Real* r = plane + v * psz + pc;
-            Real u[5] = {r[-2 * PW], r[-PW], r[0], r[PW], r[2 * PW]};
-            Real up;
-            recon1<RECON, Real>(u, flat, up, umy[v]);
-            yex[v * NT + tid] = up;
-          }
-        }
-      }
-      __syncthreads();
-      Real Fx[NF], Fy[NF];
-      const bool fxok = xyk && rowx && px >= 1;
-      const bool fyok = xyk && coly && py >= 1;
-      if (fxok) {
-        Real Ls[NV], cl[5], cr[5];
-#pragma unroll
-        for (int v = 0; v < NV; v++) Ls[v] = (v < R_NUM + ns && (kF64 || v != R_GAMC)) ? xex[v * NT + tid - 1] : Real(0.);
-        const int c0 = pc - 1;
-        cl[0] = plane[R_DENS * psz + c0]; cl[1] = plane[R_VELX * psz + c0]; cl[2] = plane[R_VELY * psz + c0];
-        cl[3] = plane[R_VELZ * psz + c0]; cl[4] = plane[R_RHOE * psz + c0];
-        cr[0] = plane[R_DENS * psz + pc]; cr[1] = plane[R_VELX * psz + pc]; cr[2] = plane[R_VELY * psz + pc];
-        cr[3] = plane[R_VELZ * psz + pc]; cr[4] = plane[R_RHOE * psz + pc];
-        face_flux<0, RIEMANN, NSM>(rp, ns, Ls, umx, cl, cr, Fx, A.err);
-      }
-      if (fyok) {
-        Real Ls[NV], cl[5], cr[5];
-#pragma unroll
-        for (int v = 0; v < NV; v++) Ls[v] = (v < R_NUM + ns && (kF64 || v != R_GAMC)) ? yex[v * NT + tid - W] : Real(0.);
-        const int c0 = pc - PW;
-        cl[0] = plane[R_DENS * psz + c0]; cl[1] = plane[R_VELX * psz + c0]; cl[2] = plane[R_VELY * psz + c0];
-        cl[3] = plane[R_VELZ * psz + c0]; cl[4] = plane[R_RHOE * psz + c0];
-        cr[0] = plane[R_DENS * psz + pc]; cr[1] = plane[R_VELX * psz + pc]; cr[2] = plane[R_VELY * psz + pc];
-        cr[3] = plane[R_VELZ * psz + pc]; cr[4] = plane[R_RHOE * psz + pc];
-        face_flux<1, RIEMANN, NSM>(rp, ns, Ls, umy, cl, cr, Fy, A.err);
-      }
-      __syncthreads();
-      if (fxok) {
-#pragma unroll
-        for (int n = 0; n < NF; n++) xex[n * NT + tid] = Fx[n];
-      }
-      if (fyok) {
-#pragma unroll
-        for (int n = 0; n < NF; n++) yex[n * NT + tid] = Fy[n];
-      }
-      __syncthreads();
-      // flux registers of interior faces (addFluxes; each face written once)
-      if (A.fb[0] != nullptr && xyk) {
-        const int g = P.ng;
-        if (fxok && (px <= I.tx || i == I.xhi + 1) && i >= g && i <= g + m.nx && j >= g && j < g + m.ny && k >= g &&
-            k < g + m.nz) {
-          const int ex = m.nx + 1, ey = m.ny;
-          const int64_t tot = (int64_t)ex * ey * m.nz;
-          const int64_t o = m.bbase + (i - g) + (int64_t)ex * ((j - g) + (int64_t)ey * (k - g));
-          for (int n = 0; n < F_NUM + ns; n++) {
-            const double w = A.weight * (double)Fx[n];
-            A.fb[0][o + n * tot] = A.add_flux ? A.fb[0][o + n * tot] + w : w;
-          }
-        }
-        if (fyok && (py <= I.ty || j == I.yhi + 1) && i >= g && i < g + m.nx && j >= g && j <= g + m.ny && k >= g &&
-            k < g + m.nz) {
-          const int ex = m.nx, ey = m.ny + 1;
-          const int64_t tot = (int64_t)ex * ey * m.nz;
-          const int64_t o = m.bbase + (i - g) + (int64_t)ex * ((j - g) + (int64_t)ey * (k - g));
-          for (int n = 0; n < F_NUM + ns; n++) {
-            const double w = A.weight * (double)Fy[n];
-            A.fb[1][o + n * tot] = A.add_flux ? A.fb[1][o + n * tot] + w : w;
-          }
-        }
-      }
-      // z reconstruction of level k (register window) and the face k-1/2
-      Real upzk[NV], umz[NV];
-      if (inner) {
-#pragma unroll
-        for (int v = 0; v < NV; v++) {
-          if (v < R_NUM + ns && (kF64 || v != R_GAMC)) {
-            Real u[5] = {win[v][0], win[v][1], win[v][2], win[v][3], win[v][4]};
-            recon1<RECON, Real>(u, flat, upzk[v], umz[v]);
+          const int pc = (py + 2) * PW + (px + 2);
+          const Real g0 = kF64 ? plane[R_GAMC * psz + pc] : rp.gamma;
+          const Real fx = flat_dir(pp[pc - 2], pp[pc - 1], pp[pc], pp[pc + 1], pp[pc + 2], g0, plane[1 * psz + pc],
+                                   plane[1 * psz + pc + 1]);
+          const Real fy = flat_dir(pp[pc - 2 * PW], pp[pc - PW], pp[pc], pp[pc + PW], pp[pc + 2 * PW], g0,
+                                   plane[2 * psz + pc], plane[2 * psz + pc + PW]);
+          Real fz;
+          if (own) {
+            fz = flat_dir(win[4][0], win[4][1], win[4][2], win[4][3], win[4][4], g0, win[3][2], win[3][3]);
           } else {
-            upzk[v] = Real(0.);
-            umz[v] = Real(0.);
+            const int x = I.x0 - 1 + px, y = I.y0 - 1 + py;
+            const double* pg = S + V_PRES * cs;
+            const double* vg = S + V_VELZ * cs;
+            const int64_t c0 = cell_at(m, nbt, g, x, y, k), c1 = cell_at(m, nbt, g, x, y, k + 1);
+            fz = flat_dir((Real)pg[cell_at(m, nbt, g, x, y, k - 2)], (Real)pg[cell_at(m, nbt, g, x, y, k - 1)],
+                          (Real)pg[c0], (Real)pg[c1], (Real)pg[cell_at(m, nbt, g, x, y, k + 2)], g0, (Real)vg[c0],
+                          (Real)vg[c1]);
+          }
+          flatb[py * (tx + 2) + px] = fmin2(fmin2(fx, fy), fz);
+        }
+      }
+      __syncthreads();
+
+      // ---- P3: reconstruction of the hydro slots
+      Real upzk[NH], umz[NH];
+#pragma unroll
+      for (int v = 0; v < NH; v++) { upzk[v] = Real(0.); umz[v] = Real(0.); }
+      {
+        const int nrxp = rup64(nrx);
+        const int utot = Cp + (xyk ? nrxp + nry : 0);
+        for (int u = tid; u < utot; u += NT) {
+          if (u < Cp) {
+            if (u < C) {
+              const Real fl = use_flat ? flatb[fcc] : Real(1.);
+#pragma unroll
+              for (int v = 0; v < NH; v++) recon1<RECON, Real>(win[v], fl, upzk[v], umz[v]);
+            }
+            continue;
+          }
+          int w = u - Cp;
+          const bool isx = w < nrxp;
+          if (isx && w >= nrx) continue;
+          if (!isx) w -= nrxp;
+          int rx, ry, st, nr;
+          Real *om, *op;
+          if (isx) { ry = fquot(w, d_tx2, rx); rx -= 1; st = 1; nr = nrx; om = xm; op = xp; }
+          else { ry = fquot(w, d_tx, rx) - 1; st = PW; nr = nry; om = ym; op = yp; }
+          const int pc = (ry + 3) * PW + (rx + 3);
+          const Real fl = use_flat ? flatb[(ry + 1) * (tx + 2) + (rx + 1)] : Real(1.);
+#pragma unroll
+          for (int v = 0; v < NH; v++) {
+            const Real* r = plane + v * psz + pc;
+            const Real u5[5] = {r[-2 * st], r[-st], r[0], r[st], r[2 * st]};
+            Real up, um;
+            recon1<RECON, Real>(u5, fl, up, um);
+            om[v * nr + w] = um;
+            op[v * nr + w] = up;
           }
         }
       }
-      if (inner && k >= I.z0) {
-        Real Hz[NF], cl[5], cr[5];
-        cl[0] = win[R_DENS][1]; cl[1] = win[R_VELX][1]; cl[2] = win[R_VELY][1]; cl[3] = win[R_VELZ][1];
-        cl[4] = win[R_RHOE][1];
-        cr[0] = win[R_DENS][2]; cr[1] = win[R_VELX][2]; cr[2] = win[R_VELY][2]; cr[3] = win[R_VELZ][2];
-        cr[4] = win[R_RHOE][2];
-        face_flux<2, RIEMANN, NSM>(rp, ns, upz, umz, cl, cr, Hz, A.err);
-        if (A.fb[2] != nullptr) {
-          const int g = P.ng;
-          if (i >= g && i < g + m.nx && j >= g && j < g + m.ny && k >= g && k <= g + m.nz) {
-            const int64_t tot = (int64_t)m.nx * m.ny * (m.nz + 1);
-            const int64_t o = m.bbase + (i - g) + (int64_t)m.nx * ((j - g) + (int64_t)m.ny * (k - g));
-            for (int n = 0; n < F_NUM + ns; n++) {
-              const double w = A.weight * (double)Hz[n];
-              A.fb[2][o + n * tot] = A.add_flux ? A.fb[2][o + n * tot] + w : w;
+      __syncthreads();
+
+      // ---- P4: hydro face fluxes
+      Real Hz[F_NUM];
+#pragma unroll
+      for (int n = 0; n < F_NUM; n++) Hz[n] = Real(0.);
+      {
+        const int nfxp = rup64(nfx);
+        const int utot = Cp + (xyk ? nfxp + nfy : 0);
+        for (int u = tid; u < utot; u += NT) {
+          if (u < Cp) {
+            if (u < C && k >= I.z0) {
+              Real Ls[R_NUM], Rs[R_NUM], cl[5], cr[5];
+#pragma unroll
+              for (int s = 0; s < NH; s++) {
+                Ls[hslot(s)] = upz[s];
+                Rs[hslot(s)] = umz[s];
+              }
+              if (!kF64) { Ls[R_GAMC] = Real(0.); Rs[R_GAMC] = Real(0.); }
+              cl[0] = win[0][1]; cl[1] = win[1][1]; cl[2] = win[2][1]; cl[3] = win[3][1]; cl[4] = win[SL_RHOE][1];
+              cr[0] = win[0][2]; cr[1] = win[1][2]; cr[2] = win[2][2]; cr[3] = win[3][2]; cr[4] = win[SL_RHOE][2];
+              hydro_face<2, RIEMANN>(rp, Ls, Rs, cl, cr, Hz, A.err);
+              if (regs && colin && k >= g && k <= g + m.nz) {
+                const int64_t tot = (int64_t)m.nx * m.ny * (m.nz + 1);
+                const int64_t o = m.bbase + (ci - g) + (int64_t)m.nx * ((cj - g) + (int64_t)m.ny * (k - g));
+#pragma unroll
+                for (int n = 0; n < F_NUM; n++) reg_add(A.fb[2], o, tot, n, A.weight, A.add_flux, Hz[n]);
+              }
+            }
+            continue;
+          }
+          int w = u - Cp;
+          const bool isx = w < nfxp;
+          if (isx && w >= nfx) continue;
+          if (!isx) w -= nfxp;
+          Real Ls[R_NUM], Rs[R_NUM], cl[5], cr[5], F[F_NUM];
+          int fi, fj;  // the face is the low face of cell (fi, fj) of the tile
+          if (isx) {
+            fj = fquot(w, d_tx1, fi);
+            const int il = fj * (tx + 2) + fi;
+            face_states(xp, xm, nrx, il, il + 1, Ls, Rs);
+            const int pc = (fj + 3) * PW + (fi + 3);
+            centred(plane, psz, pc - 1, cl);
+            centred(plane, psz, pc, cr);
+            hydro_face<0, RIEMANN>(rp, Ls, Rs, cl, cr, F, A.err);
+#pragma unroll
+            for (int n = 0; n < F_NUM; n++) fxb[n * nfx + w] = F[n];
+            mxb[w] = F[F_MASS];
+          } else {
+            fj = fquot(w, d_tx, fi);
+            const int il = fj * tx + fi;
+            face_states(yp, ym, nry, il, il + tx, Ls, Rs);
+            const int pc = (fj + 3) * PW + (fi + 3);
+            centred(plane, psz, pc - PW, cl);
+            centred(plane, psz, pc, cr);
+            hydro_face<1, RIEMANN>(rp, Ls, Rs, cl, cr, F, A.err);
+#pragma unroll
+            for (int n = 0; n < F_NUM; n++) fyb[n * nfy + w] = F[n];
+            myb[w] = F[F_MASS];
+          }
+          if (regs) {
+            const int i = I.x0 + fi, j = I.y0 + fj;
+            if (isx) {
+              if ((fi < tx || i == I.xhi + 1) && i >= g && i <= g + m.nx && j >= g && j < g + m.ny && k >= g &&
+                  k < g + m.nz) {
+                const int ex = m.nx + 1, ey = m.ny;
+                const int64_t tot = (int64_t)ex * ey * m.nz;
+                const int64_t o = m.bbase + (i - g) + (int64_t)ex * ((j - g) + (int64_t)ey * (k - g));
+#pragma unroll
+                for (int n = 0; n < F_NUM; n++) reg_add(A.fb[0], o, tot, n, A.weight, A.add_flux, F[n]);
+              }
+            } else {
+              if ((fj < ty || j == I.yhi + 1) && i >= g && i < g + m.nx && j >= g && j <= g + m.ny && k >= g &&
+                  k < g + m.nz) {
+                const int ex = m.nx, ey = m.ny + 1;
+                const int64_t tot = (int64_t)ex * ey * m.nz;
+                const int64_t o = m.bbase + (i - g) + (int64_t)ex * ((j - g) + (int64_t)ey * (k - g));
+#pragma unroll
+                for (int n = 0; n < F_NUM; n++) reg_add(A.fb[1], o, tot, n, A.weight, A.add_flux, F[n]);
+              }
             }
           }
         }
-        if (k - 1 >= I.z0) {
-          // the update of cell (i, j, k-1), FP64 (hy_rk_updateSoln.F90-mc:227-287)
-          const int64_t c = colc + (int64_t)(k - 1) * sz;
-          double Au[NF], Bu[NF];
-          const double d0 = u0p[V_DENS * cs + c];
-          Au[F_MASS] = d0;
-          Au[F_XMOM] = d0 * u0p[V_VELX * cs + c];
-          Au[F_YMOM] = d0 * u0p[V_VELY * cs + c];
-          Au[F_ZMOM] = d0 * u0p[V_VELZ * cs + c];
-          Au[F_ENER] = d0 * u0p[V_ENER * cs + c];
-          const double ds = sin[V_DENS * cs + c];
-          Bu[F_MASS] = ds;
-          Bu[F_XMOM] = ds * sin[V_VELX * cs + c];
-          Bu[F_YMOM] = ds * sin[V_VELY * cs + c];
-          Bu[F_ZMOM] = ds * sin[V_VELZ * cs + c];
-          Bu[F_ENER] = ds * sin[V_ENER * cs + c];
-#pragma unroll
-          for (int s = 0; s < NSM; s++) {
-            Au[F_NUM + s] = (s < ns) ? u0p[(V_NPROP + s) * cs + c] * d0 : 0.;
-            Bu[F_NUM + s] = (s < ns) ? sin[(V_NPROP + s) * cs + c] * ds : 0.;
-          }
-          double Sg[NF];
-#pragma unroll
-          for (int n = 0; n < NF; n++) Sg[n] = 0.;
-          if (P.use_grav) {
-            Sg[F_XMOM] = Bu[F_MASS] * P.grav[0];
-            Sg[F_YMOM] = Bu[F_MASS] * P.grav[1];
-            Sg[F_ZMOM] = Bu[F_MASS] * P.grav[2];
-            Sg[F_ENER] = Bu[F_XMOM] * P.grav[0] + Bu[F_YMOM] * P.grav[1] + Bu[F_ZMOM] * P.grav[2];
-          }
-#pragma unroll
-          for (int n = 0; n < NF; n++) {
-            double inner_n = dUxy[n] - dtdz * ((double)Hz[n] - (double)Hprev[n]);
-            inner_n = inner_n + dt * (0.0 + Sg[n]);
-            Bu[n] = A.c1 * Au[n] + A.c2 * Bu[n] + A.c3 * inner_n;
-          }
-          const double dens = fmax2(Bu[F_MASS], P.smlrho);
-          const double vx = Bu[F_XMOM] / dens, vy = Bu[F_YMOM] / dens, vz = Bu[F_ZMOM] / dens;
-          const double ener = fmax2(P.smalle, Bu[F_ENER] / dens);
-          const double ekin = .5 * (vx * vx + vy * vy + vz * vz);
-          const double eint = fmax2(P.smalle, ener - ekin);
-#pragma unroll
-          for (int s = 0; s < NSM; s++)
-            if (s < ns) sout[(V_NPROP + s) * cs + c] = Bu[F_NUM + s] / dens;
-          EosOut e = eos_dens_ei(P, dens, vx, vy, vz, ener, eint);
-          sout[V_DENS * cs + c] = dens;
-          sout[V_VELX * cs + c] = vx;
-          sout[V_VELY * cs + c] = vy;
-          sout[V_VELZ * cs + c] = vz;
-          sout[V_PRES * cs + c] = e.pres;
-          sout[V_GAMC * cs + c] = P.gamma;
-          sout[V_EINT * cs + c] = e.eint;
-          sout[V_ENER * cs + c] = e.ener;
-          if (A.final_stage) {
-            sout[V_TEMP * cs + c] = e.temp;
-            sout[V_GAME * cs + c] = e.game;
-          }
-        }
-#pragma unroll
-        for (int n = 0; n < NF; n++) Hprev[n] = Hz[n];
-      }
-      if (inner) {
-#pragma unroll
-        for (int v = 0; v < NV; v++) upz[v] = upzk[v];
-      }
-      // x/y part of level k's update, consumed at the next level
-      if (inner && xyk) {
-#pragma unroll
-        for (int n = 0; n < NF; n++) {
-          double d = -(dtdx * ((double)xex[n * NT + tid + 1] - (double)xex[n * NT + tid]));
-          dUxy[n] = d - dtdy * ((double)yex[n * NT + tid + W] - (double)yex[n * NT + tid]);
-        }
       }
       __syncthreads();
+
+      // ---- P5: update of level k-1 (hydro part), x/y differences of level k
+      const bool upd = col && k - 1 >= I.z0 && k - 1 <= I.z1;
+      double d0 = 0., ds = 0., dens1 = 1.;
+      if (upd) {
+        // FP64 (hy_rk_updateSoln.F90-mc:227-287)
+        const int64_t c = colc + (int64_t)(k - 1) * sz;
+        double Au[F_NUM], Bu[F_NUM];
+        d0 = u0p[V_DENS * cs + c];
+        Au[F_MASS] = d0;
+        Au[F_XMOM] = d0 * u0p[V_VELX * cs + c];
+        Au[F_YMOM] = d0 * u0p[V_VELY * cs + c];
+        Au[F_ZMOM] = d0 * u0p[V_VELZ * cs + c];
+        Au[F_ENER] = d0 * u0p[V_ENER * cs + c];
+        ds = sin[V_DENS * cs + c];
+        Bu[F_MASS] = ds;
+        Bu[F_XMOM] = ds * sin[V_VELX * cs + c];
+        Bu[F_YMOM] = ds * sin[V_VELY * cs + c];
+        Bu[F_ZMOM] = ds * sin[V_VELZ * cs + c];
+        Bu[F_ENER] = ds * sin[V_ENER * cs + c];
+        double Sg[F_NUM];
+#pragma unroll
+        for (int n = 0; n < F_NUM; n++) Sg[n] = 0.;
+        if (P.use_grav) {
+          Sg[F_XMOM] = Bu[F_MASS] * P.grav[0];
+          Sg[F_YMOM] = Bu[F_MASS] * P.grav[1];
+          Sg[F_ZMOM] = Bu[F_MASS] * P.grav[2];
+          Sg[F_ENER] = Bu[F_XMOM] * P.grav[0] + Bu[F_YMOM] * P.grav[1] + Bu[F_ZMOM] * P.grav[2];
+        }
+#pragma unroll
+        for (int n = 0; n < F_NUM; n++) {
+          double inner_n = dUxy[n] - dtdz * ((double)Hz[n] - (double)Hprev[n]);
+          inner_n = inner_n + dt * (0.0 + Sg[n]);
+          Bu[n] = A.c1 * Au[n] + A.c2 * Bu[n] + A.c3 * inner_n;
+        }
+        const double dens = fmax2(Bu[F_MASS], P.smlrho);
+        const double vx = Bu[F_XMOM] / dens, vy = Bu[F_YMOM] / dens, vz = Bu[F_ZMOM] / dens;
+        const double ener = fmax2(P.smalle, Bu[F_ENER] / dens);
+        const double ekin = .5 * (vx * vx + vy * vy + vz * vz);
+        const double eint = fmax2(P.smalle, ener - ekin);
+        EosOut e = eos_dens_ei(P, dens, vx, vy, vz, ener, eint);
+        sout[V_DENS * cs + c] = dens;
+        sout[V_VELX * cs + c] = vx;
+        sout[V_VELY * cs + c] = vy;
+        sout[V_VELZ * cs + c] = vz;
+        sout[V_PRES * cs + c] = e.pres;
+        sout[V_GAMC * cs + c] = P.gamma;
+        sout[V_EINT * cs + c] = e.eint;
+        sout[V_ENER * cs + c] = e.ener;
+        if (A.final_stage) {
+          sout[V_TEMP * cs + c] = e.temp;
+          sout[V_GAME * cs + c] = e.game;
+        }
+        dens1 = dens;
+      }
+      if (col) {
+        if (k >= I.z0) {
+#pragma unroll
+          for (int n = 0; n < F_NUM; n++) Hprev[n] = Hz[n];
+        }
+        if (xyk) {
+          const int ix = cy * (tx + 1) + cx, iy = cy * tx + cx;
+#pragma unroll
+          for (int n = 0; n < F_NUM; n++) {
+            const double d = -(dtdx * ((double)fxb[n * nfx + ix + 1] - (double)fxb[n * nfx + ix]));
+            dUxy[n] = d - dtdy * ((double)fyb[n * nfy + iy + tx] - (double)fyb[n * nfy + iy]);
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < NH; v++) upz[v] = upzk[v];
+      }
+
+      // ---- mass scalars, CH at a time (Hydro.ini:238-247)
+#pragma unroll
+      for (int c0 = 0; c0 < NSM; c0 += CH) {
+        if (c0 < ns) {
+          const int nc = ns - c0 < CH ? ns - c0 : CH;
+          // S1: reconstruction
+          Real upsk[CH], ums[CH];
+#pragma unroll
+          for (int j = 0; j < CH; j++) { upsk[j] = Real(0.); ums[j] = Real(0.); }
+          {
+            const int nrxp = rup64(nrx);
+            const int utot = Cp + (xyk ? nrxp + nry : 0);
+            for (int u = tid; u < utot; u += NT) {
+              if (u < Cp) {
+                if (u < C) {
+                  const Real fl = use_flat ? flatb[fcc] : Real(1.);
+#pragma unroll
+                  for (int j = 0; j < CH; j++)
+                    if (j < nc) recon1<RECON, Real>(wsc[c0 + j], fl, upsk[j], ums[j]);
+                }
+                continue;
+              }
+              int w = u - Cp;
+              const bool isx = w < nrxp;
+              if (isx && w >= nrx) continue;
+              if (!isx) w -= nrxp;
+              int rx, ry, st, nr;
+              Real *om, *op;
+              if (isx) { ry = fquot(w, d_tx2, rx); rx -= 1; st = 1; nr = nrx; om = xm; op = xp; }
+              else { ry = fquot(w, d_tx, rx) - 1; st = PW; nr = nry; om = ym; op = yp; }
+              const int pc = (ry + 3) * PW + (rx + 3);
+              const Real fl = use_flat ? flatb[(ry + 1) * (tx + 2) + (rx + 1)] : Real(1.);
+              for (int j = 0; j < nc; j++) {
+                const Real* r = plane + (NH + c0 + j) * psz + pc;
+                const Real u5[5] = {r[-2 * st], r[-st], r[0], r[st], r[2 * st]};
+                Real up, um;
+                recon1<RECON, Real>(u5, fl, up, um);
+                om[j * nr + w] = um;
+                op[j * nr + w] = up;
+              }
+            }
+          }
+          __syncthreads();
+          // S2: upwinded scalar fluxes
+          Real Hs[CH];
+#pragma unroll
+          for (int j = 0; j < CH; j++) Hs[j] = Real(0.);
+          if (col && k >= I.z0) {
+#pragma unroll
+            for (int j = 0; j < CH; j++) {
+              if (j < nc) {
+                if (Hz[F_MASS] > Real(0.)) Hs[j] = upzs[c0 + j] * Hz[F_MASS];
+                else Hs[j] = ums[j] * Hz[F_MASS];
+              }
+            }
+            if (regs && colin && k >= g && k <= g + m.nz) {
+              const int64_t tot = (int64_t)m.nx * m.ny * (m.nz + 1);
+              const int64_t o = m.bbase + (ci - g) + (int64_t)m.nx * ((cj - g) + (int64_t)m.ny * (k - g));
+#pragma unroll
+              for (int j = 0; j < CH; j++)
+                if (j < nc) reg_add(A.fb[2], o, tot, F_NUM + c0 + j, A.weight, A.add_flux, Hs[j]);
+            }
+          }
+          if (xyk) {
+            for (int u = tid; u < nfx + nfy; u += NT) {
+              const bool isx = u < nfx;
+              const int w = isx ? u : u - nfx;
+              int fi, fj, il, ir, nr;
+              const Real *bp, *bm;
+              Real fm;
+              if (isx) {
+                fj = fquot(w, d_tx1, fi);
+                il = fj * (tx + 2) + fi; ir = il + 1; nr = nrx; bp = xp; bm = xm; fm = mxb[w];
+              } else {
+                fj = fquot(w, d_tx, fi);
+                il = fj * tx + fi; ir = il + tx; nr = nry; bp = yp; bm = ym; fm = myb[w];
+              }
+              Real* fo = isx ? fxb : fyb;
+              const int nf = isx ? nfx : nfy;
+              for (int j = 0; j < nc; j++) {
+                Real f;
+                if (fm > Real(0.)) f = bp[j * nr + il] * fm;
+                else f = bm[j * nr + ir] * fm;
+                fo[j * nf + w] = f;
+              }
+              if (regs) {
+                const int i = I.x0 + fi, jj = I.y0 + fj;
+                if (isx) {
+                  if ((fi < tx || i == I.xhi + 1) && i >= g && i <= g + m.nx && jj >= g && jj < g + m.ny && k >= g &&
+                      k < g + m.nz) {
+                    const int ex = m.nx + 1, ey = m.ny;
+                    const int64_t tot = (int64_t)ex * ey * m.nz;
+                    const int64_t o = m.bbase + (i - g) + (int64_t)ex * ((jj - g) + (int64_t)ey * (k - g));
+                    for (int j = 0; j < nc; j++)
+                      reg_add(A.fb[0], o, tot, F_NUM + c0 + j, A.weight, A.add_flux, fo[j * nf + w]);
+                  }
+                } else {
+                  if ((fj < ty || jj == I.yhi + 1) && i >= g && i < g + m.nx && jj >= g && jj <= g + m.ny && k >= g &&
+                      k < g + m.nz) {
+                    const int ex = m.nx, ey = m.ny + 1;
+                    const int64_t tot = (int64_t)ex * ey * m.nz;
+                    const int64_t o = m.bbase + (i - g) + (int64_t)ex * ((jj - g) + (int64_t)ey * (k - g));
+                    for (int j = 0; j < nc; j++)
+                      reg_add(A.fb[1], o, tot, F_NUM + c0 + j, A.weight, A.add_flux, fo[j * nf + w]);
+                  }
+                }
+              }
+            }
+          }
+          __syncthreads();
+          // S3: scalar update of level k-1, x/y differences of level k
+          if (upd) {
+            const int64_t c = colc + (int64_t)(k - 1) * sz;
+#pragma unroll
+            for (int j = 0; j < CH; j++) {
+              if (j < nc) {
+                const int n = F_NUM + c0 + j;
+                const double Au = u0p[(V_NPROP + c0 + j) * cs + c] * d0;
+                double Bu = sin[(V_NPROP + c0 + j) * cs + c] * ds;
+                double inner_n = dUxy[n] - dtdz * ((double)Hs[j] - (double)Hprev[n]);
+                inner_n = inner_n + dt * (0.0 + 0.0);
+                Bu = A.c1 * Au + A.c2 * Bu + A.c3 * inner_n;
+                sout[(V_NPROP + c0 + j) * cs + c] = Bu / dens1;
+              }
+            }
+          }
+          if (col) {
+#pragma unroll
+            for (int j = 0; j < CH; j++) {
+              if (j < nc) {
+                const int n = F_NUM + c0 + j;
+                if (k >= I.z0) Hprev[n] = Hs[j];
+                if (xyk) {
+                  const int ix = cy * (tx + 1) + cx, iy = cy * tx + cx;
+                  const double d = -(dtdx * ((double)fxb[j * nfx + ix + 1] - (double)fxb[j * nfx + ix]));
+                  dUxy[n] = d - dtdy * ((double)fyb[j * nfy + iy + tx] - (double)fyb[j * nfy + iy]);
+                }
+                upzs[c0 + j] = upsk[j];
+              }
+            }
+          }
+        }
+      }
     }
   }
 }
@@ -422,7 +757,8 @@ __global__ void k_reset_counters(int* c, int n) {
 
 template <int NSM, int RECON, int RIEMANN>
 hipError_t launch_t(const FusedArgs& A, int grid, size_t lds, hipStream_t st) {
-  hipLaunchKernelGGL((k_fused<NSM, RECON, RIEMANN>), dim3(grid), dim3(FUSED_THREADS), lds, st, A);
+  hipLaunchKernelGGL((FUSED_NAME(k_fused, FUSED_SFX)<NSM, RECON, RIEMANN>), dim3(grid), dim3(FUSED_THREADS), lds, st,
+                     A);
   return hipGetLastError();
 }
 
@@ -443,13 +779,8 @@ hipError_t launch_r(const FusedArgs& A, int grid, size_t lds, hipStream_t st) {
 
 }  // namespace
 
-#define FUSED_NAME2(a, b) a##b
-#define FUSED_NAME(a, b) FUSED_NAME2(a, b)
-#define FUSED_SFX SPARK_FUSED_SUFFIX
-
-size_t FUSED_NAME(fused_lds_bytes, FUSED_SFX)(int ns, int max_plane) {
-  const int nv = R_NUM + (ns > 0 ? 8 : 0);
-  return sizeof(Real) * (size_t)nv * ((size_t)max_plane + 2 * FUSED_THREADS);
+size_t FUSED_NAME(fused_lds_bytes, FUSED_SFX)(int ns, int tx, int ty) {
+  return sizeof(Real) * (size_t)lds_layout(tx, ty, NH + ns).total;
 }
 
 hipError_t FUSED_NAME(fused_reset, FUSED_SFX)(hipStream_t st, int* counters) {
@@ -463,10 +794,10 @@ hipError_t FUSED_NAME(fused_stage, FUSED_SFX)(const FusedArgs& A, int grid, size
 
 hipError_t FUSED_NAME(fused_occupancy, FUSED_SFX)(int ns, size_t lds, int* blocks_per_cu) {
   if (ns == 0)
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_fused<0, RECON_WENO5, RIEMANN_HLLC>,
-                                                        FUSED_THREADS, lds);
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_fused<8, RECON_WENO5, RIEMANN_HLLC>,
-                                                      FUSED_THREADS, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        blocks_per_cu, FUSED_NAME(k_fused, FUSED_SFX)<0, RECON_WENO5, RIEMANN_HLLC>, FUSED_THREADS, lds);
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      blocks_per_cu, FUSED_NAME(k_fused, FUSED_SFX)<8, RECON_WENO5, RIEMANN_HLLC>, FUSED_THREADS, lds);
 }
 
 }  // namespace spark

@@ -9,7 +9,6 @@ namespace spark {
 
 constexpr int FUSED_THREADS = 256;    // workgroup size of the fused stage kernel
 constexpr int FUSED_XCDS = 8;         // MI355X: 8 XCDs, workgroups dispatched round-robin
-constexpr int FUSED_MAX_PLANE = 512;  // (tx + 6) * (ty + 6) bound of the LDS plane
 
 // one work item: a (tx x ty) column tile of one block's stage region,
 // marched over z0..z1 (guard-celled box coordinates)
@@ -32,6 +31,11 @@ struct FusedArgs {
   const double* Sin;       // stage input state
   double* Sout;            // stage output state (never equal to Sin)
   double* fb[3];           // flux registers (nullptr when not correcting)
+  // uniform block grid, guards not refilled between stages: per block the
+  // 27 pool offsets (BlockMeta::base) of the blocks around it, -1 where the
+  // domain ends (outflow).  Cells outside a block's interior are read from
+  // those neighbours.  nullptr: read the block's own guard cells.
+  const int64_t* nbr;
   double weight;
   int add_flux;
   const double* dtp;
@@ -40,12 +44,13 @@ struct FusedArgs {
   int* err;
 };
 
-// the fused stage kernel in FP32 (_f32) and FP64 (_f64) arithmetic
-size_t fused_lds_bytes_f32(int ns, int max_plane);
+// the fused stage kernel in FP32 (_f32) and FP64 (_f64) arithmetic; the LDS
+// of a work item depends on its tile (tx x ty, tx * ty <= FUSED_THREADS)
+size_t fused_lds_bytes_f32(int ns, int tx, int ty);
 hipError_t fused_reset_f32(hipStream_t st, int* counters);
 hipError_t fused_stage_f32(const FusedArgs& A, int grid, size_t lds, hipStream_t st);
 hipError_t fused_occupancy_f32(int ns, size_t lds, int* blocks_per_cu);
-size_t fused_lds_bytes_f64(int ns, int max_plane);
+size_t fused_lds_bytes_f64(int ns, int tx, int ty);
 hipError_t fused_reset_f64(hipStream_t st, int* counters);
 hipError_t fused_stage_f64(const FusedArgs& A, int grid, size_t lds, hipStream_t st);
 hipError_t fused_occupancy_f64(int ns, size_t lds, int* blocks_per_cu);

@@ -95,6 +95,7 @@ struct spark_batch {
   double* S2 = nullptr;
   int cus = 0;            // compute units of the device
   size_t lds_limit = 0;   // LDS bytes a workgroup may allocate
+  int64_t* nbr_d = nullptr;  // uniform grid: 27 neighbour pool offsets per block
   size_t stage_lds[3] = {0, 0, 0};
   int stage_grid[3] = {0, 0, 0};
   struct StagePlan {
@@ -168,8 +169,14 @@ int required_guard(const spark_params& p) {
   double c[3][3], w[3];
   rk_tables(p, &last, c, la, w);
   const int maxoff = (p.guard_mode == SPARK_GUARD_REFILL) ? 0 : la[0];
-  return maxoff + p.nstencil;
+  // the fused kernel of the batched forms stages a 3-cell halo whatever
+  // the reconstruction's own stencil
+  const bool fused = p.form == SPARK_FORM_FP32_FLUX || p.form == SPARK_FORM_BATCHED_FP64;
+  return maxoff + (fused ? std::max(p.nstencil, 3) : p.nstencil);
 }
+
+bool fused_remap(const spark_batch* b);
+void drop_graph(spark_batch* b);
 
 int run_dt(spark_batch* b, int hist_slot) {
   for (auto& c : b->classes) {
@@ -206,13 +213,15 @@ int run_fill(spark_batch* b, double* pool, bool full, int depth) {
 
 // first-stage guard reach: stencil extent of stage 1 beyond the interior
 int stage1_reach(const spark_batch* b) {
+  // the remapping batched forms only need the layer shockDetect reads
+  if (fused_remap(b)) return 1;
   return (b->p.guard_mode == SPARK_GUARD_REFILL ? 0 : b->limits_array[0]) + b->p.nstencil;
 }
 
 int run_step_begin(spark_batch* b) {
   // SHOK over the cells stage 1 reads (the reference computes NGUARD-1
   // layers; the outer layers are never read before the next refill)
-  const int band = (b->p.guard_mode == SPARK_GUARD_REFILL ? 0 : b->limits_array[0]) + 1;
+  const int band = fused_remap(b) ? 0 : (b->p.guard_mode == SPARK_GUARD_REFILL ? 0 : b->limits_array[0]) + 1;
   for (auto& c : b->classes) {
     PlainLaunch L = make_launch(b, c);
     HIPCHK(spark::plain_shock_detect(L, b->U, b->flat, band));
@@ -242,9 +251,9 @@ int stage_ngoff(const spark_batch* b, int stage) {
 int fused_plan(spark_batch* b);
 int fused_tune(spark_batch* b);
 
-size_t fused_lds(const spark_batch* b, int max_plane) {
-  return b->fused64 ? spark::fused_lds_bytes_f64(b->p.nscalars, max_plane)
-                    : spark::fused_lds_bytes_f32(b->p.nscalars, max_plane);
+size_t fused_lds(const spark_batch* b, int tx, int ty) {
+  return b->fused64 ? spark::fused_lds_bytes_f64(b->p.nscalars, tx, ty)
+                    : spark::fused_lds_bytes_f32(b->p.nscalars, tx, ty);
 }
 
 // persistent grid for a given LDS size: resident workgroups of the whole
@@ -263,6 +272,12 @@ hipError_t fused_launch(const spark_batch* b, const spark::FusedArgs& A, int gri
   return b->fused64 ? spark::fused_stage_f64(A, grid, lds, g_stream) : spark::fused_stage_f32(A, grid, lds, g_stream);
 }
 
+// the batched forms on a uniform grid in refill mode read halo cells from
+// the neighbouring blocks, so no guard fill runs between stages
+bool fused_remap(const spark_batch* b) {
+  return b->fused && b->has_ug && b->p.guard_mode == SPARK_GUARD_REFILL;
+}
+
 spark::FusedArgs fused_args(spark_batch* b, int stage) {
   spark::FusedArgs A;
   memset(&A, 0, sizeof(A));
@@ -276,6 +291,7 @@ spark::FusedArgs fused_args(spark_batch* b, int stage) {
   A.Sin = stage_in(b, stage);
   A.Sout = stage_out(b, stage);
   for (int d = 0; d < 3; d++) A.fb[d] = b->p.flux_correct ? b->fb[d] : nullptr;
+  A.nbr = fused_remap(b) ? b->nbr_d : nullptr;
   A.weight = b->weights[stage - 1];
   A.add_flux = stage > 1;
   A.dtp = b->dtp;
@@ -331,7 +347,7 @@ int run_full_step(spark_batch* b) {
   const bool refill = b->p.guard_mode == SPARK_GUARD_REFILL;
   for (int s = 1; s <= b->last_stage; s++) {
     if ((rc = run_stage(b, s, refill ? 0 : b->limits_array[s - 1]))) return rc;
-    if (refill && s < b->last_stage)
+    if (refill && s < b->last_stage && !fused_remap(b))
       if ((rc = run_fill(b, stage_out(b, s), false, b->p.nstencil))) return rc;
   }
   return run_step_end(b);
@@ -341,6 +357,7 @@ void free_batch(spark_batch* b) {
   if (!b) return;
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->nbr_d) (void)hipFree(b->nbr_d);
   for (auto& pl : b->plan) {
     if (pl.items_d) (void)hipFree(pl.items_d);
     if (pl.qoff_d) (void)hipFree(pl.qoff_d);
@@ -367,8 +384,8 @@ void free_batch(spark_batch* b) {
 // ---------------------------------------------------------------------------
 // batched form: work items and tile autotuning
 
-// candidate (tx, ty) tiles of an E x E region: near-equal splits that fit
-// the workgroup ((tx+2)(ty+2) <= threads) and the LDS plane
+// candidate (tx, ty) tiles of an ex x ey region: near-equal splits whose
+// columns fit the workgroup (tx * ty <= threads) and whose LDS fits
 std::vector<std::pair<int, int>> tile_candidates(const spark_batch* b, int ex, int ey) {
   std::vector<std::pair<int, int>> out;
   std::vector<int> xs, ys;
@@ -383,19 +400,18 @@ std::vector<std::pair<int, int>> tile_candidates(const spark_batch* b, int ex, i
   for (int tx : xs)
     for (int ty : ys) {
       if (tx < 2 || ty < 2) continue;
-      if ((tx + 2) * (ty + 2) > spark::FUSED_THREADS) continue;
-      if ((tx + 6) * (ty + 6) > spark::FUSED_MAX_PLANE) continue;
-      if (fused_lds(b, (tx + 6) * (ty + 6)) > b->lds_limit) continue;
+      if (tx * ty > spark::FUSED_THREADS) continue;
+      if (fused_lds(b, tx, ty) > b->lds_limit) continue;
       out.push_back({tx, ty});
     }
-  // rank by workgroup passes (tiles per region), then by idle lanes
+  // rank by work items per region, then by the halo cells a tile stages
   auto tiles = [&](const std::pair<int, int>& t) {
     return ((ex + t.first - 1) / t.first) * ((ey + t.second - 1) / t.second);
   };
   std::sort(out.begin(), out.end(), [&](const std::pair<int, int>& a, const std::pair<int, int>& c) {
     const int ta = tiles(a), tc = tiles(c);
     if (ta != tc) return ta < tc;
-    return (a.first + 2) * (a.second + 2) < (c.first + 2) * (c.second + 2);
+    return (a.first + 6) * (a.second + 6) < (c.first + 6) * (c.second + 6);
   });
   if (out.size() > 6) out.resize(6);
   return out;
@@ -465,15 +481,15 @@ int build_stage_plan(spark_batch* b, int stage, const std::vector<int>* only_blo
   HIPCHK(hipMemcpy(pl.items_d, pl.items.data(), sizeof(spark::FusedItem) * pl.items.size(), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(pl.qoff_d, qoff, sizeof(qoff), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(pl.qlen_d, qlen, sizeof(qlen), hipMemcpyHostToDevice));
-  int max_plane = 1;
-  for (const auto& it : pl.items) max_plane = std::max(max_plane, (it.tx + 6) * (it.ty + 6));
-  b->stage_lds[stage - 1] = fused_lds(b, max_plane);
+  size_t lds = 0;
+  for (const auto& it : pl.items) lds = std::max(lds, fused_lds(b, it.tx, it.ty));
+  b->stage_lds[stage - 1] = lds;
   b->stage_grid[stage - 1] = fused_grid_for(b, b->stage_lds[stage - 1]);
   return SPARK_OK;
 }
 
 // Autotuner: for every (shape class, stage region) time the candidate
-// tiles on (up to 64 of) the batch's own blocks and keep the fastest.
+// tiles on the batch's own blocks of that class and keep the fastest.
 // Runs once, on the first step, with the real data packed.
 int fused_tune(spark_batch* b) {
   std::vector<int> ngoffs;
@@ -484,7 +500,9 @@ int fused_tune(spark_batch* b) {
   HIPCHK(hipEventCreate(&e1));
   for (size_t ci = 0; ci < b->classes.size(); ci++) {
     const ShapeClass& c = b->classes[ci];
-    std::vector<int> sample(c.ids.begin(), c.ids.begin() + std::min<size_t>(64, c.ids.size()));
+    // time on the whole class: a sample smaller than the device would favour
+    // tiles that make more, smaller work items
+    std::vector<int> sample(c.ids.begin(), c.ids.end());
     for (int ngoff : ngoffs) {
       auto cands = tile_candidates(b, c.nx + 2 * ngoff, c.ny + 2 * ngoff);
       if (cands.empty()) return fail(SPARK_ERR_ARG, "no tile fits a %d x %d region", c.nx + 2 * ngoff, c.ny + 2 * ngoff);
@@ -536,6 +554,13 @@ int fused_plan(spark_batch* b) {
     if ((rc = build_stage_plan(b, s, nullptr))) return rc;
   b->planned = true;
   return SPARK_OK;
+}
+
+void drop_graph(spark_batch* b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
 }
 
 }  // namespace
@@ -746,8 +771,8 @@ int spark_batch_create(const spark_params* p, int nblocks, const spark_block_des
     HIPCHK(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, g_device));
     HIPCHK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, g_device));
     b->lds_limit = (size_t)lim;
-    if (fused_lds(b, 8 * 8) > b->lds_limit) {
-      const size_t need = fused_lds(b, 64), have = b->lds_limit;
+    if (fused_lds(b, 4, 4) > b->lds_limit) {
+      const size_t need = fused_lds(b, 4, 4), have = b->lds_limit;
       free_batch(b);
       return fail(SPARK_ERR_ARG, "batched form: %zu bytes of LDS needed, the device allows %zu", need, have);
     }
@@ -843,8 +868,28 @@ int spark_set_uniform_grid(spark_batch* b, int nbx, int nby, int nbz) {
   if (b->classes.size() != 1) return fail(SPARK_ERR_ARG, "uniform grid needs blocks of one shape");
   if ((b->p.ndim < 2 && nby != 1) || (b->p.ndim < 3 && nbz != 1))
     return fail(SPARK_ERR_ARG, "uniform grid extent in an unused dimension");
+  if (b->fused && b->p.guard_mode == SPARK_GUARD_REFILL) {
+    const BlockMeta& m0 = b->meta_h[0];
+    if (m0.nx < 3 || m0.ny < 3 || m0.nz < 3)
+      return fail(SPARK_ERR_ARG, "batched form on a uniform grid needs blocks of >= 3 cells per side");
+  }
   b->ug[0] = nbx; b->ug[1] = nby; b->ug[2] = nbz;
+  // neighbour table (blocks numbered x fastest)
+  std::vector<int64_t> nb((size_t)b->nblk * 27, -1);
+  for (int id = 0; id < b->nblk; id++) {
+    const int bx = id % nbx, by = (id / nbx) % nby, bz = id / (nbx * nby);
+    for (int oz = -1; oz <= 1; oz++)
+      for (int oy = -1; oy <= 1; oy++)
+        for (int ox = -1; ox <= 1; ox++) {
+          const int x = bx + ox, y = by + oy, z = bz + oz;
+          if (x < 0 || x >= nbx || y < 0 || y >= nby || z < 0 || z >= nbz) continue;
+          nb[(size_t)id * 27 + 13 + ox + 3 * oy + 9 * oz] = b->meta_h[x + nbx * (y + nby * z)].base;
+        }
+  }
+  if (!b->nbr_d) HIPCHK(hipMalloc((void**)&b->nbr_d, sizeof(int64_t) * nb.size()));
+  HIPCHK(hipMemcpy(b->nbr_d, nb.data(), sizeof(int64_t) * nb.size(), hipMemcpyHostToDevice));
   b->has_ug = true;
+  drop_graph(b);
   return SPARK_OK;
 }
 
@@ -914,12 +959,6 @@ static int one_step(spark_batch* b) {
   return run_full_step(b);
 }
 
-static void drop_graph(spark_batch* b) {
-  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  b->gexec = nullptr;
-  b->graph = nullptr;
-}
 
 int spark_advance(spark_batch* b, int nsteps, double* dts) {
   if (!b) return fail(SPARK_ERR_ARG, "spark_advance: null batch");

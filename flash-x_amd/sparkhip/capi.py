@@ -14,7 +14,8 @@ from pathlib import Path
 import numpy as np
 
 PKG_ROOT = Path(__file__).resolve().parent.parent          # flash-x_amd/
-LIB_PATH = PKG_ROOT / "libspark_hip.so"
+# SPARK_HIP_LIB selects another build of the library (same ABI)
+LIB_PATH = Path(os.environ.get("SPARK_HIP_LIB", str(PKG_ROOT / "libspark_hip.so")))
 
 SPARK_OK = 0
 SPARK_NPROP = 11
