@@ -1,12 +1,21 @@
 #!/usr/bin/env python3
 """Benchmark of the Spark RK stage on one MI355X.
 
-Default workload: 3D Sedov blast, uniform 8^3 blocks of 16^3 interior
-cells (128^3 zones), 6 guard cells refilled on the device between stages,
-WENO5 + HLLC, RK3, CFL 0.8, gamma 1.4, plain FP64 form.  A step is one
-full RK3 step of all blocks (CFL dt on the device, guard fill, 3 stages).
-Inputs are resident in HBM before the timed region; the timed region is
-K device-resident steps bracketed by device synchronisations.
+Workloads (--workload):
+  sedov128     (default) 3D Sedov blast, uniform 8^3 blocks of 16^3 interior
+               cells (128^3 zones), 6 guard cells refilled on the device
+               between stages, WENO5 + HLLC, RK3, CFL 0.8, gamma 1.4.
+  batched4096  4096 independent 3D blocks of 8^3 / 16^3 / 32^3 cells
+               (p = 0.25 / 0.625 / 0.125), levels 0-4 (dx = 2^-l / 16),
+               smooth random fields plus a Mach 2-5 planar shock in every
+               16th block, 10 guard cells (telescoping RK3, frozen guards).
+               The FP32-flux accuracy after --accuracy-steps steps is
+               reported against the plain FP64 form.
+A step is one full RK3 step of all blocks (CFL dt on the device, guard
+fill where the workload has one, 3 stages).  Inputs are resident in HBM
+before the timed region; the timed region is K device-resident steps
+bracketed by device synchronisations.  The headline value is the plain
+FP64 form; the batched FP64 and FP32-flux forms are timed beside it.
 
 Prints one JSON line (see the metric/roofline/cpu_baseline fields).
 """
@@ -24,20 +33,40 @@ METRIC = "zone-updates/s per RK3 step: plain FP64 vs batched FP32-flux; % HBM ro
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
 
+WORKLOADS = {
+    "sedov128": "3D Sedov blast, 8^3 blocks of 16^3 (128^3 zones), 6 guard cells refilled on the device between "
+                "stages, WENO5-Z + HLLC, RK3, CFL 0.8, gamma 1.4",
+    "batched4096": "4096 independent 3D blocks of 8^3/16^3/32^3 (p .25/.625/.125, seeds 20261202/20261203), "
+                   "levels 0-4, Fourier fields + Mach 2-5 shocks every 16th block, 10 guard cells, telescoping RK3, "
+                   "WENO5-Z + HLLC, CFL 0.8, gamma 1.4",
+}
+
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 50 sedov128, 10 batched4096)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 5 sedov128, 2 batched4096)")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="sedov128")
     ap.add_argument("--profile-steps", type=int, default=10, help="steps of the per-kernel timing pass (>= 1)")
+    ap.add_argument("--accuracy-steps", type=int, default=None,
+                    help="steps of the FP32-flux vs FP64 accuracy run (default 0 sedov128, 100 batched4096)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--forms", default="batched_fp64,fp32_flux",
                     help="comma list of the other forms timed beside plain FP64 (empty: none)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration) and exit")
-    return ap.parse_args()
+    a = ap.parse_args()
+    big = a.workload == "batched4096"
+    if a.steps is None:
+        a.steps = 10 if big else 50
+    if a.warmup is None:
+        a.warmup = 2 if big else 5
+    if a.accuracy_steps is None:
+        a.accuracy_steps = 100 if big else 0
+    a.profile_steps = max(1, a.profile_steps)
+    return a
 
 
 def calibrate_copy(mib: int):
@@ -53,35 +82,73 @@ def calibrate_copy(mib: int):
     print(json.dumps({"calibrate_copy_bytes": n * 8, "launches": 3}), flush=True)
 
 
-def cpu_baseline(target_s: float):
-    """The oracle (CPU restatement, OpenMP over blocks) on the same workload,
-    bounded to about target_s seconds of wall time."""
+def make_problem(workload):
+    from sparkhip import problems
+    if workload == "batched4096":
+        return problems.batched(nblocks=4096, nguard=10)
+    return problems.sedov3d(nb=8, n=16, nguard=6, guard_mode="refill")
+
+
+def cpu_baseline(workload, target_s: float):
+    """The oracle (CPU restatement, OpenMP over blocks) on a bounded sample of
+    the same workload, about target_s seconds of wall time."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle
     from sparkhip import problems
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    prob = problems.sedov3d(nb=8, n=16, nguard=6, guard_mode="refill")
+    if workload == "batched4096":
+        prob = problems.batched(nblocks=256, nguard=10)
+        run = lambda n: pyoracle.batch_advance(prob.cfg, prob.blocks, prob.arrays, n, nthreads=threads)
+        what = f"the first 256 of the 4096 mixed blocks ({prob.zones} zones)"
+    else:
+        prob = problems.sedov3d(nb=8, n=16, nguard=6, guard_mode="refill")
+        run = lambda n: pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, prob.arrays, n, refill=True,
+                                            nthreads=threads)
+        what = "3D Sedov 128^3 (8^3 blocks of 16^3, 6 guards, refill)"
     t0 = time.perf_counter()
-    pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, prob.arrays, 1, refill=True, nthreads=threads)
+    run(1)
     one = time.perf_counter() - t0
     nsteps = int(max(1, min(30, (target_s - one) / max(one, 1e-3))))
     t0 = time.perf_counter()
-    pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, prob.arrays, nsteps, refill=True, nthreads=threads)
+    run(nsteps)
     el = time.perf_counter() - t0
     return {
         "value": prob.zones * nsteps / el,
         "unit": "zone-updates/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"3D Sedov 128^3 (8^3 blocks of 16^3, 6 guards, refill), {nsteps} RK3 steps after 1 untimed, "
-                  f"{el:.1f} s wall, OpenMP over blocks",
+        "sample": f"{what}, {nsteps} RK3 steps after 1 untimed, {el:.1f} s wall, OpenMP over blocks",
     }
+
+
+def norms(prob, got, ref):
+    """Volume-weighted relative L1 and relative L-infinity of rho, rho*u, E."""
+    import numpy as np
+    from sparkhip import capi
+    g = prob.cfg["nguard"]
+    V = capi.VI
+    fields = {
+        "rho": lambda a: a[..., V["dens"]],
+        "rhou": lambda a: a[..., V["dens"]] * a[..., V["velx"]],
+        "E": lambda a: a[..., V["dens"]] * a[..., V["ener"]],
+    }
+    out = {}
+    for name, f in fields.items():
+        num = den = linf = scale = 0.0
+        for b, x, r in zip(prob.blocks, got, ref):
+            vol = b["dx"] * b["dy"] * b["dz"]
+            fx, fr = f(x[g:-g, g:-g, g:-g]), f(r[g:-g, g:-g, g:-g])
+            num += float(np.sum(np.abs(fx - fr))) * vol
+            den += float(np.sum(np.abs(fr))) * vol
+            linf = max(linf, float(np.max(np.abs(fx - fr))))
+            scale = max(scale, float(np.max(np.abs(fr))))
+        out[name] = {"L1": num / den, "Linf": linf / scale}
+    return out
 
 
 def main():
     args = parse()
-    args.profile_steps = max(1, args.profile_steps)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the block set is a single-GPU "
               f"shard and no multi-GPU path is built", file=sys.stderr)
@@ -89,15 +156,20 @@ def main():
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
         return
-    from sparkhip import capi, problems
+    from sparkhip import capi
 
-    prob = problems.sedov3d(nb=8, n=16, nguard=6, guard_mode="refill")
+    prob = make_problem(args.workload)
     capi.init(0)
 
-    def timed(form):
+    def batch(form):
         b = capi.Batch(dict(prob.cfg, form=form), prob.blocks)
         b.pack(prob.arrays)
-        b.set_uniform_grid(*prob.ug)
+        if prob.ug is not None:
+            b.set_uniform_grid(*prob.ug)
+        return b
+
+    def timed(form):
+        b = batch(form)
         if args.warmup:
             b.advance(args.warmup)   # the batched forms tune their tiles here
         b.synchronize()
@@ -137,6 +209,25 @@ def main():
             entry["fused_stage_avg_ms"] = round(k["ms"] / k["launches"], 4)
             entry["fused_stage_valu_tflops"] = round(k["flops"] / (k["ms"] * 1e-3) / 1e12, 2)
         forms[form] = entry
+
+    accuracy = None
+    if args.accuracy_steps > 0:
+        # FP32-flux against the plain FP64 form (bit-exact with the FP64
+        # oracle) after the same number of steps from the same state
+        res = {}
+        for form in ("plain_fp64", "fp32_flux"):
+            bf = batch(form)
+            dts = bf.advance(args.accuracy_steps)
+            out = [a.copy() for a in prob.arrays]
+            bf.unpack(out)
+            bf.close()
+            res[form] = (dts, out)
+        accuracy = {"steps": args.accuracy_steps, "reference": "plain_fp64", "form": "fp32_flux",
+                    "dt_final_rel_diff": float(abs(res["fp32_flux"][0][-1] - res["plain_fp64"][0][-1]) /
+                                               res["plain_fp64"][0][-1]),
+                    **norms(prob, res["fp32_flux"][1], res["plain_fp64"][1])}
+        del res
+
     tot = sum(s["ms"] for s in stats)
     flux = [s for s in stats if s["name"].startswith("k_flux")]
     dom = max(stats, key=lambda s: s["ms"])
@@ -173,9 +264,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "3D Sedov blast, 8^3 blocks of 16^3 (128^3 zones), 6 guard cells refilled on the device "
-                        "between stages, WENO5-Z + HLLC, RK3, CFL 0.8, gamma 1.4, plain FP64 form",
-            "zones": prob.zones, "blocks": len(prob.blocks), "form": "plain_fp64",
+            "workload": WORKLOADS[args.workload] + ", plain FP64 form",
+            "name": args.workload, "zones": prob.zones, "blocks": len(prob.blocks), "form": "plain_fp64",
         },
         "roofline": roofline,
         "path_hbm_frac_algorithmic": round(value * step_bytes / prob.zones / 1e9 / HBM_PEAK_GBS, 5),
@@ -183,8 +273,10 @@ def main():
         "dominant_kernel": dom["name"],
         "forms": forms,
     }
+    if accuracy is not None:
+        line["accuracy"] = accuracy
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds)
     print(json.dumps(line), flush=True)
 
 

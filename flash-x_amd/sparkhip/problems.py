@@ -155,9 +155,15 @@ class _Fourier:
         self.norm = np.sum(np.abs(self.a))
 
     def __call__(self, X, Y, Z):
-        s = np.zeros_like(X)
+        """X, Y, Z: (NZ, NY, NX) cell-centre grids of one block.  Each mode is
+        evaluated as Re(exp(i phi) ex(x) ey(y) ez(z)) from 1D factors."""
+        x, y, z = X[0, 0, :], Y[0, :, 0], Z[:, 0, 0]
+        s = np.zeros(X.shape)
         for k, a, ph in zip(self.k, self.a, self.phi):
-            s += a * np.cos(2 * np.pi * (k[0] * X + k[1] * Y + k[2] * Z) + ph)
+            ex = np.exp(2j * np.pi * k[0] * x)
+            ey = np.exp(2j * np.pi * k[1] * y)
+            ez = (a * np.exp(1j * ph)) * np.exp(2j * np.pi * k[2] * z)
+            s += ((ez[:, None] * ey[None, :])[:, :, None] * ex[None, None, :]).real
         return 0.5 + 0.5 * s / self.norm
 
 

@@ -199,3 +199,20 @@ def test_flux_registers_batched_forms_vs_plain():
         for a, c in zip(regs["plain_fp64"][d], regs["fp32_flux"][d]):
             scale = np.max(np.abs(a)) + 1e-30
             assert np.max(np.abs(a - c)) / scale < 1e-5
+
+
+def test_fp32_flux_100_steps_mixed_blocks_vs_fp64():
+    """The mixed-block workload (sizes 8/16/32, levels 0-4, shocks), 256
+    blocks, 100 steps: FP32-flux against the plain FP64 form (which equals
+    the FP64 oracle bit for bit).  Bar: L1(rho) <= 1e-5."""
+    prob = problems.batched(nblocks=256, nguard=10)
+    nsteps = 100
+    print(f"batched 256 blocks (sizes {prob.info['size_counts']}), {nsteps} steps", flush=True)
+    d64, o64, _, _ = run_gpu(prob, nsteps, "plain_fp64")
+    d32, o32, tiles, cnt = run_gpu(prob, nsteps, "fp32_flux")
+    nm = norms(prob, o32, o64)
+    print(f"tiles {tiles}; dt rel diff {abs(d32[-1] - d64[-1]) / d64[-1]:.2e}; " +
+          ", ".join(f"{k}: L1 {v[0]:.2e} Linf {v[1]:.2e}" for k, v in nm.items()), flush=True)
+    assert cnt[0] == 0
+    assert nm["rho"][0] <= 1e-5
+    assert nm["E"][0] <= 1e-5
