@@ -5,6 +5,13 @@ Workloads (--workload):
   sedov128     (default) 3D Sedov blast, uniform 8^3 blocks of 16^3 interior
                cells (128^3 zones), 6 guard cells refilled on the device
                between stages, WENO5 + HLLC, RK3, CFL 0.8, gamma 1.4.
+  rt_amr       3D Rayleigh-Taylor with 8 mass scalars on a 3-level block
+               hierarchy (4384 leaves of 16^3: 32 / 256 / 4096 per level,
+               256^3-equivalent resolution over x, y in [0, 1], z in
+               [0, 2]), gravity g = -1 as a source, device guard fill over
+               the levels and fine/coarse flux correction every step,
+               10 guard cells (telescoping RK3).  FP32-flux accuracy after
+               --accuracy-steps (default 200) against the plain FP64 form.
   batched4096  4096 independent 3D blocks of 8^3 / 16^3 / 32^3 cells
                (p = 0.25 / 0.625 / 0.125), levels 0-4 (dx = 2^-l / 16),
                smooth random fields plus a Mach 2-5 planar shock in every
@@ -36,6 +43,10 @@ FP64_PEAK_TFS = 78.6
 WORKLOADS = {
     "sedov128": "3D Sedov blast, 8^3 blocks of 16^3 (128^3 zones), 6 guard cells refilled on the device between "
                 "stages, WENO5-Z + HLLC, RK3, CFL 0.8, gamma 1.4",
+    "rt_amr": "3D Rayleigh-Taylor, 8 mass scalars, 3-level block hierarchy of 4384 16^3 leaves (32/256/4096 per "
+              "level, 256^3-equivalent), rho 2 over 1, g = -1 source, hydrostatic p (2.5 at the interface), "
+              "seeded interface modes of amplitude 0.01, device guard fill over levels and fine/coarse flux "
+              "correction every step, 10 guard cells, telescoping RK3, WENO5-Z + HLLC, CFL 0.8, gamma 1.4",
     "batched4096": "4096 independent 3D blocks of 8^3/16^3/32^3 (p .25/.625/.125, seeds 20261202/20261203), "
                    "levels 0-4, Fourier fields + Mach 2-5 shocks every 16th block, 10 guard cells, telescoping RK3, "
                    "WENO5-Z + HLLC, CFL 0.8, gamma 1.4",
@@ -45,12 +56,12 @@ WORKLOADS = {
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 50 sedov128, 10 batched4096)")
-    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 5 sedov128, 2 batched4096)")
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 50 sedov128, 10 otherwise)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 5 sedov128, 2 otherwise)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="sedov128")
     ap.add_argument("--profile-steps", type=int, default=10, help="steps of the per-kernel timing pass (>= 1)")
     ap.add_argument("--accuracy-steps", type=int, default=None,
-                    help="steps of the FP32-flux vs FP64 accuracy run (default 0 sedov128, 100 batched4096)")
+                    help="steps of the FP32-flux vs FP64 accuracy run (default 0 sedov128, 100 batched4096, 200 rt_amr)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--forms", default="batched_fp64,fp32_flux",
@@ -58,13 +69,13 @@ def parse():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration) and exit")
     a = ap.parse_args()
-    big = a.workload == "batched4096"
+    big = a.workload != "sedov128"
     if a.steps is None:
         a.steps = 10 if big else 50
     if a.warmup is None:
         a.warmup = 2 if big else 5
     if a.accuracy_steps is None:
-        a.accuracy_steps = 100 if big else 0
+        a.accuracy_steps = {"sedov128": 0, "batched4096": 100, "rt_amr": 200}[a.workload]
     a.profile_steps = max(1, a.profile_steps)
     return a
 
@@ -86,6 +97,8 @@ def make_problem(workload):
     from sparkhip import problems
     if workload == "batched4096":
         return problems.batched(nblocks=4096, nguard=10)
+    if workload == "rt_amr":
+        return problems.rt_amr()
     return problems.sedov3d(nb=8, n=16, nguard=6, guard_mode="refill")
 
 
@@ -101,6 +114,12 @@ def cpu_baseline(workload, target_s: float):
         prob = problems.batched(nblocks=256, nguard=10)
         run = lambda n: pyoracle.batch_advance(prob.cfg, prob.blocks, prob.arrays, n, nthreads=threads)
         what = f"the first 256 of the 4096 mixed blocks ({prob.zones} zones)"
+    elif workload == "rt_amr":
+        prob = problems.rt_amr(nb0=(1, 1, 2))
+        run = lambda n: pyoracle.amr_advance(prob.cfg, prob.blocks, prob.info["nb0"], prob.info["origins"],
+                                             prob.arrays, n, nthreads=threads)
+        what = (f"the same Rayleigh-Taylor setup on a 1 x 1 x 2 level-1 grid ({len(prob.blocks)} leaves, "
+                f"{prob.zones} zones, guard fill and flux correction included)")
     else:
         prob = problems.sedov3d(nb=8, n=16, nguard=6, guard_mode="refill")
         run = lambda n: pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, prob.arrays, n, refill=True,
@@ -143,7 +162,9 @@ def norms(prob, got, ref):
             den += float(np.sum(np.abs(fr))) * vol
             linf = max(linf, float(np.max(np.abs(fx - fr))))
             scale = max(scale, float(np.max(np.abs(fr))))
-        out[name] = {"L1": num / den, "Linf": linf / scale}
+        # relative norms; absolute ones where the reference field is zero
+        out[name] = {"L1": num / den if den > 0 else num, "Linf": linf / scale if scale > 0 else linf,
+                     "relative": bool(den > 0 and scale > 0)}
     return out
 
 
@@ -166,6 +187,8 @@ def main():
         b.pack(prob.arrays)
         if prob.ug is not None:
             b.set_uniform_grid(*prob.ug)
+        if "origins" in prob.info:
+            b.set_amr_grid(prob.info["nb0"], prob.info["origins"])
         return b
 
     def timed(form):
@@ -179,9 +202,14 @@ def main():
         el = time.perf_counter() - t0
         return b, el
 
+    def note(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    note(f"{args.workload}: {len(prob.blocks)} blocks, {prob.zones} zones")
     # headline: the plain FP64 form
     b, el = timed("plain_fp64")
     value = prob.zones * args.steps / el
+    note(f"plain_fp64 {el / args.steps * 1e3:.3f} ms/step")
 
     # per-kernel HIP-event timing over a separate pass (same stream)
     b.profile(True)
@@ -209,6 +237,7 @@ def main():
             entry["fused_stage_avg_ms"] = round(k["ms"] / k["launches"], 4)
             entry["fused_stage_valu_tflops"] = round(k["flops"] / (k["ms"] * 1e-3) / 1e12, 2)
         forms[form] = entry
+        note(f"{form} {entry['ms_per_step']} ms/step")
 
     accuracy = None
     if args.accuracy_steps > 0:
@@ -222,6 +251,7 @@ def main():
             bf.unpack(out)
             bf.close()
             res[form] = (dts, out)
+            note(f"accuracy run {form}: {args.accuracy_steps} steps, last dt {dts[-1]:.4e}")
         accuracy = {"steps": args.accuracy_steps, "reference": "plain_fp64", "form": "fp32_flux",
                     "dt_final_rel_diff": float(abs(res["fp32_flux"][0][-1] - res["plain_fp64"][0][-1]) /
                                                res["plain_fp64"][0][-1]),

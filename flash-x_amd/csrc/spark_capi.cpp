@@ -96,6 +96,11 @@ struct spark_batch {
   int cus = 0;            // compute units of the device
   size_t lds_limit = 0;   // LDS bytes a workgroup may allocate
   int64_t* nbr_d = nullptr;  // uniform grid: 27 neighbour pool offsets per block
+  bool amr = false;          // block hierarchy declared (spark_set_amr_grid)
+  spark::AmrGrid amr_g{};
+  int* amr_map_d = nullptr;
+  int* amr_org_d = nullptr;
+  double* fc[3] = {nullptr, nullptr, nullptr};  // flux corrections (fine/coarse faces)
   size_t stage_lds[3] = {0, 0, 0};
   int stage_grid[3] = {0, 0, 0};
   struct StagePlan {
@@ -207,7 +212,23 @@ int run_fill(spark_batch* b, double* pool, bool full, int depth) {
     for (int v : need) vl.v[vl.n++] = v;
     for (int s = 0; s < b->p.nscalars; s++) vl.v[vl.n++] = spark::V_NPROP + s;
   }
-  HIPCHK(spark::plain_ug_fill(L, b->ug[0], b->ug[1], b->ug[2], pool, vl, depth));
+  if (b->amr) HIPCHK(spark::amr_fill(L, b->amr_g, pool, vl, depth));
+  else HIPCHK(spark::plain_ug_fill(L, b->ug[0], b->ug[1], b->ug[2], pool, vl, depth));
+  return SPARK_OK;
+}
+
+// the fine/coarse flux-register exchange and hy_rk_correctFluxes on every
+// block (Spark/Hydro.F90-mc:205-236), dt from the device
+int run_amr_correct(spark_batch* b) {
+  PlainLaunch L = make_launch(b, b->classes[0]);
+  L.blist = nullptr;
+  L.nblk = b->nblk;
+  HIPCHK(spark::amr_flux_exchange(L, b->amr_g, b->fb, b->fc));
+  for (auto& c : b->classes) {
+    PlainLaunch Lc = make_launch(b, c);
+    for (int d = 0; d < 3; d++) Lc.fb[d] = b->fc[d];
+    HIPCHK(spark::plain_correct(Lc, b->U));
+  }
   return SPARK_OK;
 }
 
@@ -358,6 +379,10 @@ void free_batch(spark_batch* b) {
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->nbr_d) (void)hipFree(b->nbr_d);
+  if (b->amr_map_d) (void)hipFree(b->amr_map_d);
+  if (b->amr_org_d) (void)hipFree(b->amr_org_d);
+  for (int d = 0; d < 3; d++)
+    if (b->fc[d]) (void)hipFree(b->fc[d]);
   for (auto& pl : b->plan) {
     if (pl.items_d) (void)hipFree(pl.items_d);
     if (pl.qoff_d) (void)hipFree(pl.qoff_d);
@@ -735,6 +760,7 @@ int spark_batch_create(const spark_params* p, int nblocks, const spark_block_des
   for (int q = 0; q < 3; q++) ALLOC(b->fl[q], sizeof(double) * b->fpool);
   if (p->flux_correct)
     for (int q = 0; q < 3; q++) ALLOC(b->fb[q], sizeof(double) * b->bpool);
+    for (int q = 0; q < 3; q++) ALLOC(b->fc[q], sizeof(double) * b->bpool);
   ALLOC(b->dtp, sizeof(double));
   ALLOC(b->loc, sizeof(long long) * 2);
   b->hist_cap = 4096;
@@ -895,7 +921,7 @@ int spark_set_uniform_grid(spark_batch* b, int nbx, int nby, int nbz) {
 
 int spark_fill_guardcells(spark_batch* b) {
   if (!b) return fail(SPARK_ERR_ARG, "spark_fill_guardcells: null batch");
-  if (!b->has_ug) return fail(SPARK_ERR_STATE, "spark_fill_guardcells: no uniform grid declared");
+  if (!b->has_ug && !b->amr) return fail(SPARK_ERR_STATE, "spark_fill_guardcells: no block grid declared");
   return run_fill(b, b->U, true, b->p.nguard);
 }
 
@@ -927,8 +953,10 @@ int spark_hydro_step(spark_batch* b, double dt, double dtOld) {
     return fail(SPARK_ERR_STATE, "refill guard mode needs spark_set_uniform_grid");
   HIPCHK(spark::plain_set_scalar(g_stream, b->dtp, dt));
   int rc;
-  if (b->has_ug && (rc = run_fill(b, b->U, false, stage1_reach(b)))) return rc;
-  return run_full_step(b);
+  if ((b->has_ug || b->amr) && (rc = run_fill(b, b->U, false, stage1_reach(b)))) return rc;
+  if ((rc = run_full_step(b))) return rc;
+  if (b->amr && b->p.flux_correct) return run_amr_correct(b);
+  return SPARK_OK;
 }
 
 int spark_step_begin(spark_batch* b) {
@@ -955,8 +983,10 @@ int spark_step_end(spark_batch* b) {
 static int one_step(spark_batch* b) {
   int rc = run_dt(b, 0);
   if (rc) return rc;
-  if (b->has_ug && (rc = run_fill(b, b->U, false, stage1_reach(b)))) return rc;
-  return run_full_step(b);
+  if ((b->has_ug || b->amr) && (rc = run_fill(b, b->U, false, stage1_reach(b)))) return rc;
+  if ((rc = run_full_step(b))) return rc;
+  if (b->amr && b->p.flux_correct) return run_amr_correct(b);
+  return SPARK_OK;
 }
 
 
@@ -1084,7 +1114,7 @@ int spark_set_flux_corrections(spark_batch* b, const double* const* fbx, const d
       const double* in = ins[dir][i];
       for (int t = 0; t < tot; t++)
         for (int n = 0; n < nf; n++) tmp[(size_t)n * tot + t] = in[(size_t)t * nf + n];
-      HIPCHK(hipMemcpy(b->fb[dir] + b->meta_h[i].bbase, tmp.data(), sizeof(double) * nf * tot, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(b->fc[dir] + b->meta_h[i].bbase, tmp.data(), sizeof(double) * nf * tot, hipMemcpyHostToDevice));
     }
   }
   return SPARK_OK;
@@ -1096,8 +1126,97 @@ int spark_flux_correct(spark_batch* b, double dt) {
   HIPCHK(spark::plain_set_scalar(g_stream, b->dtp, dt));
   for (auto& c : b->classes) {
     PlainLaunch L = make_launch(b, c);
+    for (int d = 0; d < 3; d++) L.fb[d] = b->fc[d];
     HIPCHK(spark::plain_correct(L, b->U));
   }
+  return SPARK_OK;
+}
+
+int spark_flux_exchange(spark_batch* b) {
+  if (!b) return fail(SPARK_ERR_ARG, "spark_flux_exchange: null batch");
+  if (!b->p.flux_correct) return fail(SPARK_ERR_STATE, "spark_flux_exchange needs flux_correct = 1");
+  if (!b->amr) return fail(SPARK_ERR_STATE, "spark_flux_exchange: no block hierarchy declared");
+  PlainLaunch L = make_launch(b, b->classes[0]);
+  L.blist = nullptr;
+  L.nblk = b->nblk;
+  HIPCHK(spark::amr_flux_exchange(L, b->amr_g, b->fb, b->fc));
+  return SPARK_OK;
+}
+
+int spark_set_amr_grid(spark_batch* b, int nbx0, int nby0, int nbz0, const int* origins) {
+  if (!b || !origins) return fail(SPARK_ERR_ARG, "spark_set_amr_grid: null argument");
+  if (b->p.ndim != 3) return fail(SPARK_ERR_ARG, "spark_set_amr_grid: 3D only");
+  if (b->p.guard_mode != SPARK_GUARD_TELESCOPE)
+    return fail(SPARK_ERR_ARG, "spark_set_amr_grid: the block hierarchy is filled once per step (telescope mode)");
+  if (nbx0 < 1 || nby0 < 1 || nbz0 < 1) return fail(SPARK_ERR_ARG, "spark_set_amr_grid: bad root grid");
+  if (b->classes.size() != 1) return fail(SPARK_ERR_ARG, "spark_set_amr_grid: blocks of one shape only");
+  const int n = b->meta_h[0].nx;
+  if (b->meta_h[0].ny != n || b->meta_h[0].nz != n || (n & 1))
+    return fail(SPARK_ERR_ARG, "spark_set_amr_grid: blocks must be cubic with an even cell count");
+  int lmax = 1;
+  for (int i = 0; i < b->nblk; i++) {
+    const int L = b->meta_h[i].level;
+    if (L < 1 || L > 12) return fail(SPARK_ERR_ARG, "block %d: level %d out of range", i, L);
+    lmax = std::max(lmax, L);
+  }
+  const int nb0[3] = {nbx0, nby0, nbz0};
+  int md[3];
+  for (int d = 0; d < 3; d++) md[d] = nb0[d] << (lmax - 1);
+  std::vector<int> map((size_t)md[0] * md[1] * md[2], -1);
+  for (int i = 0; i < b->nblk; i++) {
+    const int L = b->meta_h[i].level, sh = lmax - L;
+    int lo[3];
+    for (int d = 0; d < 3; d++) {
+      const int o = origins[3 * i + d];
+      if (o < 0 || o % n || o / n >= (nb0[d] << (L - 1)))
+        return fail(SPARK_ERR_ARG, "block %d: origin %d not a block position of level %d", i, o, L);
+      lo[d] = (o / n) << sh;
+    }
+    for (int z = lo[2]; z < lo[2] + (1 << sh); z++)
+      for (int y = lo[1]; y < lo[1] + (1 << sh); y++)
+        for (int x = lo[0]; x < lo[0] + (1 << sh); x++) {
+          int& slot = map[x + (size_t)md[0] * (y + (size_t)md[1] * z)];
+          if (slot >= 0) return fail(SPARK_ERR_ARG, "blocks %d and %d overlap", slot, i);
+          slot = i;
+        }
+  }
+  for (size_t t = 0; t < map.size(); t++)
+    if (map[t] < 0) return fail(SPARK_ERR_ARG, "the blocks do not cover the domain");
+  // every leaf under a block's guard region is within one level of it (the
+  // guard fill and the flux exchange handle coarser-by-one, same and
+  // finer-by-one sources only)
+  for (int i = 0; i < b->nblk; i++) {
+    const int L = b->meta_h[i].level, sh = lmax - L, g = b->p.nguard;
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; d++) {
+      const int cmax = (nb0[d] * n << (L - 1)) - 1;
+      const int c0 = std::max(0, origins[3 * i + d] - g), c1 = std::min(cmax, origins[3 * i + d] + n - 1 + g);
+      lo[d] = (c0 << sh) / n;
+      hi[d] = std::min(((((c1 + 1) << sh) - 1) / n) + 1, md[d]);
+    }
+    for (int z = lo[2]; z < hi[2]; z++)
+      for (int y = lo[1]; y < hi[1]; y++)
+        for (int x = lo[0]; x < hi[0]; x++) {
+          const int L2 = b->meta_h[map[x + (size_t)md[0] * (y + (size_t)md[1] * z)]].level;
+          if (L2 - L > 1 || L - L2 > 1)
+            return fail(SPARK_ERR_ARG, "block %d (level %d): a leaf of level %d lies under its guard cells", i, L, L2);
+        }
+  }
+  if (!b->amr_map_d) HIPCHK(hipMalloc((void**)&b->amr_map_d, sizeof(int) * map.size()));
+  else {
+    (void)hipFree(b->amr_map_d);
+    HIPCHK(hipMalloc((void**)&b->amr_map_d, sizeof(int) * map.size()));
+  }
+  if (!b->amr_org_d) HIPCHK(hipMalloc((void**)&b->amr_org_d, sizeof(int) * 3 * b->nblk));
+  HIPCHK(hipMemcpy(b->amr_map_d, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->amr_org_d, origins, sizeof(int) * 3 * b->nblk, hipMemcpyHostToDevice));
+  b->amr_g.lmax = lmax;
+  b->amr_g.n = n;
+  for (int d = 0; d < 3; d++) { b->amr_g.nb0[d] = nb0[d]; b->amr_g.md[d] = md[d]; }
+  b->amr_g.map = b->amr_map_d;
+  b->amr_g.org = b->amr_org_d;
+  b->amr = true;
+  drop_graph(b);
   return SPARK_OK;
 }
 

@@ -74,6 +74,18 @@ struct VarList {
 };
 hipError_t plain_ug_fill(const PlainLaunch& L, int nbx, int nby, int nbz, double* U, const VarList& vars, int depth);
 hipError_t plain_correct(const PlainLaunch& L, double* U);
+
+// Block hierarchy (leaves of n^3 cells over a level-1 grid of nb0 blocks)
+struct AmrGrid {
+  int lmax;       // finest level (1-based)
+  int n;          // interior cells per side of every block
+  int nb0[3];     // level-1 blocks per dimension
+  int md[3];      // leaf map extent: nb0 << (lmax - 1)
+  const int* map; // finest-level block slot -> leaf block
+  const int* org; // 3 per block: first interior cell, global index at its level
+};
+hipError_t amr_fill(const PlainLaunch& L, const AmrGrid& G, double* U, const VarList& vars, int depth);
+hipError_t amr_flux_exchange(const PlainLaunch& L, const AmrGrid& G, double* const fb[3], double* const fc[3]);
 hipError_t plain_aos_to_soa(hipStream_t st, const BlockMeta* meta, int nblk, int maxcells, const double* aos,
                             const int64_t* aos_off, int nunk, const int* var_index, int nvar, double* U);
 hipError_t plain_soa_to_aos(hipStream_t st, const DevParams& P, const BlockMeta* meta, int nblk, int maxcells,

@@ -53,7 +53,7 @@ module spark_hip
             spark_hydro_step, spark_step_begin, spark_stage, spark_step_end, spark_advance, &
             spark_get_flux_registers, spark_set_flux_corrections, spark_flux_correct, &
             spark_set_option, spark_batched_tiles, spark_synchronize, spark_get_counters, &
-            spark_last_error, spark_profile, spark_profile_read
+            spark_last_error, spark_profile, spark_profile_read, spark_set_amr_grid, spark_flux_exchange
   public :: spark_check, spark_set_abort_hook, spark_error_message
 
   interface
@@ -103,6 +103,19 @@ module spark_hip
        type(c_ptr), value :: batch
        integer(c_int), value :: nbx, nby, nbz
      end function spark_set_uniform_grid
+
+     !! origins(3*nblocks): first interior cell of each block, global index at its level
+     integer(c_int) function spark_set_amr_grid(batch, nbx0, nby0, nbz0, origins) bind(C, name="spark_set_amr_grid")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: batch
+       integer(c_int), value :: nbx0, nby0, nbz0
+       integer(c_int), intent(in) :: origins(*)
+     end function spark_set_amr_grid
+
+     integer(c_int) function spark_flux_exchange(batch) bind(C, name="spark_flux_exchange")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: batch
+     end function spark_flux_exchange
 
      integer(c_int) function spark_fill_guardcells(batch) bind(C, name="spark_fill_guardcells")
        import :: c_int, c_ptr

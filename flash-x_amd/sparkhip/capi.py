@@ -93,6 +93,8 @@ def lib():
             "spark_get_flux_registers": [vp, dpp, dpp, dpp],
             "spark_set_flux_corrections": [vp, dpp, dpp, dpp],
             "spark_flux_correct": [vp, ctypes.c_double],
+            "spark_flux_exchange": [vp],
+            "spark_set_amr_grid": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
             "spark_synchronize": [vp],
             "spark_get_counters": [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.c_int],
             "spark_last_error": [ctypes.c_char_p, ctypes.c_int],
@@ -117,7 +119,7 @@ def exported_symbols():
         "spark_hydro_step", "spark_step_begin", "spark_stage", "spark_step_end", "spark_advance",
         "spark_get_flux_registers", "spark_set_flux_corrections", "spark_flux_correct", "spark_synchronize",
         "spark_get_counters", "spark_last_error", "spark_profile", "spark_profile_read", "spark_set_option",
-        "spark_batched_tiles",
+        "spark_batched_tiles", "spark_set_amr_grid", "spark_flux_exchange",
     ]
 
 
@@ -277,6 +279,16 @@ class Batch:
 
     def flux_correct(self, dt):
         check(lib().spark_flux_correct(self.h, float(dt)), "spark_flux_correct")
+
+    def set_amr_grid(self, nb0, origins):
+        """nb0: level-1 blocks per dimension; origins: (nblocks, 3) first
+        interior cell of each block in its level's global index space."""
+        o = np.ascontiguousarray(np.asarray(origins, dtype=np.int32).reshape(-1))
+        check(lib().spark_set_amr_grid(self.h, int(nb0[0]), int(nb0[1]), int(nb0[2]),
+                                       o.ctypes.data_as(ctypes.POINTER(ctypes.c_int))), "spark_set_amr_grid")
+
+    def flux_exchange(self):
+        check(lib().spark_flux_exchange(self.h), "spark_flux_exchange")
 
 
 def gpu_available() -> bool:

@@ -215,3 +215,113 @@ def batched(nblocks=4096, nguard=10, gamma=1.4, seed_geom=20261202, seed_field=2
         arrays.append(U)
     return Problem("batched", cfg, blocks, arrays, info=dict(seeds=(seed_geom, seed_field), nshock=nshock,
                                                              size_counts=np.bincount(size_idx, minlength=len(sizes))))
+
+
+def amr_leaves(nb0, n, lmax, refine, nguard):
+    """Leaves of a block hierarchy: level-1 grid nb0 (blocks per dimension),
+    blocks of n^3 cells, block (level L, index b) refined while L < lmax and
+    refine(L, lo, hi) is true for its physical box (dx at level 1 is
+    1 / (nb0[0] * n)).  Coarse leaves that lie within a finer leaf's guard
+    reach (nguard cells) by more than one level are refined further, so
+    every leaf under a block's guard cells is within one level of it.
+    Returns a sorted list of (level, (bx, by, bz))."""
+    dx0 = 1.0 / (nb0[0] * n)
+
+    def box(L, b):
+        w = n * dx0 / 2 ** (L - 1)
+        return [b[d] * w for d in range(3)], [(b[d] + 1) * w for d in range(3)]
+
+    leaves = set()
+    todo = [(1, (i, j, k)) for k in range(nb0[2]) for j in range(nb0[1]) for i in range(nb0[0])]
+    while todo:
+        L, b = todo.pop()
+        lo, hi = box(L, b)
+        if L < lmax and refine(L, lo, hi):
+            todo += [(L + 1, (2 * b[0] + i, 2 * b[1] + j, 2 * b[2] + k)) for k in (0, 1) for j in (0, 1) for i in (0, 1)]
+        else:
+            leaves.add((L, b))
+    # balance: no leaf under a block's guard reach more than one level coarser
+    while True:
+        md = [nb0[d] << (lmax - 1) for d in range(3)]
+        slot = np.full(md[::-1], -1, dtype=np.int64)
+        lv = {}
+        for idx, (L, b) in enumerate(sorted(leaves)):
+            sh = lmax - L
+            slot[b[2] << sh:(b[2] + 1) << sh, b[1] << sh:(b[1] + 1) << sh, b[0] << sh:(b[0] + 1) << sh] = L
+        split = set()
+        for (L, b) in leaves:
+            sh = lmax - L
+            lo = [max(0, ((b[d] * n - nguard) << sh) // n) for d in range(3)]
+            hi = [min(md[d], ((((b[d] + 1) * n + nguard) << sh) - 1) // n + 1) for d in range(3)]
+            region = slot[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]]
+            if region.min() < L - 1:
+                for (L2, b2) in leaves:
+                    if L2 < L - 1:
+                        sh2 = lmax - L2
+                        if all(lo[d] < (b2[d] + 1) << sh2 and (b2[d] << sh2) < hi[d] for d in range(3)):
+                            split.add((L2, b2))
+        if not split:
+            break
+        for (L, b) in split:
+            leaves.discard((L, b))
+            leaves.update((L + 1, (2 * b[0] + i, 2 * b[1] + j, 2 * b[2] + k))
+                          for k in (0, 1) for j in (0, 1) for i in (0, 1))
+    return sorted(leaves)
+
+
+def rt_amr(nb0=(4, 4, 8), n=16, lmax=3, slabs=((0.25, 1.75), (0.5, 1.5)), nguard=10, nscalars=8, gamma=1.4,
+           seed=20261204, **kw) -> Problem:
+    """3D Rayleigh-Taylor on a block hierarchy: heavy fluid (rho = 2) over
+    light (rho = 1) across z0 = the middle of the domain, constant gravity
+    g = -1 along z as a source term, hydrostatic pressure with p = 2.5 at the
+    interface, interface z0 + 0.01 * (cos 2pi x cos 2pi y + seeded random
+    modes, normalised), nscalars mass scalars X_k = (1 + tanh((z - zi) /
+    w_k)) / 2 with w_k = 0.01 (k + 1).  Level-L blocks are refined where
+    their z range meets slabs[L-1].  x, y span [0, 1]; dx = 1 / (nb0[0] n)
+    at level 1.  Returns a Problem with info nb0 / origins for
+    spark_set_amr_grid."""
+    cfg = base_cfg(**{**dict(ndim=3, nguard=nguard, gamma=gamma, nscalars=nscalars, use_gravity=1,
+                             grav=(0.0, 0.0, -1.0), flux_correct=1), **kw})
+    dx0 = 1.0 / (nb0[0] * n)
+    zlen = nb0[2] * n * dx0
+    z0 = 0.5 * zlen
+
+    def refine(L, lo, hi):
+        a, b = slabs[L - 1]
+        return lo[2] < b and hi[2] > a
+
+    leaves = amr_leaves(nb0, n, lmax, refine, nguard)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nm = 4
+    kx = rng.integers(1, 4, size=nm)
+    ky = rng.integers(1, 4, size=nm)
+    amp = rng.normal(size=nm)
+    ph = rng.uniform(0, 2 * np.pi, size=nm)
+
+    def zi(X, Y):
+        r = sum(a * np.cos(2 * np.pi * (i * X + j * Y) + p) for i, j, a, p in zip(kx, ky, amp, ph))
+        pert = np.cos(2 * np.pi * X) * np.cos(2 * np.pi * Y) + 0.5 * r / np.sum(np.abs(amp))
+        return z0 + 0.01 * pert / 1.5
+
+    blocks, arrays, origins = [], [], []
+    nunk = SPARK_NPROP + nscalars
+    for (L, b) in leaves:
+        dx = dx0 / 2 ** (L - 1)
+        blk = dict(nx=n, ny=n, nz=n, level=L, dx=dx, dy=dx, dz=dx)
+        NZ, NY, NX = gc_shape(cfg, blk)
+        Z, Y, X = _centers(cfg, blk, tuple(b[d] * n * dx for d in range(3)), NZ, NY, NX)
+        zint = zi(X, Y)
+        heavy = Z > zint
+        dens = np.where(heavy, 2.0, 1.0)
+        pres = np.where(heavy, 2.5 - 2.0 * (Z - zint), 2.5 + 1.0 * (zint - Z))
+        zero = np.zeros_like(dens)
+        scal = [0.5 * (1.0 + np.tanh((Z - zint) / (0.01 * (q + 1)))) for q in range(nscalars)] if nscalars else None
+        U = np.empty((NZ, NY, NX, nunk))
+        fill_thermo(U, dens, pres, zero, zero, zero, gamma, scalars=scal)
+        blocks.append(blk)
+        arrays.append(U)
+        origins.append([b[d] * n for d in range(3)])
+    counts = np.bincount([L for L, _ in leaves], minlength=lmax + 1)[1:]
+    return Problem("rt_amr", cfg, blocks, arrays,
+                   info=dict(nb0=tuple(nb0), origins=np.asarray(origins, dtype=np.int32), leaves_per_level=counts,
+                             seed=seed))

@@ -118,6 +118,21 @@ int spark_unpack(spark_batch* b, double* const* unk_blocks, int interior_only, i
  * Grid/GridMain/AMR/Paramesh4/Grid_fillGuardCells.F90:436-546 and
  * Grid/GridBoundaryConditions/Grid_bcApplyToRegion.F90:526-532. */
 int spark_set_uniform_grid(spark_batch* b, int nbx, int nby, int nbz);
+
+/* Declare the batch a Paramesh-like hierarchy of leaf blocks (3D,
+ * telescope guard mode, all blocks n^3 with n even): level-1 grid of
+ * nbx0 x nby0 x nbz0 blocks; block i (its spark_block_desc level is 1-based)
+ * has its first interior cell at origins[3*i .. 3*i+2], the global cell
+ * index at its own level.  The leaves must tile the domain, and every leaf
+ * under a block's guard cells must be within one level of it.  Enables the
+ * device guard fill over levels (same-level copy, injection from a coarser
+ * leaf, the mean of 8 cells of a finer leaf, OUTFLOW at the domain faces)
+ * and, with flux_correct, the device flux-register exchange after every
+ * step.
+ * Replaces: Grid_fillGuardCells over PARAMESH leaves,
+ * Grid/GridMain/AMR/Paramesh4/Grid_fillGuardCells.F90:436-546, and the
+ * flux hand-off of Spark/Hydro.F90-mc:197-236. */
+int spark_set_amr_grid(spark_batch* b, int nbx0, int nby0, int nbz0, const int* origins);
 int spark_fill_guardcells(spark_batch* b);
 
 /* CFL time step over all blocks.  minloc = (i, j, k, level, block): the
@@ -163,6 +178,16 @@ int spark_get_flux_registers(spark_batch* b, double* const* fbx, double* const* 
 int spark_set_flux_corrections(spark_batch* b, const double* const* fbx, const double* const* fby,
                                const double* const* fbz);
 int spark_flux_correct(spark_batch* b, double dt);
+
+/* The flux-register exchange on the device over the declared hierarchy:
+ * corrections = mean of the 4 fine faces (red_f = 0.25, flux densities)
+ * minus the coarse block's own face flux on fine/coarse faces, 0 on every
+ * other boundary face; they replace what spark_set_flux_corrections set.
+ * Replaces: Grid_putFluxData_block + Grid_communicateFluxes +
+ * Grid_getFluxCorrData_block (Paramesh4/Grid_putFluxData_block.F90:83-392,
+ * Grid_communicateFluxes.F90:81-106, Grid_getFluxCorrData_block.F90:88-360,
+ * PM4_package/source/amr_restrict_red.F90). */
+int spark_flux_exchange(spark_batch* b);
 
 /* Per-kernel timing of the library's own launches: HIP events recorded on
  * the library stream around every launch while enabled (enabling clears

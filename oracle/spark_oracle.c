@@ -975,3 +975,192 @@ int orc_batch_advance(const orc_params* P, const orc_block* bl, int nblocks, dou
   }
   return nerr;
 }
+
+/* ------------------------------------------------------------------ */
+/* Block hierarchy: guard fill, flux-register exchange, advance.        */
+
+typedef struct amr_map {
+  int lmax, md[3];
+  int* leaf; /* md[0]*md[1]*md[2]: block covering each finest-level block slot */
+} amr_map;
+
+static int amr_build_map(const orc_amr* A, int nblocks, amr_map* M) {
+  M->lmax = 1;
+  for (int i = 0; i < nblocks; i++)
+    if (A->level[i] > M->lmax) M->lmax = A->level[i];
+  for (int d = 0; d < 3; d++) M->md[d] = A->nb0[d] << (M->lmax - 1);
+  size_t tot = (size_t)M->md[0] * M->md[1] * M->md[2];
+  M->leaf = (int*)malloc(sizeof(int) * tot);
+  for (size_t t = 0; t < tot; t++) M->leaf[t] = -1;
+  for (int i = 0; i < nblocks; i++) {
+    const int sh = M->lmax - A->level[i];
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; d++) {
+      lo[d] = (A->origin[3 * i + d] / A->n) << sh;
+      hi[d] = lo[d] + (1 << sh);
+    }
+    for (int z = lo[2]; z < hi[2]; z++)
+      for (int y = lo[1]; y < hi[1]; y++)
+        for (int x = lo[0]; x < hi[0]; x++) M->leaf[x + (size_t)M->md[0] * (y + (size_t)M->md[1] * z)] = i;
+  }
+  return 0;
+}
+
+/* leaf covering global cell gi (level L indices) */
+static int amr_leaf_at(const orc_amr* A, const amr_map* M, int L, const int gi[3]) {
+  int s[3];
+  for (int d = 0; d < 3; d++) s[d] = (gi[d] << (M->lmax - L)) / A->n;
+  return M->leaf[s[0] + (size_t)M->md[0] * (s[1] + (size_t)M->md[1] * s[2])];
+}
+
+void orc_amr_fill_guards(const orc_params* P, const orc_amr* A, int nblocks, double* const* blocks, int nvar,
+                         int depth) {
+  amr_map M;
+  amr_build_map(A, nblocks, &M);
+  const int n = A->n, g = P->nguard, NX = n + 2 * g;
+#pragma omp parallel for schedule(dynamic)
+  for (int b = 0; b < nblocks; b++) {
+    const int L = A->level[b];
+    const int* o = A->origin + 3 * b;
+    double* dst = blocks[b];
+    for (int k = g - depth; k < g + n + depth; k++)
+      for (int j = g - depth; j < g + n + depth; j++)
+        for (int i = g - depth; i < g + n + depth; i++) {
+          const int loc[3] = {i, j, k};
+          int inside = 1, gi[3];
+          for (int d = 0; d < 3; d++) {
+            if (loc[d] < g || loc[d] >= g + n) inside = 0;
+            int x = o[d] + loc[d] - g;
+            const int xmax = (A->nb0[d] * n << (L - 1)) - 1;
+            gi[d] = x < 0 ? 0 : (x > xmax ? xmax : x);
+          }
+          if (inside) continue;
+          const int s = amr_leaf_at(A, &M, L, gi);
+          const int Ls = A->level[s];
+          const int* os = A->origin + 3 * s;
+          double* out = dst + (size_t)nvar * cidx(NX, NX, i, j, k);
+          const double* src = blocks[s];
+          if (Ls <= L) {
+            int sl[3];
+            for (int d = 0; d < 3; d++) sl[d] = (gi[d] >> (L - Ls)) - os[d] + g;
+            memcpy(out, src + (size_t)nvar * cidx(NX, NX, sl[0], sl[1], sl[2]), sizeof(double) * nvar);
+          } else {
+            int f0[3];
+            for (int d = 0; d < 3; d++) f0[d] = 2 * gi[d] - os[d] + g;
+            for (int v = 0; v < nvar; v++) {
+              double sum = 0.0;
+              for (int c = 0; c < 8; c++)
+                sum = sum + src[(size_t)nvar * cidx(NX, NX, f0[0] + (c & 1), f0[1] + ((c >> 1) & 1),
+                                                    f0[2] + ((c >> 2) & 1)) + v];
+              out[v] = 0.125 * sum;
+            }
+          }
+        }
+  }
+  free(M.leaf);
+}
+
+void orc_flux_exchange(const orc_params* P, const orc_amr* A, int nblocks, double* const* fb, double* const* fc) {
+  amr_map M;
+  amr_build_map(A, nblocks, &M);
+  const int n = A->n, nf = orc_nflux(P);
+#pragma omp parallel for schedule(dynamic)
+  for (int b = 0; b < nblocks; b++) {
+    const int L = A->level[b];
+    const int* o = A->origin + 3 * b;
+    for (int dir = 0; dir < 3; dir++) {
+      int e[3] = {n, n, n};
+      e[dir] += 1;
+      const int d2 = (dir == 0) ? 1 : 0, d3 = (dir == 2) ? 1 : 2;
+      double* out = fc[3 * b + dir];
+      memset(out, 0, sizeof(double) * (size_t)nf * e[0] * e[1] * e[2]);
+      const double* own = fb[3 * b + dir];
+      for (int side = 0; side < 2; side++) {
+        const int gmax = (A->nb0[dir] * n << (L - 1)) - 1;
+        const int across = side ? o[dir] + n : o[dir] - 1;
+        if (across < 0 || across > gmax) continue;
+        for (int a3 = 0; a3 < n; a3++)
+          for (int a2 = 0; a2 < n; a2++) {
+            int gi[3];
+            gi[dir] = across;
+            gi[d2] = o[d2] + a2;
+            gi[d3] = o[d3] + a3;
+            const int s = amr_leaf_at(A, &M, L, gi);
+            if (A->level[s] != L + 1) continue;
+            const int* os = A->origin + 3 * s;
+            int ff[3], f[3];
+            ff[dir] = side ? 0 : n;
+            ff[d2] = 2 * gi[d2] - os[d2];
+            ff[d3] = 2 * gi[d3] - os[d3];
+            f[dir] = side ? n : 0;
+            f[d2] = a2;
+            f[d3] = a3;
+            const double* fine = fb[3 * s + dir];
+            const size_t co = (size_t)nf * ((size_t)f[0] + (size_t)e[0] * ((size_t)f[1] + (size_t)e[1] * f[2]));
+            size_t fo[4];
+            for (int q = 0; q < 4; q++) {
+              int t[3] = {ff[0], ff[1], ff[2]};
+              t[d2] += q & 1;
+              t[d3] += q >> 1;
+              fo[q] = (size_t)nf * ((size_t)t[0] + (size_t)e[0] * ((size_t)t[1] + (size_t)e[1] * t[2]));
+            }
+            for (int v = 0; v < nf; v++) {
+              const double rest = (fine[fo[0] + v] + fine[fo[1] + v] + fine[fo[2] + v] + fine[fo[3] + v]) * 0.25;
+              out[co + v] = rest - own[co + v];
+            }
+          }
+      }
+    }
+  }
+  free(M.leaf);
+}
+
+int orc_amr_advance(const orc_params* P, const orc_amr* A, const orc_block* bl, int nblocks, double* const* blocks,
+                    int nsteps, int flux_correct, int nthreads, double* dts_out) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  const int nunk = orc_nunk(P), nf = orc_nflux(P), n = A->n;
+  int last, la[3];
+  double coeff[3][3], weights[3];
+  orc_rk_tables(P, &last, coeff, la, weights);
+  const int depth = la[0] + P->nstencil;
+  double** fb = NULL;
+  double** fc = NULL;
+  if (flux_correct) {
+    fb = (double**)malloc(sizeof(double*) * 3 * nblocks);
+    fc = (double**)malloc(sizeof(double*) * 3 * nblocks);
+    for (int i = 0; i < 3 * nblocks; i++) {
+      size_t sz = (size_t)nf * (n + 1) * n * n;
+      fb[i] = (double*)calloc(sz, sizeof(double));
+      fc[i] = (double*)calloc(sz, sizeof(double));
+    }
+  }
+  int nerr = 0;
+  for (int step = 0; step < nsteps; step++) {
+    orc_amr_fill_guards(P, A, nblocks, blocks, nunk, depth);
+    double dt = DBL_MAX;
+    for (int i = 0; i < nblocks; i++) {
+      double d = orc_compute_dt(P, &bl[i], blocks[i], NULL);
+      if (d < dt) dt = d;
+    }
+    if (dts_out) dts_out[step] = dt;
+#pragma omp parallel for schedule(dynamic) reduction(+ : nerr)
+    for (int i = 0; i < nblocks; i++)
+      nerr += orc_hydro_block(P, &bl[i], blocks[i], dt, fb ? fb[3 * i] : NULL, fb ? fb[3 * i + 1] : NULL,
+                              fb ? fb[3 * i + 2] : NULL);
+    if (flux_correct) {
+      orc_flux_exchange(P, A, nblocks, fb, fc);
+#pragma omp parallel for schedule(dynamic)
+      for (int i = 0; i < nblocks; i++) orc_correct_fluxes(P, &bl[i], blocks[i], dt, fc[3 * i], fc[3 * i + 1], fc[3 * i + 2]);
+    }
+  }
+  if (flux_correct) {
+    for (int i = 0; i < 3 * nblocks; i++) { free(fb[i]); free(fc[i]); }
+    free(fb);
+    free(fc);
+  }
+  return nerr;
+}

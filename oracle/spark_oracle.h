@@ -115,6 +115,39 @@ double orc_ug_compute_dt(const orc_params* P, const orc_block* b, int nblocks,
 int orc_batch_advance(const orc_params* P, const orc_block* bl, int nblocks,
                       double* const* blocks, int nsteps, int nthreads, double* dts_out);
 
+/* Block hierarchy of leaves (Paramesh-like): every block has n^3 interior
+ * cells; level[i] is 1-based; origin[3*i..] is the block's first interior
+ * cell in the global index space of its level; the level-1 grid has
+ * nb0[0] x nb0[1] x nb0[2] blocks; neighbouring leaves differ by at most
+ * one level. */
+typedef struct orc_amr {
+  int n;
+  int nb0[3];
+  const int* level;
+  const int* origin;
+} orc_amr;
+
+/* Guard fill over the hierarchy: a guard cell takes the value of the leaf
+ * covering its position -- a copy from a same-level leaf, the coarse cell
+ * containing it from a coarser leaf (injection), the mean of the 8 fine
+ * cells from a finer leaf -- with the global index clamped into the domain
+ * first (OUTFLOW).  depth guard layers of all nvar variables. */
+void orc_amr_fill_guards(const orc_params* P, const orc_amr* A, int nblocks, double* const* blocks, int nvar,
+                         int depth);
+
+/* Flux-register exchange (Grid_putFluxData_block, Grid_communicateFluxes,
+ * Grid_getFluxCorrData_block with consv_flux_densities, red_f = 0.25,
+ * PM4_package/source/amr_restrict_red.F90, Grid_getFluxCorrData_block.F90:182-216):
+ * fc = (restricted fine - own) on coarse-side faces of fine/coarse
+ * boundaries, 0 on every other boundary face.  fb / fc hold 3 arrays per
+ * block (x, y, z interior faces, flux index fastest). */
+void orc_flux_exchange(const orc_params* P, const orc_amr* A, int nblocks, double* const* fb, double* const* fc);
+
+/* nsteps of the hierarchy: guard fill, shared CFL dt, Hydro() on every
+ * leaf, then (flux_correct) the exchange and hy_rk_correctFluxes. */
+int orc_amr_advance(const orc_params* P, const orc_amr* A, const orc_block* bl, int nblocks, double* const* blocks,
+                    int nsteps, int flux_correct, int nthreads, double* dts_out);
+
 #ifdef __cplusplus
 }
 #endif
