@@ -218,11 +218,53 @@ __device__ __forceinline__ void reg_add(double* fb, int64_t o, int64_t tot, int 
   fb[o + n * tot] = add ? fb[o + n * tot] + w : w;
 }
 
+// x (DIR 0) or y (DIR 1) face w of the tile at level k: states from the
+// LDS reconstruction buffers, Riemann solve + viscosity, flux rows and the
+// mass flux to LDS, flux registers for interior faces the item owns.
+template <int DIR, int RIEMANN>
+__device__ __forceinline__ void xy_face(const FusedArgs& A, const BlockMeta& m, const FusedItem& I, const RParams& rp,
+                                        int g, int k, const Real* bp, const Real* bm, int nr, const Real* plane, int psz,
+                                        int PW, const FDiv& dv, Real* fo, Real* mo, int nf, int w) {
+  const int tx = I.tx, ty = I.ty;
+  Real Ls[R_NUM], Rs[R_NUM], cl[5], cr[5], F[F_NUM];
+  int fi;
+  const int fj = fquot(w, dv, fi);  // the face is the low face of cell (fi, fj) of the tile
+  const int il = DIR == 0 ? fj * (tx + 2) + fi : fj * tx + fi;
+  face_states(bp, bm, nr, il, il + (DIR == 0 ? 1 : tx), Ls, Rs);
+  const int pc = (fj + 3) * PW + (fi + 3);
+  centred(plane, psz, pc - (DIR == 0 ? 1 : PW), cl);
+  centred(plane, psz, pc, cr);
+  hydro_face<DIR, RIEMANN>(rp, Ls, Rs, cl, cr, F, A.err);
+#pragma unroll
+  for (int n = 0; n < F_NUM; n++) fo[n * nf + w] = F[n];
+  mo[w] = F[F_MASS];
+  if (A.fb[0] != nullptr) {
+    const int i = I.x0 + fi, j = I.y0 + fj;
+    const bool own = DIR == 0 ? (fi < tx || i == I.xhi + 1) : (fj < ty || j == I.yhi + 1);
+    const bool in = DIR == 0 ? (i >= g && i <= g + m.nx && j >= g && j < g + m.ny)
+                             : (i >= g && i < g + m.nx && j >= g && j <= g + m.ny);
+    if (own && in && k >= g && k < g + m.nz) {
+      const int ex = m.nx + (DIR == 0), ey = m.ny + (DIR == 1);
+      const int64_t tot = (int64_t)ex * ey * m.nz;
+      const int64_t o = m.bbase + (i - g) + (int64_t)ex * ((j - g) + (int64_t)ey * (k - g));
+#pragma unroll
+      for (int n = 0; n < F_NUM; n++) reg_add(A.fb[DIR], o, tot, n, A.weight, A.add_flux, F[n]);
+    }
+  }
+}
+
 template <int NSM, int RECON, int RIEMANN>
-#ifndef FUSED_WAVES
-#define FUSED_WAVES 2
+// waves per SIMD the register allocation targets: the FP32 form fits 2
+// (<= 256 VGPRs, no spills, with or without scalars); the FP64 form keeps
+// all its registers (accumulation registers included) at 1 rather than
+// spilling
+#ifndef FUSED_WAVES_OF
+#define FUSED_WAVES_OF(nsm) (kF64 ? 1 : 2)
 #endif
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_WAVES))) FUSED_NAME(k_fused, FUSED_SFX)(FusedArgs A) {
+#ifndef FUSED_XY_UNROLL
+#define FUSED_XY_UNROLL 6
+#endif
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_WAVES_OF(NSM)))) FUSED_NAME(k_fused, FUSED_SFX)(FusedArgs A) {
   constexpr int NFM = F_NUM + NSM;
   constexpr int NSA = NSM > 0 ? NSM : 1;
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -296,7 +338,6 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
     const bool colin = ci >= g && ci < g + m.nx && cj >= g && cj < g + m.ny;
 
     Real win[NH][5];    // z window k-2 .. k+2 of the column, hydro slots
-    Real wsc[NSA][5];   // same for the scalars
     Real upz[NH], upzs[NSA];  // u+ of level k-1 (z faces)
     Real Hprev[NFM];          // z flux at the low face of level k-1
     double dUxy[NFM];         // -dt/dx dF - dt/dy dG of level k-1
@@ -307,11 +348,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
       for (int w = 0; w < 5; w++) win[v][w] = Real(0.);
     }
 #pragma unroll
-    for (int s = 0; s < NSA; s++) {
-      upzs[s] = Real(0.);
-#pragma unroll
-      for (int w = 0; w < 5; w++) wsc[s][w] = Real(0.);
-    }
+    for (int s = 0; s < NSA; s++) upzs[s] = Real(0.);
 #pragma unroll
     for (int n = 0; n < NFM; n++) { Hprev[n] = Real(0.); dUxy[n] = 0.; }
     if (col) {
@@ -322,9 +359,6 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
         const int64_t c = cell_at(m, nbt, g, ci, cj, I.z0 - 4 + w);
 #pragma unroll
         for (int v = 0; v < NH; v++) win[v][w] = load_slot(S, cs, c, v, P.gamma);
-#pragma unroll
-        for (int s = 0; s < NSM; s++)
-          if (s < ns) wsc[s][w] = (Real)S[(V_NPROP + s) * cs + c];
       }
     }
 
@@ -340,15 +374,10 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
           win[v][4] = load_slot(S, cs, c, v, P.gamma);
           plane[v * psz + pcc] = win[v][2];
         }
-#pragma unroll
-        for (int s = 0; s < NSM; s++) {
-          if (s < ns) {
-#pragma unroll
-            for (int w = 0; w < 4; w++) wsc[s][w] = wsc[s][w + 1];
-            wsc[s][4] = (Real)S[(V_NPROP + s) * cs + c];
-            plane[(NH + s) * psz + pcc] = wsc[s][2];
-          }
-        }
+        // scalars: the plane only (their z stencils are read again when the
+        // scalar chunks are reconstructed)
+        const int64_t ck = cell_at(m, nbt, g, ci, cj, k);
+        for (int s = 0; s < ns; s++) plane[(NH + s) * psz + pcc] = (Real)S[(V_NPROP + s) * cs + ck];
       }
       {
         // the 3-cell halo ring: 3 full rows below and above, 3 + 3 cells in
@@ -435,7 +464,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
           else { ry = fquot(w, d_tx, rx) - 1; st = PW; nr = nry; om = ym; op = yp; }
           const int pc = (ry + 3) * PW + (rx + 3);
           const Real fl = use_flat ? flatb[(ry + 1) * (tx + 2) + (rx + 1)] : Real(1.);
-#pragma unroll
+#pragma unroll FUSED_XY_UNROLL
           for (int v = 0; v < NH; v++) {
             const Real* r = plane + v * psz + pc;
             const Real u5[5] = {r[-2 * st], r[-st], r[0], r[st], r[2 * st]};
@@ -454,7 +483,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
       for (int n = 0; n < F_NUM; n++) Hz[n] = Real(0.);
       {
         const int nfxp = rup64(nfx);
-        const int utot = Cp + (xyk ? nfxp + nfy : 0);
+        const int utot = Cp + (xyk ? nfxp : 0);
         for (int u = tid; u < utot; u += NT) {
           if (u < Cp) {
             if (u < C && k >= I.z0) {
@@ -477,57 +506,14 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
             }
             continue;
           }
-          int w = u - Cp;
-          const bool isx = w < nfxp;
-          if (isx && w >= nfx) continue;
-          if (!isx) w -= nfxp;
-          Real Ls[R_NUM], Rs[R_NUM], cl[5], cr[5], F[F_NUM];
-          int fi, fj;  // the face is the low face of cell (fi, fj) of the tile
-          if (isx) {
-            fj = fquot(w, d_tx1, fi);
-            const int il = fj * (tx + 2) + fi;
-            face_states(xp, xm, nrx, il, il + 1, Ls, Rs);
-            const int pc = (fj + 3) * PW + (fi + 3);
-            centred(plane, psz, pc - 1, cl);
-            centred(plane, psz, pc, cr);
-            hydro_face<0, RIEMANN>(rp, Ls, Rs, cl, cr, F, A.err);
-#pragma unroll
-            for (int n = 0; n < F_NUM; n++) fxb[n * nfx + w] = F[n];
-            mxb[w] = F[F_MASS];
-          } else {
-            fj = fquot(w, d_tx, fi);
-            const int il = fj * tx + fi;
-            face_states(yp, ym, nry, il, il + tx, Ls, Rs);
-            const int pc = (fj + 3) * PW + (fi + 3);
-            centred(plane, psz, pc - PW, cl);
-            centred(plane, psz, pc, cr);
-            hydro_face<1, RIEMANN>(rp, Ls, Rs, cl, cr, F, A.err);
-#pragma unroll
-            for (int n = 0; n < F_NUM; n++) fyb[n * nfy + w] = F[n];
-            myb[w] = F[F_MASS];
-          }
-          if (regs) {
-            const int i = I.x0 + fi, j = I.y0 + fj;
-            if (isx) {
-              if ((fi < tx || i == I.xhi + 1) && i >= g && i <= g + m.nx && j >= g && j < g + m.ny && k >= g &&
-                  k < g + m.nz) {
-                const int ex = m.nx + 1, ey = m.ny;
-                const int64_t tot = (int64_t)ex * ey * m.nz;
-                const int64_t o = m.bbase + (i - g) + (int64_t)ex * ((j - g) + (int64_t)ey * (k - g));
-#pragma unroll
-                for (int n = 0; n < F_NUM; n++) reg_add(A.fb[0], o, tot, n, A.weight, A.add_flux, F[n]);
-              }
-            } else {
-              if ((fj < ty || j == I.yhi + 1) && i >= g && i < g + m.nx && j >= g && j <= g + m.ny && k >= g &&
-                  k < g + m.nz) {
-                const int ex = m.nx, ey = m.ny + 1;
-                const int64_t tot = (int64_t)ex * ey * m.nz;
-                const int64_t o = m.bbase + (i - g) + (int64_t)ex * ((j - g) + (int64_t)ey * (k - g));
-#pragma unroll
-                for (int n = 0; n < F_NUM; n++) reg_add(A.fb[1], o, tot, n, A.weight, A.add_flux, F[n]);
-              }
-            }
-          }
+          const int w = u - Cp;
+          if (w < nfx) xy_face<0, RIEMANN>(A, m, I, rp, g, k, xp, xm, nrx, plane, psz, PW, d_tx1, fxb, mxb, nfx, w);
+        }
+        // y faces continue the same thread sequence (one face direction per loop)
+        if (xyk) {
+          const int u0 = Cp + nfxp;
+          for (int w = ((tid - u0) % NT + NT) % NT; w < nfy; w += NT)
+            xy_face<1, RIEMANN>(A, m, I, rp, g, k, yp, ym, nry, plane, psz, PW, d_tx, fyb, myb, nfy, w);
         }
       }
       __syncthreads();
@@ -619,9 +605,18 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
               if (u < Cp) {
                 if (u < C) {
                   const Real fl = use_flat ? flatb[fcc] : Real(1.);
+                  int64_t zc[5];
 #pragma unroll
-                  for (int j = 0; j < CH; j++)
-                    if (j < nc) recon1<RECON, Real>(wsc[c0 + j], fl, upsk[j], ums[j]);
+                  for (int w = 0; w < 5; w++) zc[w] = cell_at(m, nbt, g, ci, cj, k - 2 + w);
+#pragma unroll
+                  for (int j = 0; j < CH; j++) {
+                    if (j < nc) {
+                      const double* sp = S + (int64_t)(V_NPROP + c0 + j) * cs;
+                      const Real u5[5] = {(Real)sp[zc[0]], (Real)sp[zc[1]], (Real)sp[zc[2]], (Real)sp[zc[3]],
+                                          (Real)sp[zc[4]]};
+                      recon1<RECON, Real>(u5, fl, upsk[j], ums[j]);
+                    }
+                  }
                 }
                 continue;
               }
