@@ -45,8 +45,8 @@ WORKLOADS = {
                 "stages, WENO5-Z + HLLC, RK3, CFL 0.8, gamma 1.4",
     "rt_amr": "3D Rayleigh-Taylor, 8 mass scalars, 3-level block hierarchy of 4384 16^3 leaves (32/256/4096 per "
               "level, 256^3-equivalent), rho 2 over 1, g = -1 source, hydrostatic p (2.5 at the interface), "
-              "seeded interface modes of amplitude 0.01, device guard fill over levels and fine/coarse flux "
-              "correction every step, 10 guard cells, telescoping RK3, WENO5-Z + HLLC, CFL 0.8, gamma 1.4",
+              "seeded interface modes of amplitude 0.01, periodic x/y and reflecting z, device guard fill over levels "
+              "(limited linear prolongation) and fine/coarse flux correction every step, 10 guard cells, telescoping RK3, WENO5-Z + HLLC, CFL 0.4, gamma 1.4",
     "batched4096": "4096 independent 3D blocks of 8^3/16^3/32^3 (p .25/.625/.125, seeds 20261202/20261203), "
                    "levels 0-4, Fourier fields + Mach 2-5 shocks every 16th block, 10 guard cells, telescoping RK3, "
                    "WENO5-Z + HLLC, CFL 0.8, gamma 1.4",
@@ -117,7 +117,7 @@ def cpu_baseline(workload, target_s: float):
     elif workload == "rt_amr":
         prob = problems.rt_amr(nb0=(1, 1, 2))
         run = lambda n: pyoracle.amr_advance(prob.cfg, prob.blocks, prob.info["nb0"], prob.info["origins"],
-                                             prob.arrays, n, nthreads=threads)
+                                             prob.arrays, n, nthreads=threads, bc=prob.info["bc"])
         what = (f"the same Rayleigh-Taylor setup on a 1 x 1 x 2 level-1 grid ({len(prob.blocks)} leaves, "
                 f"{prob.zones} zones, guard fill and flux correction included)")
     else:
@@ -188,7 +188,7 @@ def main():
         if prob.ug is not None:
             b.set_uniform_grid(*prob.ug)
         if "origins" in prob.info:
-            b.set_amr_grid(prob.info["nb0"], prob.info["origins"])
+            b.set_amr_grid(prob.info["nb0"], prob.info["origins"], prob.info["bc"])
         return b
 
     def timed(form):

@@ -24,11 +24,90 @@ __device__ __forceinline__ int leaf_at(const AmrGrid& G, int L, const int gi[3])
   return G.map[s0 + (int64_t)G.md[0] * (s1 + (int64_t)G.md[1] * s2)];
 }
 
+// global cell index of level L wrapped (PERIODIC), clamped (OUTFLOW,
+// Grid_bcApplyToRegion.F90:526-532) or mirrored (REFLECT) into the domain;
+// returns the mask of the dimensions whose normal velocity changes sign
+__device__ __forceinline__ int amr_domain(const AmrGrid& G, int L, int gi[3]) {
+  int flip = 0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const int N = (G.nb0[d] * G.n) << (L - 1);
+    if (G.bc[d] == AMR_BC_PERIODIC) gi[d] = ((gi[d] % N) + N) % N;
+    else if (G.bc[d] == AMR_BC_REFLECT && (gi[d] < 0 || gi[d] > N - 1)) {
+      gi[d] = gi[d] < 0 ? -1 - gi[d] : 2 * N - 1 - gi[d];
+      flip |= 1 << d;
+    } else gi[d] = gi[d] < 0 ? 0 : (gi[d] > N - 1 ? N - 1 : gi[d]);
+  }
+  return flip;
+}
+
+__device__ __forceinline__ double flipped(double x, int flip, int v) {
+  return (v >= V_VELX && v <= V_VELZ && ((flip >> (v - V_VELX)) & 1)) ? -x : x;
+}
+
+// where the value of a level-L cell comes from: the containing cell of a
+// same-level or coarser leaf, or the 8 cells of a finer leaf (averaged)
+struct Src {
+  int64_t off, sy, sz, cs;
+  bool avg;
+};
+
+__device__ __forceinline__ Src amr_src(const BlockMeta* __restrict__ meta, const AmrGrid& G, int g, int L,
+                                       const int gi[3]) {
+  const int s = leaf_at(G, L, gi);
+  const BlockMeta ms = meta[s];
+  Src r;
+  r.sy = ms.NX;
+  r.sz = (int64_t)ms.NX * ms.NX;
+  r.cs = ms.cstride;
+  int c[3];
+  const bool coarse = ms.level <= L;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const int o = G.org[3 * s + d];
+    c[d] = coarse ? (gi[d] >> (L - ms.level)) - o + g : 2 * gi[d] - o + g;
+    // a leaf more than one level finer is excluded when the hierarchy is
+    // declared; keep the read inside the block regardless
+    c[d] = c[d] < 0 ? 0 : (c[d] > ms.NX - 2 ? ms.NX - 2 : c[d]);
+  }
+  r.avg = !coarse;
+  r.off = ms.base + acix(ms.NX, c[0], c[1], c[2]);
+  return r;
+}
+
+__device__ __forceinline__ double src_val(const double* __restrict__ U, const Src& r, int v) {
+  const double* p = U + r.off + v * r.cs;
+  if (!r.avg) return p[0];
+  double sum = 0.0;
+  sum = sum + p[0];
+  sum = sum + p[1];
+  sum = sum + p[r.sy];
+  sum = sum + p[1 + r.sy];
+  sum = sum + p[r.sz];
+  sum = sum + p[1 + r.sz];
+  sum = sum + p[r.sy + r.sz];
+  sum = sum + p[1 + r.sy + r.sz];
+  return 0.125 * sum;
+}
+
+// monotonized-central slope of a coarse cell from its two differences
+__device__ __forceinline__ double mc_slope(double a, double b) {
+  if (a * b <= 0.0) return 0.0;
+  const double t = __builtin_fabs(a) < __builtin_fabs(b) ? 2.0 * __builtin_fabs(a) : 2.0 * __builtin_fabs(b);
+  const double c = 0.5 * __builtin_fabs(a + b);
+  const double m = t < c ? t : c;
+  return a > 0.0 ? m : -m;
+}
+
 // Guard fill (Grid_fillGuardCells over a Paramesh hierarchy): each guard
-// cell of depth <= depth takes, with its global index clamped into the
-// domain (OUTFLOW, Grid_bcApplyToRegion.F90:526-532), the value of the leaf
-// covering it: a copy from a same-level leaf, the containing coarse cell of
-// a coarser leaf (injection), or the mean of the 8 cells of a finer leaf.
+// cell of depth <= depth takes, with its global index wrapped, clamped or
+// mirrored into the domain (the normal velocity flips sign across a
+// reflecting face), the value of the leaf covering it: a copy from a
+// same-level leaf, the mean of the 8 cells of a finer leaf, and from a
+// coarser leaf a limited linear prolongation (the parent coarse cell plus,
+// per dimension, the monotonized-central slope of its face neighbours times
+// +-1/4; piecewise-constant injection feeds an odd-even mode into the fine
+// blocks that grows at fine/coarse faces under gravity).
 __global__ void __launch_bounds__(256) k_amr_fill(DevParams P, const BlockMeta* __restrict__ meta, AmrGrid G,
                                                   double* __restrict__ U, VarList vars, int depth) {
   const int bid = blockIdx.y;
@@ -44,37 +123,43 @@ __global__ void __launch_bounds__(256) k_amr_fill(DevParams P, const BlockMeta* 
 #pragma unroll
     for (int d = 0; d < 3; d++) {
       if (loc[d] < g || loc[d] >= g + n) inside = false;
-      const int x = o[d] + loc[d] - g;
-      const int xmax = ((G.nb0[d] * n) << (L - 1)) - 1;
-      gi[d] = x < 0 ? 0 : (x > xmax ? xmax : x);
+      gi[d] = o[d] + loc[d] - g;
     }
     if (inside) continue;
-    const int s = leaf_at(G, L, gi);
-    const BlockMeta ms = meta[s];
-    const int os[3] = {G.org[3 * s], G.org[3 * s + 1], G.org[3 * s + 2]};
+    const int flip = amr_domain(G, L, gi);
     const int64_t dst = m.base + acix(m.NX, loc[0], loc[1], loc[2]);
-    if (ms.level < L - 1 || ms.level > L + 1) continue;  // excluded when the hierarchy is declared
-    if (ms.level <= L) {
-      const int sh = L - ms.level;
-      const int64_t src =
-          ms.base + acix(ms.NX, (gi[0] >> sh) - os[0] + g, (gi[1] >> sh) - os[1] + g, (gi[2] >> sh) - os[2] + g);
-      for (int q = 0; q < vars.n; q++) U[dst + vars.v[q] * m.cstride] = U[src + vars.v[q] * ms.cstride];
-    } else {
-      const int64_t src = ms.base + acix(ms.NX, 2 * gi[0] - os[0] + g, 2 * gi[1] - os[1] + g, 2 * gi[2] - os[2] + g);
-      const int64_t sx = 1, sy = ms.NX, sz = (int64_t)ms.NX * ms.NX;
-      for (int q = 0; q < vars.n; q++) {
-        const double* p = U + src + vars.v[q] * ms.cstride;
-        double sum = 0.0;
-        sum = sum + p[0];
-        sum = sum + p[sx];
-        sum = sum + p[sy];
-        sum = sum + p[sx + sy];
-        sum = sum + p[sz];
-        sum = sum + p[sx + sz];
-        sum = sum + p[sy + sz];
-        sum = sum + p[sx + sy + sz];
-        U[dst + vars.v[q] * m.cstride] = 0.125 * sum;
+    const int s = leaf_at(G, L, gi);
+    if (meta[s].level != L - 1) {
+      const Src r = amr_src(meta, G, g, L, gi);
+      for (int q = 0; q < vars.n; q++) U[dst + vars.v[q] * m.cstride] = flipped(src_val(U, r, vars.v[q]), flip, vars.v[q]);
+      continue;
+    }
+    int gc[3] = {gi[0] >> 1, gi[1] >> 1, gi[2] >> 1};
+    const Src c0 = amr_src(meta, G, g, L - 1, gc);
+    Src sm[3], sp[3];
+    int fm[3], fp[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      int gm[3] = {gc[0], gc[1], gc[2]}, gp[3] = {gc[0], gc[1], gc[2]};
+      gm[d] -= 1;
+      gp[d] += 1;
+      fm[d] = amr_domain(G, L - 1, gm);
+      fp[d] = amr_domain(G, L - 1, gp);
+      sm[d] = amr_src(meta, G, g, L - 1, gm);
+      sp[d] = amr_src(meta, G, g, L - 1, gp);
+    }
+    for (int q = 0; q < vars.n; q++) {
+      const int v = vars.v[q];
+      const double c = src_val(U, c0, v);
+      double val = c;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        const double cm = flipped(src_val(U, sm[d], v), fm[d], v);
+        const double cp = flipped(src_val(U, sp[d], v), fp[d], v);
+        const double sl = mc_slope(c - cm, cp - c);
+        val = val + sl * ((gi[d] & 1) ? 0.25 : -0.25);
       }
+      U[dst + v * m.cstride] = flipped(val, flip, v);
     }
   }
 }
@@ -107,7 +192,8 @@ __global__ void __launch_bounds__(256) k_flux_exchange(DevParams P, const BlockM
     f[d2] = a2;
     f[d3] = a3;
     const int64_t co = m.bbase + f[0] + (int64_t)e[0] * (f[1] + (int64_t)e[1] * f[2]);
-    const int across = side ? o[dir] + n : o[dir] - 1;
+    int across = side ? o[dir] + n : o[dir] - 1;
+    if (G.bc[dir] == AMR_BC_PERIODIC) across = (across + gmax + 1) % (gmax + 1);
     int s = -1;
     if (across >= 0 && across <= gmax) {
       int gi[3];

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -1143,7 +1144,7 @@ int spark_flux_exchange(spark_batch* b) {
   return SPARK_OK;
 }
 
-int spark_set_amr_grid(spark_batch* b, int nbx0, int nby0, int nbz0, const int* origins) {
+int spark_set_amr_grid(spark_batch* b, int nbx0, int nby0, int nbz0, const int* origins, const int* bc) {
   if (!b || !origins) return fail(SPARK_ERR_ARG, "spark_set_amr_grid: null argument");
   if (b->p.ndim != 3) return fail(SPARK_ERR_ARG, "spark_set_amr_grid: 3D only");
   if (b->p.guard_mode != SPARK_GUARD_TELESCOPE)
@@ -1185,19 +1186,30 @@ int spark_set_amr_grid(spark_batch* b, int nbx0, int nby0, int nbz0, const int* 
   // every leaf under a block's guard region is within one level of it (the
   // guard fill and the flux exchange handle coarser-by-one, same and
   // finer-by-one sources only)
+  int per[3], bcs[3];
+  for (int d = 0; d < 3; d++) {
+    bcs[d] = bc ? bc[d] : SPARK_BC_OUTFLOW;
+    if (bcs[d] < SPARK_BC_OUTFLOW || bcs[d] > SPARK_BC_REFLECT)
+      return fail(SPARK_ERR_ARG, "spark_set_amr_grid: boundary code %d in dimension %d", bcs[d], d);
+    per[d] = bcs[d] == SPARK_BC_PERIODIC;
+  }
   for (int i = 0; i < b->nblk; i++) {
     const int L = b->meta_h[i].level, sh = lmax - L, g = b->p.nguard;
     int lo[3], hi[3];
     for (int d = 0; d < 3; d++) {
       const int cmax = (nb0[d] * n << (L - 1)) - 1;
-      const int c0 = std::max(0, origins[3 * i + d] - g), c1 = std::min(cmax, origins[3 * i + d] + n - 1 + g);
-      lo[d] = (c0 << sh) / n;
-      hi[d] = std::min(((((c1 + 1) << sh) - 1) / n) + 1, md[d]);
+      int c0 = origins[3 * i + d] - g, c1 = origins[3 * i + d] + n - 1 + g;
+      if (!per[d]) { c0 = std::max(0, c0); c1 = std::min(cmax, c1); }
+      // slot range (floor division), wrapped below for periodic dimensions
+      lo[d] = (int)std::floor((double)((int64_t)c0 << sh) / n);
+      hi[d] = (int)std::floor((double)((((int64_t)c1 + 1) << sh) - 1) / n) + 1;
+      if (!per[d]) hi[d] = std::min(hi[d], md[d]);
     }
     for (int z = lo[2]; z < hi[2]; z++)
       for (int y = lo[1]; y < hi[1]; y++)
         for (int x = lo[0]; x < hi[0]; x++) {
-          const int L2 = b->meta_h[map[x + (size_t)md[0] * (y + (size_t)md[1] * z)]].level;
+          const int X = ((x % md[0]) + md[0]) % md[0], Y = ((y % md[1]) + md[1]) % md[1], Z = ((z % md[2]) + md[2]) % md[2];
+          const int L2 = b->meta_h[map[X + (size_t)md[0] * (Y + (size_t)md[1] * Z)]].level;
           if (L2 - L > 1 || L - L2 > 1)
             return fail(SPARK_ERR_ARG, "block %d (level %d): a leaf of level %d lies under its guard cells", i, L, L2);
         }
@@ -1212,7 +1224,7 @@ int spark_set_amr_grid(spark_batch* b, int nbx0, int nby0, int nbz0, const int* 
   HIPCHK(hipMemcpy(b->amr_org_d, origins, sizeof(int) * 3 * b->nblk, hipMemcpyHostToDevice));
   b->amr_g.lmax = lmax;
   b->amr_g.n = n;
-  for (int d = 0; d < 3; d++) { b->amr_g.nb0[d] = nb0[d]; b->amr_g.md[d] = md[d]; }
+  for (int d = 0; d < 3; d++) { b->amr_g.nb0[d] = nb0[d]; b->amr_g.md[d] = md[d]; b->amr_g.bc[d] = bcs[d]; }
   b->amr_g.map = b->amr_map_d;
   b->amr_g.org = b->amr_org_d;
   b->amr = true;

@@ -83,7 +83,9 @@ struct AmrGrid {
   int md[3];      // leaf map extent: nb0 << (lmax - 1)
   const int* map; // finest-level block slot -> leaf block
   const int* org; // 3 per block: first interior cell, global index at its level
+  int bc[3];      // per dimension: AMR_BC_OUTFLOW, AMR_BC_PERIODIC, AMR_BC_REFLECT
 };
+enum { AMR_BC_OUTFLOW = 0, AMR_BC_PERIODIC = 1, AMR_BC_REFLECT = 2 };
 hipError_t amr_fill(const PlainLaunch& L, const AmrGrid& G, double* U, const VarList& vars, int depth);
 hipError_t amr_flux_exchange(const PlainLaunch& L, const AmrGrid& G, double* const fb[3], double* const fc[3]);
 hipError_t plain_aos_to_soa(hipStream_t st, const BlockMeta* meta, int nblk, int maxcells, const double* aos,

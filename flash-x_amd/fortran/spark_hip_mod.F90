@@ -23,6 +23,7 @@ module spark_hip
                                 SPARK_FORM_BATCHED_FP64 = 2
   integer, parameter, public :: SPARK_GUARD_TELESCOPE = 0, SPARK_GUARD_REFILL = 1
   integer, parameter, public :: SPARK_OPT_GRAPH = 1, SPARK_OPT_AUTOTUNE = 2
+  integer, parameter, public :: SPARK_BC_OUTFLOW = 0, SPARK_BC_PERIODIC = 1, SPARK_BC_REFLECT = 2
 
   !! struct spark_params (same member order and types)
   type, bind(C), public :: spark_params_t
@@ -104,12 +105,15 @@ module spark_hip
        integer(c_int), value :: nbx, nby, nbz
      end function spark_set_uniform_grid
 
-     !! origins(3*nblocks): first interior cell of each block, global index at its level
-     integer(c_int) function spark_set_amr_grid(batch, nbx0, nby0, nbz0, origins) bind(C, name="spark_set_amr_grid")
+     !! origins(3*nblocks): first interior cell of each block, global index at its level;
+     !! bc(3): SPARK_BC_OUTFLOW, SPARK_BC_PERIODIC or SPARK_BC_REFLECT per dimension
+     integer(c_int) function spark_set_amr_grid(batch, nbx0, nby0, nbz0, origins, bc) &
+          bind(C, name="spark_set_amr_grid")
        import :: c_int, c_ptr
        type(c_ptr), value :: batch
        integer(c_int), value :: nbx0, nby0, nbz0
        integer(c_int), intent(in) :: origins(*)
+       integer(c_int), intent(in) :: bc(3)
      end function spark_set_amr_grid
 
      integer(c_int) function spark_flux_exchange(batch) bind(C, name="spark_flux_exchange")

@@ -22,6 +22,7 @@ SPARK_NPROP = 11
 SPARK_MAX_SCALARS = 8
 
 RECON = {"weno5": 0, "mp5": 1, "plm": 2, "fog": 3}
+BC = {"outflow": 0, "periodic": 1, "reflect": 2}
 RIEMANN = {"hllc": 0, "hll": 1}
 FORM = {"plain_fp64": 0, "fp32_flux": 1, "batched_fp64": 2}
 GUARD = {"telescope": 0, "refill": 1}
@@ -94,7 +95,8 @@ def lib():
             "spark_set_flux_corrections": [vp, dpp, dpp, dpp],
             "spark_flux_correct": [vp, ctypes.c_double],
             "spark_flux_exchange": [vp],
-            "spark_set_amr_grid": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+            "spark_set_amr_grid": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                   ctypes.POINTER(ctypes.c_int)],
             "spark_synchronize": [vp],
             "spark_get_counters": [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.c_int],
             "spark_last_error": [ctypes.c_char_p, ctypes.c_int],
@@ -280,12 +282,15 @@ class Batch:
     def flux_correct(self, dt):
         check(lib().spark_flux_correct(self.h, float(dt)), "spark_flux_correct")
 
-    def set_amr_grid(self, nb0, origins):
+    def set_amr_grid(self, nb0, origins, bc=(0, 0, 0)):
         """nb0: level-1 blocks per dimension; origins: (nblocks, 3) first
-        interior cell of each block in its level's global index space."""
+        interior cell of each block in its level's global index space;
+        bc: per dimension, BC["outflow"] (0), BC["periodic"] (1) or
+        BC["reflect"] (2)."""
         o = np.ascontiguousarray(np.asarray(origins, dtype=np.int32).reshape(-1))
+        per = (ctypes.c_int * 3)(*[BC[x] if isinstance(x, str) else int(x) for x in bc])
         check(lib().spark_set_amr_grid(self.h, int(nb0[0]), int(nb0[1]), int(nb0[2]),
-                                       o.ctypes.data_as(ctypes.POINTER(ctypes.c_int))), "spark_set_amr_grid")
+                                       o.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), per), "spark_set_amr_grid")
 
     def flux_exchange(self):
         check(lib().spark_flux_exchange(self.h), "spark_flux_exchange")

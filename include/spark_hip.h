@@ -123,16 +123,20 @@ int spark_set_uniform_grid(spark_batch* b, int nbx, int nby, int nbz);
  * telescope guard mode, all blocks n^3 with n even): level-1 grid of
  * nbx0 x nby0 x nbz0 blocks; block i (its spark_block_desc level is 1-based)
  * has its first interior cell at origins[3*i .. 3*i+2], the global cell
- * index at its own level.  The leaves must tile the domain, and every leaf
- * under a block's guard cells must be within one level of it.  Enables the
- * device guard fill over levels (same-level copy, injection from a coarser
- * leaf, the mean of 8 cells of a finer leaf, OUTFLOW at the domain faces)
+ * index at its own level; bc[3] (NULL: all OUTFLOW) gives each dimension's
+ * domain boundary, SPARK_BC_OUTFLOW, SPARK_BC_PERIODIC or SPARK_BC_REFLECT.  The leaves must tile the domain, and
+ * every leaf under a block's guard cells must be within one level of it.
+ * Enables the device guard fill over levels (same-level copy, the mean of 8
+ * cells of a finer leaf, limited linear prolongation from a coarser leaf)
  * and, with flux_correct, the device flux-register exchange after every
  * step.
  * Replaces: Grid_fillGuardCells over PARAMESH leaves,
  * Grid/GridMain/AMR/Paramesh4/Grid_fillGuardCells.F90:436-546, and the
  * flux hand-off of Spark/Hydro.F90-mc:197-236. */
-int spark_set_amr_grid(spark_batch* b, int nbx0, int nby0, int nbz0, const int* origins);
+#define SPARK_BC_OUTFLOW 0   /* guard = first interior layer (Grid_bcApplyToRegion.F90:526-532) */
+#define SPARK_BC_PERIODIC 1
+#define SPARK_BC_REFLECT 2   /* mirror image, normal velocity negated */
+int spark_set_amr_grid(spark_batch* b, int nbx0, int nby0, int nbz0, const int* origins, const int* bc);
 int spark_fill_guardcells(spark_batch* b);
 
 /* CFL time step over all blocks.  minloc = (i, j, k, level, block): the

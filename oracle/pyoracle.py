@@ -224,10 +224,10 @@ def eos_cell(cfg, v, full=True):
 
 class OrcAmr(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("nb0", ctypes.c_int * 3), ("level", ctypes.POINTER(ctypes.c_int)),
-                ("origin", ctypes.POINTER(ctypes.c_int))]
+                ("origin", ctypes.POINTER(ctypes.c_int)), ("bc", ctypes.c_int * 3)]
 
 
-def _amr(n, nb0, levels, origins):
+def _amr(n, nb0, levels, origins, bc=(0, 0, 0)):
     lv = np.ascontiguousarray(np.asarray(levels, dtype=np.int32))
     og = np.ascontiguousarray(np.asarray(origins, dtype=np.int32).reshape(-1))
     A = OrcAmr()
@@ -236,26 +236,28 @@ def _amr(n, nb0, levels, origins):
         A.nb0[d] = int(nb0[d])
     A.level = lv.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
     A.origin = og.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    for d in range(3):
+        A.bc[d] = int(bc[d])
     return A, (lv, og)
 
 
-def amr_fill(cfg, blocks, nb0, origins, arrays, depth):
+def amr_fill(cfg, blocks, nb0, origins, arrays, depth, bc=(0, 0, 0)):
     P = params(cfg)
-    A, keep = _amr(blocks[0]["nx"], nb0, [b["level"] for b in blocks], origins)
+    A, keep = _amr(blocks[0]["nx"], nb0, [b["level"] for b in blocks], origins, bc)
     nunk = 11 + cfg["nscalars"]
     lib().orc_amr_fill_guards(ctypes.byref(P), ctypes.byref(A), len(blocks), ptr_array(arrays), nunk, int(depth))
 
 
-def flux_exchange(cfg, blocks, nb0, origins, fb, fc):
+def flux_exchange(cfg, blocks, nb0, origins, fb, fc, bc=(0, 0, 0)):
     """fb / fc: lists of 3 arrays per block (x, y, z registers, flux fastest)."""
     P = params(cfg)
-    A, keep = _amr(blocks[0]["nx"], nb0, [b["level"] for b in blocks], origins)
+    A, keep = _amr(blocks[0]["nx"], nb0, [b["level"] for b in blocks], origins, bc)
     lib().orc_flux_exchange(ctypes.byref(P), ctypes.byref(A), len(blocks), ptr_array(fb), ptr_array(fc))
 
 
-def amr_advance(cfg, blocks, nb0, origins, arrays, nsteps, flux_correct=True, nthreads=0):
+def amr_advance(cfg, blocks, nb0, origins, arrays, nsteps, flux_correct=True, nthreads=0, bc=(0, 0, 0)):
     P = params(cfg)
-    A, keep = _amr(blocks[0]["nx"], nb0, [b["level"] for b in blocks], origins)
+    A, keep = _amr(blocks[0]["nx"], nb0, [b["level"] for b in blocks], origins, bc)
     bl = (OrcBlock * len(blocks))(*[block(b) for b in blocks])
     dts = np.zeros(max(1, nsteps))
     nerr = lib().orc_amr_advance(ctypes.byref(P), ctypes.byref(A), bl, len(blocks), ptr_array(arrays), nsteps,

@@ -1013,6 +1013,57 @@ static int amr_leaf_at(const orc_amr* A, const amr_map* M, int L, const int gi[3
   return M->leaf[s[0] + (size_t)M->md[0] * (s[1] + (size_t)M->md[1] * s[2])];
 }
 
+/* global cell index of level L wrapped (PERIODIC), clamped (OUTFLOW,
+ * Grid_bcApplyToRegion.F90:526-532) or mirrored (REFLECT) into the domain;
+ * returns the mask of the dimensions whose normal velocity changes sign */
+static int amr_domain(const orc_amr* A, int L, int gi[3]) {
+  int flip = 0;
+  for (int d = 0; d < 3; d++) {
+    const int N = A->nb0[d] * A->n << (L - 1);
+    if (A->bc[d] == ORC_BC_PERIODIC) gi[d] = ((gi[d] % N) + N) % N;
+    else if (A->bc[d] == ORC_BC_REFLECT && (gi[d] < 0 || gi[d] > N - 1)) {
+      gi[d] = gi[d] < 0 ? -1 - gi[d] : 2 * N - 1 - gi[d];
+      flip |= 1 << d;
+    } else gi[d] = gi[d] < 0 ? 0 : (gi[d] > N - 1 ? N - 1 : gi[d]);
+  }
+  return flip;
+}
+
+static inline double flipped(double x, int flip, int v) {
+  return (v >= ORC_VELX && v <= ORC_VELZ && ((flip >> (v - ORC_VELX)) & 1)) ? -x : x;
+}
+
+/* variable v at level-L cell gi (inside the domain) from the leaf covering
+ * it: the containing cell of a same-level or coarser leaf, the mean of the
+ * 8 cells of a finer one */
+static double amr_value(const orc_amr* A, const amr_map* M, double* const* blocks, int nvar, int g, int NX, int L,
+                        const int gi[3], int v) {
+  const int s = amr_leaf_at(A, M, L, gi);
+  const int Ls = A->level[s];
+  const int* os = A->origin + 3 * s;
+  const double* src = blocks[s];
+  if (Ls <= L) {
+    int sl[3];
+    for (int d = 0; d < 3; d++) sl[d] = (gi[d] >> (L - Ls)) - os[d] + g;
+    return src[(size_t)nvar * cidx(NX, NX, sl[0], sl[1], sl[2]) + v];
+  }
+  int f0[3];
+  for (int d = 0; d < 3; d++) f0[d] = 2 * gi[d] - os[d] + g;
+  double sum = 0.0;
+  for (int c = 0; c < 8; c++)
+    sum = sum + src[(size_t)nvar * cidx(NX, NX, f0[0] + (c & 1), f0[1] + ((c >> 1) & 1), f0[2] + ((c >> 2) & 1)) + v];
+  return 0.125 * sum;
+}
+
+/* monotonized-central slope of a coarse cell from its two differences */
+static double mc_slope(double a, double b) {
+  if (a * b <= 0.0) return 0.0;
+  const double t = fabs(a) < fabs(b) ? 2.0 * fabs(a) : 2.0 * fabs(b);
+  const double c = 0.5 * fabs(a + b);
+  const double m = t < c ? t : c;
+  return a > 0.0 ? m : -m;
+}
+
 void orc_amr_fill_guards(const orc_params* P, const orc_amr* A, int nblocks, double* const* blocks, int nvar,
                          int depth) {
   amr_map M;
@@ -1030,30 +1081,34 @@ void orc_amr_fill_guards(const orc_params* P, const orc_amr* A, int nblocks, dou
           int inside = 1, gi[3];
           for (int d = 0; d < 3; d++) {
             if (loc[d] < g || loc[d] >= g + n) inside = 0;
-            int x = o[d] + loc[d] - g;
-            const int xmax = (A->nb0[d] * n << (L - 1)) - 1;
-            gi[d] = x < 0 ? 0 : (x > xmax ? xmax : x);
+            gi[d] = o[d] + loc[d] - g;
           }
           if (inside) continue;
+          const int flip = amr_domain(A, L, gi);
           const int s = amr_leaf_at(A, &M, L, gi);
-          const int Ls = A->level[s];
-          const int* os = A->origin + 3 * s;
           double* out = dst + (size_t)nvar * cidx(NX, NX, i, j, k);
-          const double* src = blocks[s];
-          if (Ls <= L) {
-            int sl[3];
-            for (int d = 0; d < 3; d++) sl[d] = (gi[d] >> (L - Ls)) - os[d] + g;
-            memcpy(out, src + (size_t)nvar * cidx(NX, NX, sl[0], sl[1], sl[2]), sizeof(double) * nvar);
-          } else {
-            int f0[3];
-            for (int d = 0; d < 3; d++) f0[d] = 2 * gi[d] - os[d] + g;
-            for (int v = 0; v < nvar; v++) {
-              double sum = 0.0;
-              for (int c = 0; c < 8; c++)
-                sum = sum + src[(size_t)nvar * cidx(NX, NX, f0[0] + (c & 1), f0[1] + ((c >> 1) & 1),
-                                                    f0[2] + ((c >> 2) & 1)) + v];
-              out[v] = 0.125 * sum;
+          if (A->level[s] != L - 1) {
+            for (int v = 0; v < nvar; v++) out[v] = flipped(amr_value(A, &M, blocks, nvar, g, NX, L, gi, v), flip, v);
+            continue;
+          }
+          /* a coarser leaf: limited linear prolongation from the parent
+           * coarse cell and its 6 face neighbours (MC slopes) */
+          int gc[3];
+          for (int d = 0; d < 3; d++) gc[d] = gi[d] >> 1;
+          for (int v = 0; v < nvar; v++) {
+            const double c0 = amr_value(A, &M, blocks, nvar, g, NX, L - 1, gc, v);
+            double val = c0;
+            for (int d = 0; d < 3; d++) {
+              int gm[3] = {gc[0], gc[1], gc[2]}, gp[3] = {gc[0], gc[1], gc[2]};
+              gm[d] -= 1;
+              gp[d] += 1;
+              const int fm = amr_domain(A, L - 1, gm), fp = amr_domain(A, L - 1, gp);
+              const double cm = flipped(amr_value(A, &M, blocks, nvar, g, NX, L - 1, gm, v), fm, v);
+              const double cp = flipped(amr_value(A, &M, blocks, nvar, g, NX, L - 1, gp, v), fp, v);
+              const double sl = mc_slope(c0 - cm, cp - c0);
+              val = val + sl * ((gi[d] & 1) ? 0.25 : -0.25);
             }
+            out[v] = flipped(val, flip, v);
           }
         }
   }
@@ -1077,8 +1132,9 @@ void orc_flux_exchange(const orc_params* P, const orc_amr* A, int nblocks, doubl
       const double* own = fb[3 * b + dir];
       for (int side = 0; side < 2; side++) {
         const int gmax = (A->nb0[dir] * n << (L - 1)) - 1;
-        const int across = side ? o[dir] + n : o[dir] - 1;
-        if (across < 0 || across > gmax) continue;
+        int across = side ? o[dir] + n : o[dir] - 1;
+        if (A->bc[dir] == ORC_BC_PERIODIC) across = (across + gmax + 1) % (gmax + 1);
+        else if (across < 0 || across > gmax) continue;
         for (int a3 = 0; a3 < n; a3++)
           for (int a2 = 0; a2 < n; a2++) {
             int gi[3];

@@ -125,13 +125,18 @@ typedef struct orc_amr {
   int nb0[3];
   const int* level;
   const int* origin;
+  int bc[3]; /* per dimension: ORC_BC_OUTFLOW, ORC_BC_PERIODIC or ORC_BC_REFLECT */
 } orc_amr;
+enum { ORC_BC_OUTFLOW = 0, ORC_BC_PERIODIC = 1, ORC_BC_REFLECT = 2 };
 
-/* Guard fill over the hierarchy: a guard cell takes the value of the leaf
- * covering its position -- a copy from a same-level leaf, the coarse cell
- * containing it from a coarser leaf (injection), the mean of the 8 fine
- * cells from a finer leaf -- with the global index clamped into the domain
- * first (OUTFLOW).  depth guard layers of all nvar variables. */
+/* Guard fill over the hierarchy: a guard cell takes, with its global index
+ * first wrapped (PERIODIC), clamped (OUTFLOW) or mirrored (REFLECT, the
+ * normal velocity changing sign) into the domain, the value of the leaf
+ * covering its position -- a copy from a same-level
+ * leaf, the mean of the 8 fine cells from a finer leaf, and from a coarser
+ * leaf a limited linear prolongation: the parent coarse cell plus, per
+ * dimension, a monotonized-central slope of its face neighbours times
+ * +-1/4.  depth guard layers of all nvar variables. */
 void orc_amr_fill_guards(const orc_params* P, const orc_amr* A, int nblocks, double* const* blocks, int nvar,
                          int depth);
 

@@ -1,5 +1,6 @@
-"""Block hierarchy on the GPU: device guard fill over levels and the
-fine/coarse flux-register exchange + correction, against the oracle.
+"""Block hierarchy on the GPU: device guard fill over levels (periodic x/y,
+reflecting z) and the fine/coarse flux-register exchange + correction,
+against the oracle.
 
 Bar: the plain FP64 form equals the oracle bit for bit (dt sequence and
 every interior value) over several steps of a 2-level Rayleigh-Taylor
@@ -30,7 +31,7 @@ def run(prob, nsteps, form):
     b = capi.Batch(dict(prob.cfg, form=form), prob.blocks)
     try:
         b.pack(prob.arrays)
-        b.set_amr_grid(prob.info["nb0"], prob.info["origins"])
+        b.set_amr_grid(prob.info["nb0"], prob.info["origins"], prob.info["bc"])
         dts = b.advance(nsteps)
         out = [a.copy() for a in prob.arrays]
         b.unpack(out)
@@ -50,14 +51,15 @@ def test_guard_fill_matches_oracle():
     b = capi.Batch(prob.cfg, prob.blocks)
     try:
         b.pack(prob.arrays)
-        b.set_amr_grid(prob.info["nb0"], prob.info["origins"])
+        b.set_amr_grid(prob.info["nb0"], prob.info["origins"], prob.info["bc"])
         b.fill_guardcells()
         out = [a.copy() for a in prob.arrays]
         b.unpack(out)
     finally:
         b.close()
     ref = [a.copy() for a in prob.arrays]
-    pyoracle.amr_fill(prob.cfg, prob.blocks, prob.info["nb0"], prob.info["origins"], ref, depth=prob.cfg["nguard"])
+    pyoracle.amr_fill(prob.cfg, prob.blocks, prob.info["nb0"], prob.info["origins"], ref, depth=prob.cfg["nguard"],
+                      bc=prob.info["bc"])
     nd = sum(int(np.count_nonzero(x != y)) for x, y in zip(out, ref))
     print(f"{len(prob.blocks)} blocks, levels {prob.info['leaves_per_level']}: differing values {nd}", flush=True)
     assert nd == 0
@@ -71,7 +73,7 @@ def test_plain_fp64_amr_steps_match_oracle(flux_correct):
     d, out, cnt = run(prob, nsteps, "plain_fp64")
     ref = [a.copy() for a in prob.arrays]
     dref, nerr = pyoracle.amr_advance(prob.cfg, prob.blocks, prob.info["nb0"], prob.info["origins"], ref, nsteps,
-                                      flux_correct=bool(flux_correct))
+                                      flux_correct=bool(flux_correct), bc=prob.info["bc"])
     assert nerr == 0 and cnt[0] == 0
     assert np.array_equal(d, dref), (d, dref)
     nd = sum(int(np.count_nonzero(x != y)) for x, y in zip(interiors(prob, out), interiors(prob, ref)))
@@ -92,7 +94,8 @@ def test_fp32_flux_amr_vs_oracle():
     nsteps = 5
     d32, o32, cnt = run(prob, nsteps, "fp32_flux")
     ref = [a.copy() for a in prob.arrays]
-    dref, _ = pyoracle.amr_advance(prob.cfg, prob.blocks, prob.info["nb0"], prob.info["origins"], ref, nsteps)
+    dref, _ = pyoracle.amr_advance(prob.cfg, prob.blocks, prob.info["nb0"], prob.info["origins"], ref, nsteps,
+                                   bc=prob.info["bc"])
     vols = [b["dx"] ** 3 for b in prob.blocks]
     gi, ri = interiors(prob, o32), interiors(prob, ref)
     num = sum(np.sum(np.abs(x[..., 0] - y[..., 0])) * v for x, y, v in zip(gi, ri, vols))

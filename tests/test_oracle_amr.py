@@ -42,10 +42,12 @@ def linear(X, Y, Z):
     return 1.0 + 2.0 * X + 3.0 * Y + 5.0 * Z
 
 
-def test_fill_across_levels_restriction_and_injection():
+def test_fill_across_levels_restriction_and_prolongation():
+    """On a linear field the mean of 8 fine cells (restriction) and the
+    limited linear prolongation (MC slopes) both reproduce the field at the
+    guard cell's centre; injection would miss it by a quarter coarse cell."""
     p = two_level()
     g, n = p.cfg["nguard"], 8
-    dx0 = 1.0 / 16
     arrays = []
     for b, o in zip(p.blocks, p.info["origins"]):
         NZ, NY, NX = problems.gc_shape(p.cfg, b)
@@ -54,27 +56,40 @@ def test_fill_across_levels_restriction_and_injection():
         U[...] = linear(X, Y, Z)[..., None]
         arrays.append(U)
     ref = [a.copy() for a in arrays]
-    pyoracle.amr_fill(p.cfg, p.blocks, p.info["nb0"], p.info["origins"], arrays, depth=g)
+    pyoracle.amr_fill(p.cfg, p.blocks, p.info["nb0"], p.info["origins"], arrays, depth=g)  # outflow everywhere
     checked_fine = checked_coarse = 0
     for b, o, a, r in zip(p.blocks, p.info["origins"], arrays, ref):
-        L, dx = b["level"], b["dx"]
-        NZ, NY, NX = a.shape[:3]
-        for (k, j, i) in [(g - 1, g + 3, g + 2), (g + n, g + 1, g + 5), (g + 2, g - 2, g + n + 1)]:
+        L = b["level"]
+        for (k, j, i) in [(g - 1, g + 3, g + 2), (g - 2, g + 4, g + 5), (g + n, g + 1, g + 5), (g - 3, g + 6, g + 1)]:
             gi = [o[0] + i - g, o[1] + j - g, o[2] + k - g]
-            nmax = 2 * 16 * 2 ** (L - 1)
-            if not all(0 <= gi[d] < (nmax if d == 2 else nmax // 2) for d in range(3)):
-                continue
-            zc = (gi[2] + 0.5) * dx
+            N = 16 * 2 ** (L - 1)
+            if not all(2 <= gi[d] < N - 2 for d in range(2)) or not 0 <= gi[2] < N // 2 * 2:
+                continue  # stencils clamped at the domain edge are not exact
+            zc = (gi[2] + 0.5) * b["dx"]
             if L == 1 and zc > 0.5:
-                # the source is the finer leaf: the mean of 8 cells of a linear field
-                assert abs(a[k, j, i, 0] - r[k, j, i, 0]) < 1e-13
+                assert abs(a[k, j, i, 0] - r[k, j, i, 0]) < 1e-13  # restriction from the finer leaf
                 checked_fine += 1
             if L == 2 and zc < 0.5:
-                # the source is the coarser leaf: the coarse cell containing it
-                c = [(gi[d] // 2 + 0.5) * 2 * dx for d in range(3)]
-                assert a[k, j, i, 0] == linear(c[0], c[1], c[2])
+                assert abs(a[k, j, i, 0] - r[k, j, i, 0]) < 1e-13  # prolongation from the coarser leaf
                 checked_coarse += 1
     assert checked_fine > 0 and checked_coarse > 0
+
+
+def test_prolongation_is_limited_at_a_jump():
+    """Across a discontinuity the MC slope vanishes, so fine guard cells
+    next to a jump take no value outside the two states."""
+    p = two_level()
+    g = p.cfg["nguard"]
+    arrays = []
+    for b, o in zip(p.blocks, p.info["origins"]):
+        NZ, NY, NX = problems.gc_shape(p.cfg, b)
+        Z, Y, X = problems._centers(p.cfg, b, tuple(o[d] * b["dx"] for d in range(3)), NZ, NY, NX)
+        U = np.zeros((NZ, NY, NX, 11))
+        U[...] = np.where(Z < 0.45, 1.0, 2.0)[..., None]
+        arrays.append(U)
+    pyoracle.amr_fill(p.cfg, p.blocks, p.info["nb0"], p.info["origins"], arrays, depth=g)
+    for a in arrays:
+        assert a[..., 0].min() >= 1.0 and a[..., 0].max() <= 2.0
 
 
 def test_flux_exchange_constant_registers():
@@ -97,3 +112,29 @@ def test_flux_exchange_constant_registers():
             assert np.all(fc[3 * bi] == 0.0) and np.all(fc[3 * bi + 1] == 0.0)
         else:
             assert np.all(cz == 0.0)
+
+
+def test_reflecting_and_periodic_boundaries():
+    """REFLECT mirrors the interior across the domain face and negates the
+    normal velocity; PERIODIC wraps to the opposite side."""
+    p = two_level(nscalars=0)
+    g = p.cfg["nguard"]
+    arrays = []
+    rng = np.random.default_rng(3)
+    for b in p.blocks:
+        arrays.append(rng.random(problems.gc_shape(p.cfg, b) + (11,)))
+    pyoracle.amr_fill(p.cfg, p.blocks, p.info["nb0"], p.info["origins"], arrays, depth=g, bc=(1, 1, 2))
+    V = {"velx": 8, "vely": 9, "velz": 10}
+    for b, o, a in zip(p.blocks, p.info["origins"], arrays):
+        if b["level"] != 1 or o[2] != 0:
+            continue
+        # z = 0 face of the domain: guard layer m mirrors interior layer m
+        for m in range(3):
+            assert np.array_equal(a[g - 1 - m, g:-g, g:-g, V["velz"]], -a[g + m, g:-g, g:-g, V["velz"]])
+            assert np.array_equal(a[g - 1 - m, g:-g, g:-g, V["velx"]], a[g + m, g:-g, g:-g, V["velx"]])
+            assert np.array_equal(a[g - 1 - m, g:-g, g:-g, 0], a[g + m, g:-g, g:-g, 0])
+        if o[0] == 0:
+            # x = 0 face: periodic, the guard layer left of x = 0 is the last interior layer of the x-last block
+            other = [c for c, oo, bb in zip(arrays, p.info["origins"], p.blocks)
+                     if bb["level"] == 1 and oo[0] == 8 and oo[1] == o[1] and oo[2] == 0][0]
+            assert np.array_equal(a[g:-g, g:-g, g - 1, :], other[g:-g, g:-g, -g - 1, :])
