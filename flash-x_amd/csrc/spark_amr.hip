@@ -108,9 +108,10 @@ __device__ __forceinline__ double mc_slope(double a, double b) {
 // per dimension, the monotonized-central slope of its face neighbours times
 // +-1/4; piecewise-constant injection feeds an odd-even mode into the fine
 // blocks that grows at fine/coarse faces under gravity).
-__global__ void __launch_bounds__(256) k_amr_fill(DevParams P, const BlockMeta* __restrict__ meta, AmrGrid G,
+__global__ void __launch_bounds__(256) k_amr_fill(DevParams P, const BlockMeta* __restrict__ meta,
+                                                  const int* __restrict__ blist, AmrGrid G,
                                                   double* __restrict__ U, VarList vars, int depth) {
-  const int bid = blockIdx.y;
+  const int bid = blist ? blist[blockIdx.y] : (int)blockIdx.y;
   const BlockMeta m = meta[bid];
   const int n = G.n, g = P.ng, L = m.level;
   const int ext = n + 2 * depth;
@@ -240,8 +241,8 @@ hipError_t amr_fill(const PlainLaunch& L, const AmrGrid& G, double* U, const Var
   const int box = ext * ext * ext;
   const int nin = G.n * G.n * G.n;
   if (L.prof) L.prof->begin(L.stream, "k_amr_fill", 16.0 * L.nblk * (double)(box - nin) * vars.n, 0.0);
-  hipLaunchKernelGGL(k_amr_fill, dim3(grid_x(box, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta, G, U, vars,
-                     depth);
+  hipLaunchKernelGGL(k_amr_fill, dim3(grid_x(box, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta, L.blist, G, U,
+                     vars, depth);
   if (L.prof) L.prof->end(L.stream);
   return hipGetLastError();
 }

@@ -322,7 +322,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
     if (A.nbr) {
       if (tid < 27) s_nb[tid] = A.nbr[(int64_t)I.blk * 27 + tid];
       __syncthreads();
-      nbt = s_nb;
+      if (s_nb[13] >= 0) nbt = s_nb;  // -1: the block reads its own guard cells
     }
     const double* u0p = A.U0 + m.base;
     double* sout = A.Sout + m.base;

@@ -31,10 +31,11 @@ struct FusedArgs {
   const double* Sin;       // stage input state
   double* Sout;            // stage output state (never equal to Sin)
   double* fb[3];           // flux registers (nullptr when not correcting)
-  // uniform block grid, guards not refilled between stages: per block the
-  // 27 pool offsets (BlockMeta::base) of the blocks around it, -1 where the
-  // domain ends (outflow).  Cells outside a block's interior are read from
-  // those neighbours.  nullptr: read the block's own guard cells.
+  // per block the 27 pool offsets (BlockMeta::base) of the blocks around
+  // it: cells outside the block's interior are read from those neighbours
+  // (uniform grid, guards not refilled between stages: -1 where the domain
+  // ends, outflow).  A row whose centre entry is -1, or nullptr: the block
+  // reads its own guard cells.
   const int64_t* nbr;
   double weight;
   int add_flux;

@@ -98,6 +98,14 @@ struct spark_batch {
   size_t lds_limit = 0;   // LDS bytes a workgroup may allocate
   int64_t* nbr_d = nullptr;  // uniform grid: 27 neighbour pool offsets per block
   bool amr = false;          // block hierarchy declared (spark_set_amr_grid)
+  // batched forms on a block hierarchy: blocks whose 26 neighbours are leaves
+  // of their own level read their halos from them (nbr_d row), every stage
+  // over the interior only; the others telescope over their guard cells
+  std::vector<char> amr_remap_blk;
+  int amr_remap_count = 0;
+  int* remap_ids_d = nullptr;  // the remapping blocks
+  int* tele_ids_d = nullptr;   // the telescoping blocks
+  int n_tele = 0;
   spark::AmrGrid amr_g{};
   int* amr_map_d = nullptr;
   int* amr_org_d = nullptr;
@@ -213,8 +221,22 @@ int run_fill(spark_batch* b, double* pool, bool full, int depth) {
     for (int v : need) vl.v[vl.n++] = v;
     for (int s = 0; s < b->p.nscalars; s++) vl.v[vl.n++] = spark::V_NPROP + s;
   }
-  if (b->amr) HIPCHK(spark::amr_fill(L, b->amr_g, pool, vl, depth));
-  else HIPCHK(spark::plain_ug_fill(L, b->ug[0], b->ug[1], b->ug[2], pool, vl, depth));
+  if (b->amr && !full && b->amr_remap_count > 0) {
+    // remapping blocks read no guard cell of their own in the stages: fill
+    // only the two layers shockDetect reads
+    if (b->n_tele > 0) {
+      L.blist = b->tele_ids_d;
+      L.nblk = b->n_tele;
+      HIPCHK(spark::amr_fill(L, b->amr_g, pool, vl, depth));
+    }
+    L.blist = b->remap_ids_d;
+    L.nblk = b->amr_remap_count;
+    HIPCHK(spark::amr_fill(L, b->amr_g, pool, vl, std::min(depth, 2)));
+  } else if (b->amr) {
+    HIPCHK(spark::amr_fill(L, b->amr_g, pool, vl, depth));
+  } else {
+    HIPCHK(spark::plain_ug_fill(L, b->ug[0], b->ug[1], b->ug[2], pool, vl, depth));
+  }
   return SPARK_OK;
 }
 
@@ -244,9 +266,23 @@ int run_step_begin(spark_batch* b) {
   // SHOK over the cells stage 1 reads (the reference computes NGUARD-1
   // layers; the outer layers are never read before the next refill)
   const int band = fused_remap(b) ? 0 : (b->p.guard_mode == SPARK_GUARD_REFILL ? 0 : b->limits_array[0]) + 1;
-  for (auto& c : b->classes) {
-    PlainLaunch L = make_launch(b, c);
-    HIPCHK(spark::plain_shock_detect(L, b->U, b->flat, band));
+  if (b->amr_remap_count > 0) {
+    // remapping blocks: SHOK of the interior only (nothing reads theirs
+    // outside it); a hierarchy has one block shape
+    PlainLaunch L = make_launch(b, b->classes[0]);
+    if (b->n_tele > 0) {
+      L.blist = b->tele_ids_d;
+      L.nblk = b->n_tele;
+      HIPCHK(spark::plain_shock_detect(L, b->U, b->flat, band));
+    }
+    L.blist = b->remap_ids_d;
+    L.nblk = b->amr_remap_count;
+    HIPCHK(spark::plain_shock_detect(L, b->U, b->flat, 0));
+  } else {
+    for (auto& c : b->classes) {
+      PlainLaunch L = make_launch(b, c);
+      HIPCHK(spark::plain_shock_detect(L, b->U, b->flat, band));
+    }
   }
   b->step_open = true;
   return SPARK_OK;
@@ -268,6 +304,13 @@ double* stage_out(spark_batch* b, int stage) {
 
 int stage_ngoff(const spark_batch* b, int stage) {
   return b->p.guard_mode == SPARK_GUARD_REFILL ? 0 : b->limits_array[stage - 1];
+}
+
+// stage region of one block in the batched forms: blocks that read their
+// halos from same-level neighbours update only their interior
+int blk_ngoff(const spark_batch* b, int blk, int stage) {
+  if (b->amr_remap_count > 0 && b->amr_remap_blk[blk]) return 0;
+  return stage_ngoff(b, stage);
 }
 
 int fused_plan(spark_batch* b);
@@ -313,7 +356,7 @@ spark::FusedArgs fused_args(spark_batch* b, int stage) {
   A.Sin = stage_in(b, stage);
   A.Sout = stage_out(b, stage);
   for (int d = 0; d < 3; d++) A.fb[d] = b->p.flux_correct ? b->fb[d] : nullptr;
-  A.nbr = fused_remap(b) ? b->nbr_d : nullptr;
+  A.nbr = (fused_remap(b) || b->amr_remap_count > 0) ? b->nbr_d : nullptr;
   A.weight = b->weights[stage - 1];
   A.add_flux = stage > 1;
   A.dtp = b->dtp;
@@ -380,6 +423,8 @@ void free_batch(spark_batch* b) {
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->nbr_d) (void)hipFree(b->nbr_d);
+  if (b->remap_ids_d) (void)hipFree(b->remap_ids_d);
+  if (b->tele_ids_d) (void)hipFree(b->tele_ids_d);
   if (b->amr_map_d) (void)hipFree(b->amr_map_d);
   if (b->amr_org_d) (void)hipFree(b->amr_org_d);
   for (int d = 0; d < 3; d++)
@@ -465,8 +510,8 @@ void tile_items(const spark_batch* b, int blk, int ngoff, int tx, int ty, std::v
 
 // build the per-XCD queues of one stage from the chosen tiles.  A block's
 // items stay in one queue; blocks go to the least-loaded queue, largest first.
-int build_stage_plan(spark_batch* b, int stage, const std::vector<int>* only_blocks) {
-  const int ngoff = stage_ngoff(b, stage);
+// force_ngoff >= 0 (tuning): every block over that region.
+int build_stage_plan(spark_batch* b, int stage, const std::vector<int>* only_blocks, int force_ngoff) {
   std::vector<int> blocks;
   if (only_blocks) blocks = *only_blocks;
   else for (int i = 0; i < b->nblk; i++) blocks.push_back(i);
@@ -474,8 +519,11 @@ int build_stage_plan(spark_batch* b, int stage, const std::vector<int>* only_blo
   std::map<int, int> cls_of;
   for (size_t ci = 0; ci < b->classes.size(); ci++)
     for (int id : b->classes[ci].ids) cls_of[id] = (int)ci;
+  std::vector<int> ngoff_of(b->nblk, 0);
   for (int id : blocks) {
     const BlockMeta& m = b->meta_h[id];
+    const int ngoff = force_ngoff >= 0 ? force_ngoff : blk_ngoff(b, id, stage);
+    ngoff_of[id] = ngoff;
     cost.push_back({(double)(m.nx + 2 * ngoff) * (m.ny + 2 * ngoff) * (m.nz + 2 * ngoff + 2), id});
   }
   std::sort(cost.begin(), cost.end(), [](const std::pair<double, int>& a, const std::pair<double, int>& c) {
@@ -485,6 +533,7 @@ int build_stage_plan(spark_batch* b, int stage, const std::vector<int>* only_blo
   std::vector<double> load(spark::FUSED_XCDS, 0.);
   for (auto& pr : cost) {
     const int id = pr.second;
+    const int ngoff = ngoff_of[id];
     const auto key = std::make_pair(cls_of[id], ngoff);
     auto tt = b->tiles.find(key);
     if (tt == b->tiles.end()) return fail(SPARK_ERR_STATE, "no tile chosen for class %d ngoff %d", key.first, ngoff);
@@ -521,6 +570,7 @@ int fused_tune(spark_batch* b) {
   std::vector<int> ngoffs;
   for (int s = 1; s <= b->last_stage; s++)
     if (std::find(ngoffs.begin(), ngoffs.end(), stage_ngoff(b, s)) == ngoffs.end()) ngoffs.push_back(stage_ngoff(b, s));
+  if (b->amr_remap_count > 0 && std::find(ngoffs.begin(), ngoffs.end(), 0) == ngoffs.end()) ngoffs.push_back(0);
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -545,7 +595,7 @@ int fused_tune(spark_batch* b) {
       std::pair<int, int> pick = cands[0];
       for (auto& cand : cands) {
         b->tiles[key] = cand;
-        int rc = build_stage_plan(b, stage, &sample);
+        int rc = build_stage_plan(b, stage, &sample, ngoff);
         if (rc) return rc;
         spark::FusedArgs A = fused_args(b, stage);
         A.Sin = b->U;                     // read the packed solution,
@@ -577,7 +627,7 @@ int fused_plan(spark_batch* b) {
   int rc;
   if ((rc = fused_tune(b))) return rc;
   for (int s = 1; s <= b->last_stage; s++)
-    if ((rc = build_stage_plan(b, s, nullptr))) return rc;
+    if ((rc = build_stage_plan(b, s, nullptr, -1))) return rc;
   b->planned = true;
   return SPARK_OK;
 }
@@ -759,9 +809,10 @@ int spark_batch_create(const spark_params* p, int nblocks, const spark_block_des
   ALLOC(b->S, sizeof(double) * b->pool);
   ALLOC(b->flat, sizeof(double) * b->opool);
   for (int q = 0; q < 3; q++) ALLOC(b->fl[q], sizeof(double) * b->fpool);
-  if (p->flux_correct)
+  if (p->flux_correct) {
     for (int q = 0; q < 3; q++) ALLOC(b->fb[q], sizeof(double) * b->bpool);
     for (int q = 0; q < 3; q++) ALLOC(b->fc[q], sizeof(double) * b->bpool);
+  }
   ALLOC(b->dtp, sizeof(double));
   ALLOC(b->loc, sizeof(long long) * 2);
   b->hist_cap = 4096;
@@ -893,6 +944,7 @@ int spark_set_uniform_grid(spark_batch* b, int nbx, int nby, int nbz) {
   if (nbx < 1 || nby < 1 || nbz < 1 || (int64_t)nbx * nby * nbz != b->nblk)
     return fail(SPARK_ERR_ARG, "uniform grid %d x %d x %d does not match %d blocks", nbx, nby, nbz, b->nblk);
   if (b->classes.size() != 1) return fail(SPARK_ERR_ARG, "uniform grid needs blocks of one shape");
+  if (b->amr) return fail(SPARK_ERR_STATE, "spark_set_uniform_grid: the batch has a block hierarchy");
   if ((b->p.ndim < 2 && nby != 1) || (b->p.ndim < 3 && nbz != 1))
     return fail(SPARK_ERR_ARG, "uniform grid extent in an unused dimension");
   if (b->fused && b->p.guard_mode == SPARK_GUARD_REFILL) {
@@ -1150,6 +1202,7 @@ int spark_set_amr_grid(spark_batch* b, int nbx0, int nby0, int nbz0, const int* 
   if (b->p.guard_mode != SPARK_GUARD_TELESCOPE)
     return fail(SPARK_ERR_ARG, "spark_set_amr_grid: the block hierarchy is filled once per step (telescope mode)");
   if (nbx0 < 1 || nby0 < 1 || nbz0 < 1) return fail(SPARK_ERR_ARG, "spark_set_amr_grid: bad root grid");
+  if (b->has_ug) return fail(SPARK_ERR_STATE, "spark_set_amr_grid: the batch has a uniform grid");
   if (b->classes.size() != 1) return fail(SPARK_ERR_ARG, "spark_set_amr_grid: blocks of one shape only");
   const int n = b->meta_h[0].nx;
   if (b->meta_h[0].ny != n || b->meta_h[0].nz != n || (n & 1))
@@ -1213,6 +1266,63 @@ int spark_set_amr_grid(spark_batch* b, int nbx0, int nby0, int nbz0, const int* 
           if (L2 - L > 1 || L - L2 > 1)
             return fail(SPARK_ERR_ARG, "block %d (level %d): a leaf of level %d lies under its guard cells", i, L, L2);
         }
+  }
+  // batched forms: a block whose 26 neighbours (periodic wrap allowed) are
+  // all leaves of its own level gets their pool offsets; its guard region
+  // then holds nothing a stage needs that the neighbours' interiors do not,
+  // so it reads them there and every stage covers only its interior.  The
+  // stage values this gives are those of telescoping (the same cells from
+  // the same inputs); blocks at a level change or a non-periodic domain
+  // edge keep telescoping over their own guard cells.
+  b->amr_remap_blk.assign(b->nblk, 0);
+  b->amr_remap_count = 0;
+  if (b->fused && n >= b->p.nguard) {
+    std::vector<int64_t> nb((size_t)b->nblk * 27, -1);
+    for (int i = 0; i < b->nblk; i++) {
+      const int L = b->meta_h[i].level, sh = lmax - L;
+      int bp[3], nbl[3];
+      for (int d = 0; d < 3; d++) { bp[d] = origins[3 * i + d] / n; nbl[d] = nb0[d] << (L - 1); }
+      int64_t row[27];
+      bool ok = true;
+      for (int oz = -1; oz <= 1 && ok; oz++)
+        for (int oy = -1; oy <= 1 && ok; oy++)
+          for (int ox = -1; ox <= 1 && ok; ox++) {
+            const int o[3] = {ox, oy, oz};
+            int q[3];
+            for (int d = 0; d < 3 && ok; d++) {
+              q[d] = bp[d] + o[d];
+              if (q[d] < 0 || q[d] >= nbl[d]) {
+                if (!per[d]) ok = false;
+                q[d] = (q[d] + nbl[d]) % nbl[d];
+              }
+            }
+            if (!ok) break;
+            const int j = map[((size_t)q[0] << sh) + (size_t)md[0] * (((size_t)q[1] << sh) + (size_t)md[1] * ((size_t)q[2] << sh))];
+            if (b->meta_h[j].level != L) { ok = false; break; }
+            row[13 + ox + 3 * oy + 9 * oz] = b->meta_h[j].base;
+          }
+      if (!ok) continue;
+      std::copy(row, row + 27, nb.begin() + (size_t)i * 27);
+      b->amr_remap_blk[i] = 1;
+      b->amr_remap_count++;
+    }
+    if (b->amr_remap_count > 0) {
+      std::vector<int> rl, tl;
+      for (int i = 0; i < b->nblk; i++) (b->amr_remap_blk[i] ? rl : tl).push_back(i);
+      auto F = [](void* p) { if (p) (void)hipFree(p); };
+      F(b->nbr_d); F(b->remap_ids_d); F(b->tele_ids_d);
+      b->nbr_d = nullptr; b->remap_ids_d = nullptr; b->tele_ids_d = nullptr;
+      HIPCHK(hipMalloc((void**)&b->nbr_d, sizeof(int64_t) * nb.size()));
+      HIPCHK(hipMemcpy(b->nbr_d, nb.data(), sizeof(int64_t) * nb.size(), hipMemcpyHostToDevice));
+      HIPCHK(hipMalloc((void**)&b->remap_ids_d, sizeof(int) * rl.size()));
+      HIPCHK(hipMemcpy(b->remap_ids_d, rl.data(), sizeof(int) * rl.size(), hipMemcpyHostToDevice));
+      b->n_tele = (int)tl.size();
+      if (!tl.empty()) {
+        HIPCHK(hipMalloc((void**)&b->tele_ids_d, sizeof(int) * tl.size()));
+        HIPCHK(hipMemcpy(b->tele_ids_d, tl.data(), sizeof(int) * tl.size(), hipMemcpyHostToDevice));
+      }
+    }
+    b->planned = false;
   }
   if (!b->amr_map_d) HIPCHK(hipMalloc((void**)&b->amr_map_d, sizeof(int) * map.size()));
   else {

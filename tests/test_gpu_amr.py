@@ -89,6 +89,73 @@ def test_batched_fp64_amr_bit_exact_with_plain():
     assert all(np.array_equal(x, y) for x, y in zip(interiors(prob, o64), interiors(prob, ob)))
 
 
+def three_level(**kw):
+    # 1 x 1 x 2 level-1 blocks of 16^3, 3 levels: 32 of the 72 leaves (levels
+    # 2 and 3) have 26 leaves of their own level around them, so the batched
+    # forms read their halos from those neighbours and update only their
+    # interior every stage; the other 40 telescope over their guard cells
+    return problems.rt_amr(nb0=(1, 1, 2), n=16, lmax=3, **kw)
+
+
+def same_level_blocks(prob):
+    """Leaves whose 26 neighbours (periodic wrap, no other domain edge) are
+    leaves of their own level."""
+    info = prob.info
+    n = prob.blocks[0]["nx"]
+    lv = [b["level"] for b in prob.blocks]
+    lmax = max(lv)
+    org = np.asarray(info["origins"]).reshape(-1, 3)
+    md = [x << (lmax - 1) for x in info["nb0"]]
+    slot = -np.ones(md[::-1], dtype=int)
+    for i, (o, L) in enumerate(zip(org, lv)):
+        sh = lmax - L
+        lo = [(int(o[d]) // n) << sh for d in range(3)]
+        slot[lo[2]:lo[2] + (1 << sh), lo[1]:lo[1] + (1 << sh), lo[0]:lo[0] + (1 << sh)] = i
+    per = [b == problems.BC_PERIODIC for b in info["bc"]]
+    out = 0
+    for o, L in zip(org, lv):
+        sh = lmax - L
+        nbl = [info["nb0"][d] << (L - 1) for d in range(3)]
+        ok = True
+        for off in np.ndindex(3, 3, 3):
+            q = [int(o[d]) // n + off[d] - 1 for d in range(3)]
+            for d in range(3):
+                if not 0 <= q[d] < nbl[d]:
+                    ok = ok and per[d]
+                    q[d] %= nbl[d]
+            ok = ok and lv[slot[q[2] << sh, q[1] << sh, q[0] << sh]] == L
+        out += ok
+    return out
+
+
+def test_batched_fp64_amr_same_level_halos_bit_exact_with_plain():
+    prob = three_level(nscalars=8)
+    nsame = same_level_blocks(prob)
+    print(f"RT 3 levels {prob.info['leaves_per_level']}: {nsame} blocks read same-level halos", flush=True)
+    assert 0 < nsame < len(prob.blocks)
+    d64, o64, _ = run(prob, 3, "plain_fp64")
+    db, ob, cnt = run(prob, 3, "batched_fp64")
+    assert cnt[0] == 0
+    assert np.array_equal(d64, db)
+    nd = sum(int(np.count_nonzero(x != y)) for x, y in zip(interiors(prob, o64), interiors(prob, ob)))
+    print(f"differing interior values {nd}", flush=True)
+    assert nd == 0
+
+
+def test_fp32_flux_amr_same_level_halos_vs_plain():
+    prob = three_level(nscalars=8)
+    nsteps = 10
+    d64, o64, _ = run(prob, nsteps, "plain_fp64")
+    d32, o32, cnt = run(prob, nsteps, "fp32_flux")
+    vols = [b["dx"] ** 3 for b in prob.blocks]
+    gi, ri = interiors(prob, o32), interiors(prob, o64)
+    num = sum(np.sum(np.abs(x[..., 0] - y[..., 0])) * v for x, y, v in zip(gi, ri, vols))
+    den = sum(np.sum(np.abs(y[..., 0])) * v for y, v in zip(ri, vols))
+    print(f"L1(rho) {num / den:.2e}, dt rel diff {np.max(np.abs(d32 - d64) / d64):.2e}", flush=True)
+    assert cnt[0] == 0
+    assert num / den <= 1e-5
+
+
 def test_fp32_flux_amr_vs_oracle():
     prob = two_level(nscalars=8)
     nsteps = 5
