@@ -234,7 +234,9 @@ def main():
     b.profile(False)
     b.close()
 
-    forms = {"plain_fp64": {"zone_updates_per_s": round(value, 1), "ms_per_step": round(el / args.steps * 1e3, 4)}}
+    forms = {"plain_fp64": {"zone_updates_per_s": round(value, 1), "ms_per_step": round(el / args.steps * 1e3, 4),
+                            "kernel_ms_per_step": {x["name"]: round(x["ms"] / args.profile_steps, 4)
+                                                   for x in sorted(stats, key=lambda x: -x["ms"])}}}
     for form in [f for f in args.forms.split(",") if f]:
         bf, elf = timed(form)
         bf.profile(True)
@@ -244,10 +246,12 @@ def main():
         tiles = bf.tiles()
         bf.close()
         fk = [x for x in st if x["name"] == "k_fused_stage"]
+        entry_k = {x["name"]: round(x["ms"] / args.profile_steps, 4) for x in sorted(st, key=lambda x: -x["ms"])}
         entry = {"zone_updates_per_s": round(prob.zones * args.steps / elf, 1),
                  "ms_per_step": round(elf / args.steps * 1e3, 4),
                  "speedup_vs_plain_fp64": round(el / elf, 3),
-                 "tiles": [list(t) for t in tiles]}
+                 "tiles": [list(t) for t in tiles],
+                 "kernel_ms_per_step": entry_k}
         if fk:
             k = fk[0]
             entry["fused_stage_avg_ms"] = round(k["ms"] / k["launches"], 4)

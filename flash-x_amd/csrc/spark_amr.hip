@@ -48,45 +48,49 @@ __device__ __forceinline__ double flipped(double x, int flip, int v) {
 // where the value of a level-L cell comes from: the containing cell of a
 // same-level or coarser leaf, or the 8 cells of a finer leaf (averaged)
 struct Src {
-  int64_t off, sy, sz, cs;
+  int64_t off;
   bool avg;
+};
+
+// every block of a hierarchy has the same shape, so the strides of a
+// source block are those of the block being filled
+struct Shape {
+  int64_t sy, sz, cs;
 };
 
 __device__ __forceinline__ Src amr_src(const BlockMeta* __restrict__ meta, const AmrGrid& G, int g, int L,
                                        const int gi[3]) {
   const int s = leaf_at(G, L, gi);
-  const BlockMeta ms = meta[s];
-  Src r;
-  r.sy = ms.NX;
-  r.sz = (int64_t)ms.NX * ms.NX;
-  r.cs = ms.cstride;
+  const BlockMeta& ms = meta[s];
+  const int lv = ms.level, NX = ms.NX;
   int c[3];
-  const bool coarse = ms.level <= L;
+  const bool coarse = lv <= L;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     const int o = G.org[3 * s + d];
-    c[d] = coarse ? (gi[d] >> (L - ms.level)) - o + g : 2 * gi[d] - o + g;
+    c[d] = coarse ? (gi[d] >> (L - lv)) - o + g : 2 * gi[d] - o + g;
     // a leaf more than one level finer is excluded when the hierarchy is
     // declared; keep the read inside the block regardless
-    c[d] = c[d] < 0 ? 0 : (c[d] > ms.NX - 2 ? ms.NX - 2 : c[d]);
+    c[d] = c[d] < 0 ? 0 : (c[d] > NX - 2 ? NX - 2 : c[d]);
   }
+  Src r;
   r.avg = !coarse;
-  r.off = ms.base + acix(ms.NX, c[0], c[1], c[2]);
+  r.off = ms.base + acix(NX, c[0], c[1], c[2]);
   return r;
 }
 
-__device__ __forceinline__ double src_val(const double* __restrict__ U, const Src& r, int v) {
-  const double* p = U + r.off + v * r.cs;
+__device__ __forceinline__ double src_val(const double* __restrict__ U, const Src& r, const Shape& S, int v) {
+  const double* p = U + r.off + v * S.cs;
   if (!r.avg) return p[0];
   double sum = 0.0;
   sum = sum + p[0];
   sum = sum + p[1];
-  sum = sum + p[r.sy];
-  sum = sum + p[1 + r.sy];
-  sum = sum + p[r.sz];
-  sum = sum + p[1 + r.sz];
-  sum = sum + p[r.sy + r.sz];
-  sum = sum + p[1 + r.sy + r.sz];
+  sum = sum + p[S.sy];
+  sum = sum + p[1 + S.sy];
+  sum = sum + p[S.sz];
+  sum = sum + p[1 + S.sz];
+  sum = sum + p[S.sy + S.sz];
+  sum = sum + p[1 + S.sy + S.sz];
   return 0.125 * sum;
 }
 
@@ -117,6 +121,7 @@ __global__ void __launch_bounds__(256) k_amr_fill(DevParams P, const BlockMeta* 
   const int ext = n + 2 * depth;
   const int total = ext * ext * ext;
   const int o[3] = {G.org[3 * bid], G.org[3 * bid + 1], G.org[3 * bid + 2]};
+  const Shape S{m.NX, (int64_t)m.NX * m.NX, m.cstride};
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
     const int loc[3] = {g - depth + t % ext, g - depth + (t / ext) % ext, g - depth + t / (ext * ext)};
     bool inside = true;
@@ -131,8 +136,20 @@ __global__ void __launch_bounds__(256) k_amr_fill(DevParams P, const BlockMeta* 
     const int64_t dst = m.base + acix(m.NX, loc[0], loc[1], loc[2]);
     const int s = leaf_at(G, L, gi);
     if (meta[s].level != L - 1) {
+      // copy / restriction: the loads of a chunk of variables are issued
+      // before its stores (a store to U could alias a later load, so the
+      // compiler keeps them in program order)
       const Src r = amr_src(meta, G, g, L, gi);
-      for (int q = 0; q < vars.n; q++) U[dst + vars.v[q] * m.cstride] = flipped(src_val(U, r, vars.v[q]), flip, vars.v[q]);
+      constexpr int CHV = 8;
+      for (int q0 = 0; q0 < vars.n; q0 += CHV) {
+        double val[CHV];
+#pragma unroll
+        for (int u = 0; u < CHV; u++)
+          if (q0 + u < vars.n) val[u] = flipped(src_val(U, r, S, vars.v[q0 + u]), flip, vars.v[q0 + u]);
+#pragma unroll
+        for (int u = 0; u < CHV; u++)
+          if (q0 + u < vars.n) U[dst + vars.v[q0 + u] * m.cstride] = val[u];
+      }
       continue;
     }
     int gc[3] = {gi[0] >> 1, gi[1] >> 1, gi[2] >> 1};
@@ -151,12 +168,12 @@ __global__ void __launch_bounds__(256) k_amr_fill(DevParams P, const BlockMeta* 
     }
     for (int q = 0; q < vars.n; q++) {
       const int v = vars.v[q];
-      const double c = src_val(U, c0, v);
+      const double c = src_val(U, c0, S, v);
       double val = c;
 #pragma unroll
       for (int d = 0; d < 3; d++) {
-        const double cm = flipped(src_val(U, sm[d], v), fm[d], v);
-        const double cp = flipped(src_val(U, sp[d], v), fp[d], v);
+        const double cm = flipped(src_val(U, sm[d], S, v), fm[d], v);
+        const double cp = flipped(src_val(U, sp[d], S, v), fp[d], v);
         const double sl = mc_slope(c - cm, cp - c);
         val = val + sl * ((gi[d] & 1) ? 0.25 : -0.25);
       }
@@ -248,12 +265,16 @@ hipError_t amr_fill(const PlainLaunch& L, const AmrGrid& G, double* U, const Var
 }
 
 hipError_t amr_flux_exchange(const PlainLaunch& L, const AmrGrid& G, double* const fb[3], double* const fc[3]) {
+  // algorithmic bytes: read own + up to 4 fine faces, write the correction
+  const double faces = 2.0 * G.n * G.n * L.nblk * 3, nf = 5.0 + L.P.ns;
+  if (L.prof) L.prof->begin(L.stream, "k_flux_exchange", 8.0 * faces * nf * 3, faces * nf * 4);
   for (int dir = 0; dir < 3; dir++) {
     hipLaunchKernelGGL(k_flux_exchange, dim3(grid_x(2 * G.n * G.n, 64), L.nblk), dim3(256), 0, L.stream, L.P, L.meta,
                        G, fb[dir], fc[dir], dir);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
+  if (L.prof) L.prof->end(L.stream);
   return hipSuccess;
 }
 

@@ -106,6 +106,11 @@ struct spark_batch {
   int* remap_ids_d = nullptr;  // the remapping blocks
   int* tele_ids_d = nullptr;   // the telescoping blocks
   int n_tele = 0;
+  // per block the faces (bit 2d+s) with a finer leaf across, and the blocks
+  // that have one (the FP32-flux form corrects only the cells on them)
+  unsigned char* amr_fmask_d = nullptr;
+  int* corr_ids_d = nullptr;
+  int n_corr = 0;
   spark::AmrGrid amr_g{};
   int* amr_map_d = nullptr;
   int* amr_org_d = nullptr;
@@ -223,15 +228,19 @@ int run_fill(spark_batch* b, double* pool, bool full, int depth) {
   }
   if (b->amr && !full && b->amr_remap_count > 0) {
     // remapping blocks read no guard cell of their own in the stages: fill
-    // only the two layers shockDetect reads
+    // only what shockDetect of their interior reads
     if (b->n_tele > 0) {
       L.blist = b->tele_ids_d;
       L.nblk = b->n_tele;
       HIPCHK(spark::amr_fill(L, b->amr_g, pool, vl, depth));
     }
+    spark::VarList vs;  // shockDetect: dens, gamc, pres and the velocities one layer out
+    vs.n = 0;
+    for (int v : {spark::V_DENS, spark::V_GAMC, spark::V_PRES, spark::V_VELX, spark::V_VELY, spark::V_VELZ})
+      vs.v[vs.n++] = v;
     L.blist = b->remap_ids_d;
     L.nblk = b->amr_remap_count;
-    HIPCHK(spark::amr_fill(L, b->amr_g, pool, vl, std::min(depth, 2)));
+    HIPCHK(spark::amr_fill(L, b->amr_g, pool, vs, std::min(depth, 1)));
   } else if (b->amr) {
     HIPCHK(spark::amr_fill(L, b->amr_g, pool, vl, depth));
   } else {
@@ -247,6 +256,21 @@ int run_amr_correct(spark_batch* b) {
   L.blist = nullptr;
   L.nblk = b->nblk;
   HIPCHK(spark::amr_flux_exchange(L, b->amr_g, b->fb, b->fc));
+  if (b->fused && !b->fused64) {
+    // FP32-flux form: only the cells on a face toward a finer leaf (their
+    // whole x, y, z sequence).  Everywhere else the correction is zero and
+    // the reference's pass is the identity up to one FP64 rounding of the
+    // primitive -> conserved -> primitive round trip, which the FP64 forms
+    // reproduce and this form leaves out.
+    if (b->n_corr > 0) {
+      PlainLaunch Lc = make_launch(b, b->classes[0]);
+      Lc.blist = b->corr_ids_d;
+      Lc.nblk = b->n_corr;
+      for (int d = 0; d < 3; d++) Lc.fb[d] = b->fc[d];
+      HIPCHK(spark::plain_correct(Lc, b->U, b->amr_fmask_d));
+    }
+    return SPARK_OK;
+  }
   for (auto& c : b->classes) {
     PlainLaunch Lc = make_launch(b, c);
     for (int d = 0; d < 3; d++) Lc.fb[d] = b->fc[d];
@@ -265,7 +289,9 @@ int stage1_reach(const spark_batch* b) {
 int run_step_begin(spark_batch* b) {
   // SHOK over the cells stage 1 reads (the reference computes NGUARD-1
   // layers; the outer layers are never read before the next refill)
-  const int band = fused_remap(b) ? 0 : (b->p.guard_mode == SPARK_GUARD_REFILL ? 0 : b->limits_array[0]) + 1;
+  // (the batched forms have no hybrid Riemann solver and read SHOK nowhere:
+  // they compute it for the interior, where it is part of the solution)
+  const int band = b->fused ? 0 : (b->p.guard_mode == SPARK_GUARD_REFILL ? 0 : b->limits_array[0]) + 1;
   if (b->amr_remap_count > 0) {
     // remapping blocks: SHOK of the interior only (nothing reads theirs
     // outside it); a hierarchy has one block shape
@@ -425,6 +451,8 @@ void free_batch(spark_batch* b) {
   if (b->nbr_d) (void)hipFree(b->nbr_d);
   if (b->remap_ids_d) (void)hipFree(b->remap_ids_d);
   if (b->tele_ids_d) (void)hipFree(b->tele_ids_d);
+  if (b->amr_fmask_d) (void)hipFree(b->amr_fmask_d);
+  if (b->corr_ids_d) (void)hipFree(b->corr_ids_d);
   if (b->amr_map_d) (void)hipFree(b->amr_map_d);
   if (b->amr_org_d) (void)hipFree(b->amr_org_d);
   for (int d = 0; d < 3; d++)
@@ -1323,6 +1351,42 @@ int spark_set_amr_grid(spark_batch* b, int nbx0, int nby0, int nbz0, const int* 
       }
     }
     b->planned = false;
+  }
+  // faces with a finer leaf across (the region across a face is one block
+  // position of the block's level: one leaf, or finer leaves only)
+  {
+    std::vector<unsigned char> fm(b->nblk, 0);
+    std::vector<int> cl;
+    for (int i = 0; i < b->nblk; i++) {
+      const int L = b->meta_h[i].level, sh = lmax - L;
+      for (int d = 0; d < 3; d++)
+        for (int sd = 0; sd < 2; sd++) {
+          int q[3];
+          bool in = true;
+          for (int e = 0; e < 3; e++) {
+            const int nbl = nb0[e] << (L - 1);
+            q[e] = origins[3 * i + e] / n + (e == d ? (sd ? 1 : -1) : 0);
+            if (q[e] < 0 || q[e] >= nbl) {
+              if (per[e]) q[e] = (q[e] + nbl) % nbl;
+              else in = false;
+            }
+          }
+          if (!in) continue;
+          const int j = map[((size_t)q[0] << sh) + (size_t)md[0] * (((size_t)q[1] << sh) + (size_t)md[1] * ((size_t)q[2] << sh))];
+          if (b->meta_h[j].level > L) fm[i] |= (unsigned char)(1 << (2 * d + sd));
+        }
+      if (fm[i]) cl.push_back(i);
+    }
+    auto F = [](void* p) { if (p) (void)hipFree(p); };
+    F(b->amr_fmask_d); F(b->corr_ids_d);
+    b->amr_fmask_d = nullptr; b->corr_ids_d = nullptr;
+    HIPCHK(hipMalloc((void**)&b->amr_fmask_d, b->nblk));
+    HIPCHK(hipMemcpy(b->amr_fmask_d, fm.data(), b->nblk, hipMemcpyHostToDevice));
+    b->n_corr = (int)cl.size();
+    if (!cl.empty()) {
+      HIPCHK(hipMalloc((void**)&b->corr_ids_d, sizeof(int) * cl.size()));
+      HIPCHK(hipMemcpy(b->corr_ids_d, cl.data(), sizeof(int) * cl.size(), hipMemcpyHostToDevice));
+    }
   }
   if (!b->amr_map_d) HIPCHK(hipMalloc((void**)&b->amr_map_d, sizeof(int) * map.size()));
   else {

@@ -73,7 +73,7 @@ struct VarList {
   int v[24];
 };
 hipError_t plain_ug_fill(const PlainLaunch& L, int nbx, int nby, int nbz, double* U, const VarList& vars, int depth);
-hipError_t plain_correct(const PlainLaunch& L, double* U);
+hipError_t plain_correct(const PlainLaunch& L, double* U, const unsigned char* fmask = nullptr);
 
 // Block hierarchy (leaves of n^3 cells over a level-1 grid of nb0 blocks)
 struct AmrGrid {

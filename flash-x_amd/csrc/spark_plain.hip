@@ -517,67 +517,92 @@ __global__ void __launch_bounds__(256) k_add_fluxes(DevParams P, const BlockMeta
 }
 
 // ---------------------------------------------------------------------------
-// flux correction on the boundary layer of one direction
-// (hy_rk_correctFluxes.F90:126-338): correctZone + per-cell EOS.
+// flux correction of every block's boundary cells (correctZone + per-cell
+// EOS, hy_rk_correctFluxes.F90:126-338).  The reference sweeps the low and
+// high layer of x, then of y, then of z, each sweep rebuilding the
+// conserved state from the primitives the previous one wrote.  Here a
+// thread owns one boundary cell and runs its x, y, z corrections in that
+// order with the primitives carried in registers (a store and reload of a
+// double is exact), so every cell gets the values of the three sweeps and
+// the state is read and written once.  Lanes map to consecutive x cells.
+// With a face mask (bit 2d+s: low / high face of dimension d), only cells
+// on a masked face are corrected (each through all of its faces).
 __global__ void __launch_bounds__(256) k_correct(DevParams P, const BlockMeta* __restrict__ meta,
                                                  const int* __restrict__ blist, double* __restrict__ U,
-                                                 const double* __restrict__ fb, int dir,
-                                                 const double* __restrict__ dtp) {
-  const BlockMeta m = meta[blist[blockIdx.y]];
+                                                 const double* __restrict__ fbx, const double* __restrict__ fby,
+                                                 const double* __restrict__ fbz, const double* __restrict__ dtp,
+                                                 const unsigned char* __restrict__ fmask) {
+  const int bid = blist[blockIdx.y];
+  const BlockMeta m = meta[bid];
   Lims L = stage_lims(P, m, 0);
   const int n[3] = {L.hi[0] - L.lo[0] + 1, L.hi[1] - L.lo[1] + 1, L.hi[2] - L.lo[2] + 1};
-  int e[3] = {n[0], n[1], n[2]};
-  e[dir] += 1;
-  const int64_t bstride = (int64_t)e[0] * e[1] * e[2];
-  const int d2 = (dir == 0) ? 1 : 0, d3 = (dir == 2) ? 1 : 2;
-  const int total = n[d2] * n[d3];
-  const int nf = F_NUM + P.ns, ns = P.ns;
+  const int total = n[0] * n[1] * n[2];
+  const int nf = F_NUM + P.ns, ns = P.ns, nd = P.ndim;
+  const int mask = fmask ? fmask[bid] : 0x3f;
   const double dt = *dtp;
-  const double del = (dir == 0) ? m.dx : (dir == 1) ? m.dy : m.dz;
   const int64_t cs = m.cstride;
   double* up = U + m.base;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-    const int a2 = t % n[d2], a3 = t / n[d2];
-    for (int side = 0; side < 2; side++) {
-      int c[3];
-      c[d2] = a2; c[d3] = a3;
-      c[dir] = side ? n[dir] - 1 : 0;
-      int f[3] = {c[0], c[1], c[2]};
-      if (side) f[dir] += 1;
-      const int64_t bo = m.bbase + (int64_t)f[0] + (int64_t)e[0] * ((int64_t)f[1] + (int64_t)e[1] * f[2]);
-      const int64_t cc = cix(m, L.lo[0] + c[0], L.lo[1] + c[1], L.lo[2] + c[2]);
-      double Us[F_NUM + 8];
-      const double d0 = up[V_DENS * cs + cc];
-      Us[F_MASS] = d0;
-      Us[F_XMOM] = d0 * up[V_VELX * cs + cc];
-      Us[F_YMOM] = d0 * up[V_VELY * cs + cc];
-      Us[F_ZMOM] = d0 * up[V_VELZ * cs + cc];
-      Us[F_ENER] = d0 * up[V_ENER * cs + cc];
-      for (int v = 0; v < ns; v++) Us[F_NUM + v] = up[(V_NPROP + v) * cs + cc] * d0;
-      const double fac = 1.0;
-      for (int q = 0; q < nf; q++) {
-        const double b = fb[bo + q * bstride];
-        const double dF = side ? b : -b;
-        Us[q] = Us[q] - dt / del * (fac * dF);
+    const int c[3] = {t % n[0], (t / n[0]) % n[1], t / (n[0] * n[1])};
+    bool edge = false;
+    for (int d = 0; d < nd; d++)
+      edge = edge || (c[d] == 0 && (mask >> (2 * d) & 1)) || (c[d] == n[d] - 1 && (mask >> (2 * d + 1) & 1));
+    if (!edge) continue;
+    const int64_t cc = cix(m, L.lo[0] + c[0], L.lo[1] + c[1], L.lo[2] + c[2]);
+    double dens = up[V_DENS * cs + cc];
+    double vx = up[V_VELX * cs + cc], vy = up[V_VELY * cs + cc], vz = up[V_VELZ * cs + cc];
+    double ener = up[V_ENER * cs + cc];
+    double X[8];  // at most 8 mass scalars (the plain kernels instantiate NSM 0..8)
+    for (int v = 0; v < ns; v++) X[v] = up[(V_NPROP + v) * cs + cc];
+    EosOut o;
+    for (int d = 0; d < nd; d++) {
+      const double* fb = (d == 0) ? fbx : (d == 1) ? fby : fbz;
+      int e[3] = {n[0], n[1], n[2]};
+      e[d] += 1;
+      const int64_t bstride = (int64_t)e[0] * e[1] * e[2];
+      const double del = (d == 0) ? m.dx : (d == 1) ? m.dy : m.dz;
+      for (int side = 0; side < 2; side++) {
+        if (c[d] != (side ? n[d] - 1 : 0)) continue;
+        int f[3] = {c[0], c[1], c[2]};
+        if (side) f[d] += 1;
+        const int64_t bo = m.bbase + (int64_t)f[0] + (int64_t)e[0] * ((int64_t)f[1] + (int64_t)e[1] * f[2]);
+        double Us[F_NUM + 8];
+        const double d0 = dens;
+        Us[F_MASS] = d0;
+        Us[F_XMOM] = d0 * vx;
+        Us[F_YMOM] = d0 * vy;
+        Us[F_ZMOM] = d0 * vz;
+        Us[F_ENER] = d0 * ener;
+        for (int v = 0; v < ns; v++) Us[F_NUM + v] = X[v] * d0;
+        const double fac = 1.0;
+        for (int q = 0; q < nf; q++) {
+          const double b = fb[bo + q * bstride];
+          const double dF = side ? b : -b;
+          Us[q] = Us[q] - dt / del * (fac * dF);
+        }
+        dens = fmax2(Us[F_MASS], P.smlrho);
+        vx = Us[F_XMOM] / dens;
+        vy = Us[F_YMOM] / dens;
+        vz = Us[F_ZMOM] / dens;
+        const double en = fmax2(P.smalle, Us[F_ENER] / dens);
+        const double ekin = .5 * (vx * vx + vy * vy + vz * vz);
+        const double eint = fmax2(P.smalle, en - ekin);
+        for (int v = 0; v < ns; v++) X[v] = Us[F_NUM + v] / dens;
+        o = eos_dens_ei(P, dens, vx, vy, vz, en, eint);
+        ener = o.ener;
       }
-      const double dens = fmax2(Us[F_MASS], P.smlrho);
-      const double vx = Us[F_XMOM] / dens, vy = Us[F_YMOM] / dens, vz = Us[F_ZMOM] / dens;
-      const double ener = fmax2(P.smalle, Us[F_ENER] / dens);
-      const double ekin = .5 * (vx * vx + vy * vy + vz * vz);
-      const double eint = fmax2(P.smalle, ener - ekin);
-      for (int v = 0; v < ns; v++) up[(V_NPROP + v) * cs + cc] = Us[F_NUM + v] / dens;
-      EosOut o = eos_dens_ei(P, dens, vx, vy, vz, ener, eint);
-      up[V_DENS * cs + cc] = dens;
-      up[V_VELX * cs + cc] = vx;
-      up[V_VELY * cs + cc] = vy;
-      up[V_VELZ * cs + cc] = vz;
-      up[V_PRES * cs + cc] = o.pres;
-      up[V_GAMC * cs + cc] = P.gamma;
-      up[V_EINT * cs + cc] = o.eint;
-      up[V_ENER * cs + cc] = o.ener;
-      up[V_TEMP * cs + cc] = o.temp;
-      up[V_GAME * cs + cc] = o.game;
     }
+    for (int v = 0; v < ns; v++) up[(V_NPROP + v) * cs + cc] = X[v];
+    up[V_DENS * cs + cc] = dens;
+    up[V_VELX * cs + cc] = vx;
+    up[V_VELY * cs + cc] = vy;
+    up[V_VELZ * cs + cc] = vz;
+    up[V_PRES * cs + cc] = o.pres;
+    up[V_GAMC * cs + cc] = P.gamma;
+    up[V_EINT * cs + cc] = o.eint;
+    up[V_ENER * cs + cc] = o.ener;
+    up[V_TEMP * cs + cc] = o.temp;
+    up[V_GAME * cs + cc] = o.game;
   }
 }
 
@@ -811,16 +836,18 @@ hipError_t plain_ug_fill(const PlainLaunch& L, int nbx, int nby, int nbz, double
   return hipGetLastError();
 }
 
-hipError_t plain_correct(const PlainLaunch& L, double* U) {
-  for (int dir = 0; dir < L.P.ndim; dir++) {
-    const int n[3] = {L.nx, L.P.ndim > 1 ? L.ny : 1, L.P.ndim > 2 ? L.nz : 1};
-    const int d2 = (dir == 0) ? 1 : 0, d3 = (dir == 2) ? 1 : 2;
-    hipLaunchKernelGGL(k_correct, dim3(grid_x_for(n[d2] * n[d3], 64), L.nblk), dim3(256), 0, L.stream, L.P, L.meta,
-                       L.blist, U, L.fb[dir], dir, L.dtp);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  return hipSuccess;
+hipError_t plain_correct(const PlainLaunch& L, double* U, const unsigned char* fmask) {
+  const int total = L.nx * (L.P.ndim > 1 ? L.ny : 1) * (L.P.ndim > 2 ? L.nz : 1);
+  int inner = 1;
+  const int n[3] = {L.nx, L.ny, L.nz};
+  for (int d = 0; d < 3; d++) inner *= (d < L.P.ndim && n[d] > 2) ? n[d] - 2 : (d < L.P.ndim ? 0 : 1);
+  // boundary cells: read 5+ns state fields and the registers, write 10+ns
+  const double cells = (double)(total - inner) * L.nblk, nf = 5.0 + L.P.ns;
+  if (L.prof) L.prof->begin(L.stream, "k_correct", 8.0 * cells * (nf + nf + 10.0 + L.P.ns), cells * (nf * 4 + 40.0));
+  hipLaunchKernelGGL(k_correct, dim3(grid_x_for(total, 64), L.nblk), dim3(256), 0, L.stream, L.P, L.meta, L.blist, U,
+                     L.fb[0], L.fb[1], L.fb[2], L.dtp, fmask);
+  if (L.prof) L.prof->end(L.stream);
+  return hipGetLastError();
 }
 
 hipError_t plain_aos_to_soa(hipStream_t st, const BlockMeta* meta, int nblk, int maxcells, const double* aos,
