@@ -129,7 +129,12 @@ int spark_set_uniform_grid(spark_batch* b, int nbx, int nby, int nbz);
  * Enables the device guard fill over levels (same-level copy, the mean of 8
  * cells of a finer leaf, limited linear prolongation from a coarser leaf)
  * and, with flux_correct, the device flux-register exchange after every
- * step.
+ * step.  In the batched forms a leaf whose 26 neighbours are leaves of its
+ * own level reads its halo from them and updates only its interior in every
+ * stage (the values telescoping would give); with flux_correct the
+ * FP32-flux form corrects only the cells on faces toward a finer leaf (the
+ * correction is zero elsewhere; the FP64 forms still pass those cells
+ * through the reference's conserved/primitive round trip).
  * Replaces: Grid_fillGuardCells over PARAMESH leaves,
  * Grid/GridMain/AMR/Paramesh4/Grid_fillGuardCells.F90:436-546, and the
  * flux hand-off of Spark/Hydro.F90-mc:197-236. */
