@@ -241,8 +241,9 @@ __device__ __forceinline__ void xy_face(const FusedArgs& A, const BlockMeta& m, 
   if (A.fb[0] != nullptr) {
     const int i = I.x0 + fi, j = I.y0 + fj;
     const bool own = DIR == 0 ? (fi < tx || i == I.xhi + 1) : (fj < ty || j == I.yhi + 1);
-    const bool in = DIR == 0 ? (i >= g && i <= g + m.nx && j >= g && j < g + m.ny)
-                             : (i >= g && i < g + m.nx && j >= g && j <= g + m.ny);
+    // the block's boundary faces only (the register hand-off reads no other)
+    const bool in = DIR == 0 ? ((i == g || i == g + m.nx) && j >= g && j < g + m.ny)
+                             : (i >= g && i < g + m.nx && (j == g || j == g + m.ny));
     if (own && in && k >= g && k < g + m.nz) {
       const int ex = m.nx + (DIR == 0), ey = m.ny + (DIR == 1);
       const int64_t tot = (int64_t)ex * ey * m.nz;
@@ -497,7 +498,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
               cl[0] = win[0][1]; cl[1] = win[1][1]; cl[2] = win[2][1]; cl[3] = win[3][1]; cl[4] = win[SL_RHOE][1];
               cr[0] = win[0][2]; cr[1] = win[1][2]; cr[2] = win[2][2]; cr[3] = win[3][2]; cr[4] = win[SL_RHOE][2];
               hydro_face<2, RIEMANN>(rp, Ls, Rs, cl, cr, Hz, A.err);
-              if (regs && colin && k >= g && k <= g + m.nz) {
+              if (regs && colin && (k == g || k == g + m.nz)) {
                 const int64_t tot = (int64_t)m.nx * m.ny * (m.nz + 1);
                 const int64_t o = m.bbase + (ci - g) + (int64_t)m.nx * ((cj - g) + (int64_t)m.ny * (k - g));
 #pragma unroll
@@ -653,7 +654,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
                 else Hs[j] = ums[j] * Hz[F_MASS];
               }
             }
-            if (regs && colin && k >= g && k <= g + m.nz) {
+            if (regs && colin && (k == g || k == g + m.nz)) {
               const int64_t tot = (int64_t)m.nx * m.ny * (m.nz + 1);
               const int64_t o = m.bbase + (ci - g) + (int64_t)m.nx * ((cj - g) + (int64_t)m.ny * (k - g));
 #pragma unroll
@@ -686,8 +687,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
               if (regs) {
                 const int i = I.x0 + fi, jj = I.y0 + fj;
                 if (isx) {
-                  if ((fi < tx || i == I.xhi + 1) && i >= g && i <= g + m.nx && jj >= g && jj < g + m.ny && k >= g &&
-                      k < g + m.nz) {
+                  if ((fi < tx || i == I.xhi + 1) && (i == g || i == g + m.nx) && jj >= g && jj < g + m.ny &&
+                      k >= g && k < g + m.nz) {
                     const int ex = m.nx + 1, ey = m.ny;
                     const int64_t tot = (int64_t)ex * ey * m.nz;
                     const int64_t o = m.bbase + (i - g) + (int64_t)ex * ((jj - g) + (int64_t)ey * (k - g));
@@ -695,8 +696,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
                       reg_add(A.fb[0], o, tot, F_NUM + c0 + j, A.weight, A.add_flux, fo[j * nf + w]);
                   }
                 } else {
-                  if ((fj < ty || jj == I.yhi + 1) && i >= g && i < g + m.nx && jj >= g && jj <= g + m.ny && k >= g &&
-                      k < g + m.nz) {
+                  if ((fj < ty || jj == I.yhi + 1) && i >= g && i < g + m.nx && (jj == g || jj == g + m.ny) &&
+                      k >= g && k < g + m.nz) {
                     const int ex = m.nx, ey = m.ny + 1;
                     const int64_t tot = (int64_t)ex * ey * m.nz;
                     const int64_t o = m.bbase + (i - g) + (int64_t)ex * ((jj - g) + (int64_t)ey * (k - g));

@@ -840,6 +840,14 @@ int spark_batch_create(const spark_params* p, int nblocks, const spark_block_des
   if (p->flux_correct) {
     for (int q = 0; q < 3; q++) ALLOC(b->fb[q], sizeof(double) * b->bpool);
     for (int q = 0; q < 3; q++) ALLOC(b->fc[q], sizeof(double) * b->bpool);
+    // only boundary faces are ever accumulated: the interior ones read 0
+    for (int q = 0; q < 3; q++) {
+      hipError_t e_ = hipMemset(b->fb[q], 0, sizeof(double) * b->bpool);
+      if (e_ != hipSuccess) {
+        free_batch(b);
+        return fail(SPARK_ERR_HIP, "hipMemset: %s", hipGetErrorString(e_));
+      }
+    }
   }
   ALLOC(b->dtp, sizeof(double));
   ALLOC(b->loc, sizeof(long long) * 2);

@@ -493,25 +493,35 @@ __global__ void __launch_bounds__(256) k_ug_fill(DevParams P, const BlockMeta* _
 }
 
 // ---------------------------------------------------------------------------
-// flux-register accumulation (addFluxes, Hydro_funcs.F90-mc:481-507):
-// buffers hold the interior faces of each block, dims (n+1 along dir).
+// flux-register accumulation (addFluxes, Hydro_funcs.F90-mc:481-507) on the
+// block's two boundary faces of direction dir.  The registers are laid out
+// over all faces of the block (n+1 along dir, as hy_fluxBuf), but the
+// register hand-off reads only the boundary faces
+// (Grid_putFluxData_block.F90:200-202, 278-279) and hy_rk_correctFluxes
+// only their corrections, so interior faces are never accumulated (they
+// stay zero).
 __global__ void __launch_bounds__(256) k_add_fluxes(DevParams P, const BlockMeta* __restrict__ meta,
                                                     const int* __restrict__ blist, const double* __restrict__ fl,
                                                     double* __restrict__ fb, int dir, double weight, int add) {
   const BlockMeta m = meta[blist[blockIdx.y]];
   Lims L = stage_lims(P, m, 0);
-  int e[3] = {L.hi[0] - L.lo[0] + 1, L.hi[1] - L.lo[1] + 1, L.hi[2] - L.lo[2] + 1};
+  const int n[3] = {L.hi[0] - L.lo[0] + 1, L.hi[1] - L.lo[1] + 1, L.hi[2] - L.lo[2] + 1};
+  int e[3] = {n[0], n[1], n[2]};
   e[dir] += 1;
-  const int total = e[0] * e[1] * e[2];
+  const int d2 = (dir == 0) ? 1 : 0, d3 = (dir == 2) ? 1 : 2;
+  const int total = 2 * n[d2] * n[d3];
   const int nf = F_NUM + P.ns;
-  const int64_t bstride = (int64_t)total;
+  const int64_t bstride = (int64_t)e[0] * e[1] * e[2];
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-    const int i = t % e[0], j = (t / e[0]) % e[1], k = t / (e[0] * e[1]);
-    const int64_t f = m.fbase + fix(m, L.lo[0] + i, L.lo[1] + j, L.lo[2] + k);
-    const int64_t o = m.bbase + t;
-    for (int n = 0; n < nf; n++) {
-      const double w = weight * fl[f + n * m.fstride];
-      fb[o + n * bstride] = add ? fb[o + n * bstride] + w : w;
+    int f[3];
+    f[d2] = t % n[d2];
+    f[d3] = (t / n[d2]) % n[d3];
+    f[dir] = (t / (n[d2] * n[d3])) ? n[dir] : 0;
+    const int64_t fo = m.fbase + fix(m, L.lo[0] + f[0], L.lo[1] + f[1], L.lo[2] + f[2]);
+    const int64_t o = m.bbase + f[0] + (int64_t)e[0] * (f[1] + (int64_t)e[1] * f[2]);
+    for (int q = 0; q < nf; q++) {
+      const double w = weight * fl[fo + q * m.fstride];
+      fb[o + q * bstride] = add ? fb[o + q * bstride] + w : w;
     }
   }
 }
@@ -742,11 +752,18 @@ hipError_t plain_stage(const PlainLaunch& L, int ngoff, double c1, double c2, do
     if (e != hipSuccess) return e;
   }
   if (L.fb[0] != nullptr) {
+    const int nt[3] = {L.nx, L.P.ndim > 1 ? L.ny : 1, L.P.ndim > 2 ? L.nz : 1};
+    double nfc = 0.;
+    for (int dir = 0; dir < L.P.ndim; dir++) nfc += 2.0 * nt[dir == 0 ? 1 : 0] * nt[dir == 2 ? 1 : 2];
+    // per boundary face: read the face flux and the register, write the register
+    if (L.prof) L.prof->begin(L.stream, "k_add_fluxes", 24.0 * nfc * L.nblk * (5 + L.P.ns), nfc * L.nblk * (5 + L.P.ns) * 2);
     for (int dir = 0; dir < L.P.ndim; dir++) {
-      hipLaunchKernelGGL(k_add_fluxes, dim3(grid_x_for(box * 2, 256), L.nblk), dim3(256), 0, L.stream, L.P, L.meta,
+      const int faces = 2 * nt[dir == 0 ? 1 : 0] * nt[dir == 2 ? 1 : 2];
+      hipLaunchKernelGGL(k_add_fluxes, dim3(grid_x_for(faces, 256), L.nblk), dim3(256), 0, L.stream, L.P, L.meta,
                          L.blist, L.fl[dir], L.fb[dir], dir, L.weight, L.add_flux);
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
+    if (L.prof) L.prof->end(L.stream);
   }
   if (L.prof) {
     // reads U0 and S (5+ns each) and one face flux per face of the box,

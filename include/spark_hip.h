@@ -175,7 +175,11 @@ int spark_step_end(spark_batch* b);
  * Driver/DriverMain/Unsplit/Driver_evolveAll.F90:136-275. */
 int spark_advance(spark_batch* b, int nsteps, double* dts);
 
-/* Flux registers (interior faces, weighted sum over stages).
+/* Flux registers (weighted sum over stages) in the shape of hy_fluxBuf[XYZ]:
+ * every face of the block's interior, (NFLUXES, nx+1, ny, nz) for x etc.
+ * Only the block's boundary faces (index 0 and n along the direction) are
+ * accumulated, the faces Grid_putFluxData_block hands to PARAMESH
+ * (Paramesh4/Grid_putFluxData_block.F90:200-202, 278-279); the others read 0.
  * get: device -> host fluxBufX/Y/Z(NFLUXES, ...) per block, Fortran order;
  *      replaces the hy_fluxBuf[XYZ] hand-off of Grid_putFluxData_block,
  *      Spark/Hydro.F90-mc:197-198.
