@@ -14,6 +14,21 @@ namespace spark {
 
 struct Lims { int lo[3], hi[3]; };
 
+// Workgroups of a launch are dispatched round-robin over the 8 XCDs
+// (linear id mod 8), each with its own L2.  For the cell-chunk kernels
+// (grid.x = chunks of a block's region, grid.y = blocks) this remaps the
+// linear id so that each XCD runs one contiguous range of (chunk, block)
+// pairs: consecutive chunks of a block, which read each other's face and
+// state rows (k+1 planes, j+1 rows), then share an L2.
+__device__ __forceinline__ void xcd_chunk(int& cx, int& cy) {
+  const int gx = (int)gridDim.x, G = gx * (int)gridDim.y;
+  const int L = (int)blockIdx.x + gx * (int)blockIdx.y;
+  const int q = G >> 3, r = G & 7, x = L & 7;
+  const int l = x * q + (x < r ? x : r) + (L >> 3);
+  cx = l % gx;
+  cy = l / gx;
+}
+
 __device__ __forceinline__ Lims stage_lims(const DevParams& P, const BlockMeta& m, int ngoff) {
   Lims L;
   const int n[3] = {m.nx, m.ny, m.nz};
@@ -37,7 +52,9 @@ __device__ __forceinline__ int64_t fix(const BlockMeta& m, int i, int j, int k) 
 __global__ void __launch_bounds__(256) k_flatten(DevParams P, const BlockMeta* __restrict__ meta,
                                                  const int* __restrict__ blist, const double* __restrict__ S,
                                                  double* __restrict__ flat, int ngoff) {
-  const BlockMeta m = meta[blist[blockIdx.y]];
+  int cx, cy;
+  xcd_chunk(cx, cy);
+  const BlockMeta m = meta[blist[cy]];
   Lims L = stage_lims(P, m, ngoff);
   const int kx = 1, ky = (P.ndim > 1), kz = (P.ndim == 3);
   const int lo0 = L.lo[0] - kx, lo1 = L.lo[1] - ky, lo2 = L.lo[2] - kz;
@@ -48,7 +65,7 @@ __global__ void __launch_bounds__(256) k_flatten(DevParams P, const BlockMeta* _
   const int64_t str[3] = {1, (int64_t)m.NX, (int64_t)m.NX * m.NY};
   const int velv[3] = {V_VELX, V_VELY, V_VELZ};
   const double betaMin = 0.75, betaMax = 0.85, Zmin = 0.25, Zmax = 0.75;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+  for (int t = cx * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
     int i = lo0 + t % ex, j = lo1 + (t / ex) % ey, k = lo2 + t / (ex * ey);
     int64_t c = cix(m, i, j, k);
     double mn = 0.;
@@ -78,7 +95,8 @@ template <int DIR, int RECON, int RIEMANN, int NSM>
 __global__ void __launch_bounds__(1024) k_flux(DevParams P, const BlockMeta* __restrict__ meta,
                                                const int* __restrict__ blist, const double* __restrict__ S,
                                                const double* __restrict__ Ushok, const double* __restrict__ flat,
-                                               double* __restrict__ fl, int ngoff, int lanes, int* __restrict__ err) {
+                                               double* __restrict__ fl, int ngoff, int lanes, int* __restrict__ err,
+                                               double* __restrict__ fb, double weight, int add) {
   constexpr int R = ReconRadius<RECON>::R;
   constexpr int NRM = R_NUM + NSM;
   constexpr int NFM = F_NUM + NSM;
@@ -200,6 +218,36 @@ __global__ void __launch_bounds__(1024) k_flux(DevParams P, const BlockMeta* __r
 #pragma unroll
   for (int n = 0; n < NFM; n++)
     if (n < F_NUM + ns) fl[f + n * m.fstride] = F[n];
+  // flux registers (addFluxes, Hydro_funcs.F90-mc:481-507) of the block's
+  // two boundary faces of this direction: the register hand-off reads no
+  // other face (Grid_putFluxData_block.F90:200-202, 278-279), so the
+  // interior entries are never accumulated and stay zero
+  if (fb != nullptr) {
+    const int g = P.ng;
+    const int nn[3] = {m.nx, m.ny, m.nz};
+    bool bnd = ijk[DIR] == g || ijk[DIR] == g + nn[DIR];
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+      if (d != DIR && d < P.ndim) bnd = bnd && ijk[d] >= g && ijk[d] < g + nn[d];
+    if (bnd) {
+      int e[3], fi[3];
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        e[d] = d < P.ndim ? nn[d] : 1;
+        fi[d] = d < P.ndim ? ijk[d] - g : 0;
+      }
+      e[DIR] += 1;
+      const int64_t bstride = (int64_t)e[0] * e[1] * e[2];
+      const int64_t o = m.bbase + fi[0] + (int64_t)e[0] * (fi[1] + (int64_t)e[1] * fi[2]);
+#pragma unroll
+      for (int n = 0; n < NFM; n++) {
+        if (n < F_NUM + ns) {
+          const double w = weight * F[n];
+          fb[o + n * bstride] = add ? fb[o + n * bstride] + w : w;
+        }
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -213,7 +261,9 @@ __global__ void __launch_bounds__(256) k_update(DevParams P, const BlockMeta* __
                                                 const double* __restrict__ dtp, double c1, double c2, double c3,
                                                 int ngoff, int final_stage) {
   constexpr int NFM = F_NUM + NSM;
-  const BlockMeta m = meta[blist[blockIdx.y]];
+  int cx, cy;
+  xcd_chunk(cx, cy);
+  const BlockMeta m = meta[blist[cy]];
   Lims L = stage_lims(P, m, ngoff);
   const int ex = L.hi[0] - L.lo[0] + 1, ey = L.hi[1] - L.lo[1] + 1, ez = L.hi[2] - L.lo[2] + 1;
   const int total = ex * ey * ez;
@@ -226,7 +276,7 @@ __global__ void __launch_bounds__(256) k_update(DevParams P, const BlockMeta* __
   const double* u0 = U0 + m.base;
   const double* ui = Sin + m.base;
   double* us = Sout + m.base;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+  for (int t = cx * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
     const int i = L.lo[0] + t % ex, j = L.lo[1] + (t / ex) % ey, k = L.lo[2] + t / (ex * ey);
     const int64_t c = cix(m, i, j, k);
     double A[NFM], B[NFM];
@@ -303,13 +353,15 @@ __global__ void __launch_bounds__(256) k_update(DevParams P, const BlockMeta* __
 __global__ void __launch_bounds__(256) k_shock_vc(DevParams P, const BlockMeta* __restrict__ meta,
                                                   const int* __restrict__ blist, double* __restrict__ U,
                                                   double* __restrict__ vc, int band) {
-  const BlockMeta m = meta[blist[blockIdx.y]];
+  int cx, cy;
+  xcd_chunk(cx, cy);
+  const BlockMeta m = meta[blist[cy]];
   Lims L = stage_lims(P, m, band + 1);
   const int ex = L.hi[0] - L.lo[0] + 1, ey = L.hi[1] - L.lo[1] + 1, ez = L.hi[2] - L.lo[2] + 1;
   const int total = ex * ey * ez;
   double* up = U + m.base;
   const int64_t cs = m.cstride;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+  for (int t = cx * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
     const int i = L.lo[0] + t % ex, j = L.lo[1] + (t / ex) % ey, k = L.lo[2] + t / (ex * ey);
     const int64_t c = cix(m, i, j, k);
     up[V_SHOK * cs + c] = 0.;
@@ -320,7 +372,9 @@ __global__ void __launch_bounds__(256) k_shock_vc(DevParams P, const BlockMeta* 
 __global__ void __launch_bounds__(256) k_shock(DevParams P, const BlockMeta* __restrict__ meta,
                                                const int* __restrict__ blist, double* __restrict__ U,
                                                const double* __restrict__ vc, int band) {
-  const BlockMeta m = meta[blist[blockIdx.y]];
+  int cx, cy;
+  xcd_chunk(cx, cy);
+  const BlockMeta m = meta[blist[cy]];
   Lims L = stage_lims(P, m, band);
   const int ex = L.hi[0] - L.lo[0] + 1, ey = L.hi[1] - L.lo[1] + 1, ez = L.hi[2] - L.lo[2] + 1;
   const int total = ex * ey * ez;
@@ -330,7 +384,7 @@ __global__ void __launch_bounds__(256) k_shock(DevParams P, const BlockMeta* __r
   const int ky = (P.ndim > 1), kz = (P.ndim == 3);
   const int64_t sx = 1, sy = m.NX, sz = (int64_t)m.NX * m.NY;
   const double beta = 0.5, delta = 0.1;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+  for (int t = cx * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
     const int i = L.lo[0] + t % ex, j = L.lo[1] + (t / ex) % ey, k = L.lo[2] + t / (ex * ey);
     const int64_t c = cix(m, i, j, k);
     const double* p = up + V_PRES * cs;
@@ -489,40 +543,6 @@ __global__ void __launch_bounds__(256) k_ug_fill(DevParams P, const BlockMeta* _
     const int64_t src = ms.base + cix(ms, sl[0], sl[1], sl[2]);
     const int64_t dst = m.base + cix(m, loc[0], loc[1], loc[2]);
     for (int q = 0; q < vars.n; q++) U[dst + vars.v[q] * m.cstride] = U[src + vars.v[q] * ms.cstride];
-  }
-}
-
-// ---------------------------------------------------------------------------
-// flux-register accumulation (addFluxes, Hydro_funcs.F90-mc:481-507) on the
-// block's two boundary faces of direction dir.  The registers are laid out
-// over all faces of the block (n+1 along dir, as hy_fluxBuf), but the
-// register hand-off reads only the boundary faces
-// (Grid_putFluxData_block.F90:200-202, 278-279) and hy_rk_correctFluxes
-// only their corrections, so interior faces are never accumulated (they
-// stay zero).
-__global__ void __launch_bounds__(256) k_add_fluxes(DevParams P, const BlockMeta* __restrict__ meta,
-                                                    const int* __restrict__ blist, const double* __restrict__ fl,
-                                                    double* __restrict__ fb, int dir, double weight, int add) {
-  const BlockMeta m = meta[blist[blockIdx.y]];
-  Lims L = stage_lims(P, m, 0);
-  const int n[3] = {L.hi[0] - L.lo[0] + 1, L.hi[1] - L.lo[1] + 1, L.hi[2] - L.lo[2] + 1};
-  int e[3] = {n[0], n[1], n[2]};
-  e[dir] += 1;
-  const int d2 = (dir == 0) ? 1 : 0, d3 = (dir == 2) ? 1 : 2;
-  const int total = 2 * n[d2] * n[d3];
-  const int nf = F_NUM + P.ns;
-  const int64_t bstride = (int64_t)e[0] * e[1] * e[2];
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-    int f[3];
-    f[d2] = t % n[d2];
-    f[d3] = (t / n[d2]) % n[d3];
-    f[dir] = (t / (n[d2] * n[d3])) ? n[dir] : 0;
-    const int64_t fo = m.fbase + fix(m, L.lo[0] + f[0], L.lo[1] + f[1], L.lo[2] + f[2]);
-    const int64_t o = m.bbase + f[0] + (int64_t)e[0] * (f[1] + (int64_t)e[1] * f[2]);
-    for (int q = 0; q < nf; q++) {
-      const double w = weight * fl[fo + q * m.fstride];
-      fb[o + q * bstride] = add ? fb[o + q * bstride] + w : w;
-    }
   }
 }
 
@@ -697,7 +717,7 @@ static hipError_t launch_flux_t(const PlainLaunch& L, int ngoff) {
     L.prof->begin(L.stream, names[DIR], bytes, flops);
   }
   hipLaunchKernelGGL((k_flux<DIR, RECON, RIEMANN, NSM>), grid, block, shm, L.stream, L.P, L.meta, L.blist, L.Sin,
-                     L.U0, L.flat, L.fl[DIR], ngoff, lanes, L.err);
+                     L.U0, L.flat, L.fl[DIR], ngoff, lanes, L.err, L.fb[DIR], L.weight, L.add_flux);
   if (L.prof) L.prof->end(L.stream);
   return hipGetLastError();
 }
@@ -750,20 +770,6 @@ hipError_t plain_stage(const PlainLaunch& L, int ngoff, double c1, double c2, do
                                                                 : launch_flux_r<2, 8>(L, ngoff);
     }
     if (e != hipSuccess) return e;
-  }
-  if (L.fb[0] != nullptr) {
-    const int nt[3] = {L.nx, L.P.ndim > 1 ? L.ny : 1, L.P.ndim > 2 ? L.nz : 1};
-    double nfc = 0.;
-    for (int dir = 0; dir < L.P.ndim; dir++) nfc += 2.0 * nt[dir == 0 ? 1 : 0] * nt[dir == 2 ? 1 : 2];
-    // per boundary face: read the face flux and the register, write the register
-    if (L.prof) L.prof->begin(L.stream, "k_add_fluxes", 24.0 * nfc * L.nblk * (5 + L.P.ns), nfc * L.nblk * (5 + L.P.ns) * 2);
-    for (int dir = 0; dir < L.P.ndim; dir++) {
-      const int faces = 2 * nt[dir == 0 ? 1 : 0] * nt[dir == 2 ? 1 : 2];
-      hipLaunchKernelGGL(k_add_fluxes, dim3(grid_x_for(faces, 256), L.nblk), dim3(256), 0, L.stream, L.P, L.meta,
-                         L.blist, L.fl[dir], L.fb[dir], dir, L.weight, L.add_flux);
-      if ((e = hipGetLastError()) != hipSuccess) return e;
-    }
-    if (L.prof) L.prof->end(L.stream);
   }
   if (L.prof) {
     // reads U0 and S (5+ns each) and one face flux per face of the box,
