@@ -20,7 +20,16 @@ __device__ __forceinline__ int64_t acix(int NX, int i, int j, int k) {
 
 __device__ __forceinline__ int leaf_at(const AmrGrid& G, int L, const int gi[3]) {
   const int sh = G.lmax - L;
-  const int s0 = (gi[0] << sh) / G.n, s1 = (gi[1] << sh) / G.n, s2 = (gi[2] << sh) / G.n;
+  int s0, s1, s2;
+  if (G.nshift >= 0) {  // n a power of two (the usual 8, 16, 32): shifts, no integer divide
+    s0 = (gi[0] << sh) >> G.nshift;
+    s1 = (gi[1] << sh) >> G.nshift;
+    s2 = (gi[2] << sh) >> G.nshift;
+  } else {
+    s0 = (gi[0] << sh) / G.n;
+    s1 = (gi[1] << sh) / G.n;
+    s2 = (gi[2] << sh) / G.n;
+  }
   return G.map[s0 + (int64_t)G.md[0] * (s1 + (int64_t)G.md[1] * s2)];
 }
 
@@ -32,7 +41,9 @@ __device__ __forceinline__ int amr_domain(const AmrGrid& G, int L, int gi[3]) {
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     const int N = (G.nb0[d] * G.n) << (L - 1);
-    if (G.bc[d] == AMR_BC_PERIODIC) gi[d] = ((gi[d] % N) + N) % N;
+    // guard cells and prolongation neighbours lie less than one period
+    // outside the domain, so one add or subtract wraps them
+    if (G.bc[d] == AMR_BC_PERIODIC) gi[d] = gi[d] < 0 ? gi[d] + N : (gi[d] >= N ? gi[d] - N : gi[d]);
     else if (G.bc[d] == AMR_BC_REFLECT && (gi[d] < 0 || gi[d] > N - 1)) {
       gi[d] = gi[d] < 0 ? -1 - gi[d] : 2 * N - 1 - gi[d];
       flip |= 1 << d;

@@ -103,6 +103,7 @@ struct spark_batch {
   // over the interior only; the others telescope over their guard cells
   std::vector<char> amr_remap_blk;
   int amr_remap_count = 0;
+  bool same_level = true;      // SPARK_OPT_SAME_LEVEL_HALOS
   int* remap_ids_d = nullptr;  // the remapping blocks
   int* tele_ids_d = nullptr;   // the telescoping blocks
   int n_tele = 0;
@@ -197,6 +198,9 @@ int required_guard(const spark_params& p) {
 bool fused_remap(const spark_batch* b);
 void drop_graph(spark_batch* b);
 
+// batched forms on a block hierarchy: same-level halos in use
+bool amr_remap_on(const spark_batch* b) { return b->same_level && b->amr_remap_count > 0; }
+
 int run_dt(spark_batch* b, int hist_slot) {
   for (auto& c : b->classes) {
     PlainLaunch L = make_launch(b, c);
@@ -226,7 +230,7 @@ int run_fill(spark_batch* b, double* pool, bool full, int depth) {
     for (int v : need) vl.v[vl.n++] = v;
     for (int s = 0; s < b->p.nscalars; s++) vl.v[vl.n++] = spark::V_NPROP + s;
   }
-  if (b->amr && !full && b->amr_remap_count > 0) {
+  if (b->amr && !full && amr_remap_on(b)) {
     // remapping blocks read no guard cell of their own in the stages: fill
     // only what shockDetect of their interior reads
     if (b->n_tele > 0) {
@@ -292,7 +296,7 @@ int run_step_begin(spark_batch* b) {
   // (the batched forms have no hybrid Riemann solver and read SHOK nowhere:
   // they compute it for the interior, where it is part of the solution)
   const int band = b->fused ? 0 : (b->p.guard_mode == SPARK_GUARD_REFILL ? 0 : b->limits_array[0]) + 1;
-  if (b->amr_remap_count > 0) {
+  if (amr_remap_on(b)) {
     // remapping blocks: SHOK of the interior only (nothing reads theirs
     // outside it); a hierarchy has one block shape
     PlainLaunch L = make_launch(b, b->classes[0]);
@@ -335,7 +339,7 @@ int stage_ngoff(const spark_batch* b, int stage) {
 // stage region of one block in the batched forms: blocks that read their
 // halos from same-level neighbours update only their interior
 int blk_ngoff(const spark_batch* b, int blk, int stage) {
-  if (b->amr_remap_count > 0 && b->amr_remap_blk[blk]) return 0;
+  if (amr_remap_on(b) && b->amr_remap_blk[blk]) return 0;
   return stage_ngoff(b, stage);
 }
 
@@ -382,7 +386,7 @@ spark::FusedArgs fused_args(spark_batch* b, int stage) {
   A.Sin = stage_in(b, stage);
   A.Sout = stage_out(b, stage);
   for (int d = 0; d < 3; d++) A.fb[d] = b->p.flux_correct ? b->fb[d] : nullptr;
-  A.nbr = (fused_remap(b) || b->amr_remap_count > 0) ? b->nbr_d : nullptr;
+  A.nbr = (fused_remap(b) || amr_remap_on(b)) ? b->nbr_d : nullptr;
   A.weight = b->weights[stage - 1];
   A.add_flux = stage > 1;
   A.dtp = b->dtp;
@@ -598,7 +602,7 @@ int fused_tune(spark_batch* b) {
   std::vector<int> ngoffs;
   for (int s = 1; s <= b->last_stage; s++)
     if (std::find(ngoffs.begin(), ngoffs.end(), stage_ngoff(b, s)) == ngoffs.end()) ngoffs.push_back(stage_ngoff(b, s));
-  if (b->amr_remap_count > 0 && std::find(ngoffs.begin(), ngoffs.end(), 0) == ngoffs.end()) ngoffs.push_back(0);
+  if (amr_remap_on(b) && std::find(ngoffs.begin(), ngoffs.end(), 0) == ngoffs.end()) ngoffs.push_back(0);
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -1134,6 +1138,11 @@ int spark_set_option(spark_batch* b, int option, int value) {
     case SPARK_OPT_AUTOTUNE:
       b->autotune = value != 0;
       return SPARK_OK;
+    case SPARK_OPT_SAME_LEVEL_HALOS:
+      b->same_level = value != 0;
+      b->planned = false;
+      drop_graph(b);
+      return SPARK_OK;
     default:
       return fail(SPARK_ERR_ARG, "spark_set_option: unknown option %d", option);
   }
@@ -1281,6 +1290,11 @@ int spark_set_amr_grid(spark_batch* b, int nbx0, int nby0, int nbz0, const int* 
     if (bcs[d] < SPARK_BC_OUTFLOW || bcs[d] > SPARK_BC_REFLECT)
       return fail(SPARK_ERR_ARG, "spark_set_amr_grid: boundary code %d in dimension %d", bcs[d], d);
     per[d] = bcs[d] == SPARK_BC_PERIODIC;
+    // a periodic guard cell (and its prolongation stencil) must lie within
+    // one period of the domain
+    if (per[d] && nb0[d] * n < b->p.nguard + 1)
+      return fail(SPARK_ERR_ARG, "spark_set_amr_grid: periodic dimension %d spans %d cells, fewer than nguard + 1 = %d",
+                  d, nb0[d] * n, b->p.nguard + 1);
   }
   for (int i = 0; i < b->nblk; i++) {
     const int L = b->meta_h[i].level, sh = lmax - L, g = b->p.nguard;
@@ -1406,6 +1420,9 @@ int spark_set_amr_grid(spark_batch* b, int nbx0, int nby0, int nbz0, const int* 
   HIPCHK(hipMemcpy(b->amr_org_d, origins, sizeof(int) * 3 * b->nblk, hipMemcpyHostToDevice));
   b->amr_g.lmax = lmax;
   b->amr_g.n = n;
+  b->amr_g.nshift = -1;
+  for (int q = 0; q < 31; q++)
+    if ((1 << q) == n) b->amr_g.nshift = q;
   for (int d = 0; d < 3; d++) { b->amr_g.nb0[d] = nb0[d]; b->amr_g.md[d] = md[d]; b->amr_g.bc[d] = bcs[d]; }
   b->amr_g.map = b->amr_map_d;
   b->amr_g.org = b->amr_org_d;

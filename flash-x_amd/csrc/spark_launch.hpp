@@ -79,6 +79,7 @@ hipError_t plain_correct(const PlainLaunch& L, double* U, const unsigned char* f
 struct AmrGrid {
   int lmax;       // finest level (1-based)
   int n;          // interior cells per side of every block
+  int nshift;     // log2(n) when n is a power of two, else -1
   int nb0[3];     // level-1 blocks per dimension
   int md[3];      // leaf map extent: nb0 << (lmax - 1)
   const int* map; // finest-level block slot -> leaf block

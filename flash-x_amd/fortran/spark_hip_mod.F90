@@ -22,7 +22,7 @@ module spark_hip
   integer, parameter, public :: SPARK_FORM_PLAIN_FP64 = 0, SPARK_FORM_FP32_FLUX = 1, &
                                 SPARK_FORM_BATCHED_FP64 = 2
   integer, parameter, public :: SPARK_GUARD_TELESCOPE = 0, SPARK_GUARD_REFILL = 1
-  integer, parameter, public :: SPARK_OPT_GRAPH = 1, SPARK_OPT_AUTOTUNE = 2
+  integer, parameter, public :: SPARK_OPT_GRAPH = 1, SPARK_OPT_AUTOTUNE = 2, SPARK_OPT_SAME_LEVEL_HALOS = 3
   integer, parameter, public :: SPARK_BC_OUTFLOW = 0, SPARK_BC_PERIODIC = 1, SPARK_BC_REFLECT = 2
 
   !! struct spark_params (same member order and types)

@@ -260,7 +260,7 @@ class Batch:
               "spark_set_flux_corrections")
 
     def set_option(self, option: str, value: int):
-        code = {"graph": 1, "autotune": 2}[option]
+        code = {"graph": 1, "autotune": 2, "same_level_halos": 3}[option]
         check(lib().spark_set_option(self.h, code, int(value)), "spark_set_option")
 
     def tiles(self):

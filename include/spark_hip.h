@@ -221,10 +221,14 @@ int spark_profile_read(spark_batch* b, spark_kernel_stat* stats, int max, int* n
 /* Options.  SPARK_OPT_GRAPH (default 1): spark_advance captures one step
  * into a hipGraph and replays it.  SPARK_OPT_AUTOTUNE (default 1): the
  * batched FP32-flux form times candidate tile shapes per block shape and
- * stage region on the first step and keeps the fastest.  No counterpart
- * in the reference. */
+ * stage region on the first step and keeps the fastest.
+ * SPARK_OPT_SAME_LEVEL_HALOS (default 1): on a block hierarchy the batched
+ * forms let leaves surrounded by same-level leaves read their halos from
+ * them; 0 makes every leaf telescope over its own guard cells (the same
+ * values, more work).  No counterpart in the reference. */
 #define SPARK_OPT_GRAPH 1
 #define SPARK_OPT_AUTOTUNE 2
+#define SPARK_OPT_SAME_LEVEL_HALOS 3
 int spark_set_option(spark_batch* b, int option, int value);
 /* Tiles chosen by the batched form: rows of (nx, ny, nz, ngoff, tx, ty). */
 int spark_batched_tiles(spark_batch* b, int* rows, int max_rows, int* nrows);

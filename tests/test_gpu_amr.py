@@ -27,9 +27,11 @@ def two_level(**kw):
     return problems.rt_amr(nb0=(2, 2, 2), n=8, lmax=2, slabs=((0.5, 1.0),), **kw)
 
 
-def run(prob, nsteps, form):
+def run(prob, nsteps, form, **options):
     b = capi.Batch(dict(prob.cfg, form=form), prob.blocks)
     try:
+        for k, v in options.items():
+            b.set_option(k, v)
         b.pack(prob.arrays)
         b.set_amr_grid(prob.info["nb0"], prob.info["origins"], prob.info["bc"])
         dts = b.advance(nsteps)
@@ -139,6 +141,20 @@ def test_batched_fp64_amr_same_level_halos_bit_exact_with_plain():
     assert np.array_equal(d64, db)
     nd = sum(int(np.count_nonzero(x != y)) for x, y in zip(interiors(prob, o64), interiors(prob, ob)))
     print(f"differing interior values {nd}", flush=True)
+    assert nd == 0
+
+
+@pytest.mark.parametrize("form", ["batched_fp64", "fp32_flux"])
+def test_same_level_halos_equal_telescoping(form):
+    """Reading the halo of a leaf surrounded by same-level leaves from those
+    leaves gives the values telescoping computes in its guard cells: the
+    batched forms with and without it agree bit for bit."""
+    prob = three_level(nscalars=3)
+    d_on, o_on, _ = run(prob, 3, form)
+    d_off, o_off, _ = run(prob, 3, form, same_level_halos=0)
+    assert np.array_equal(d_on, d_off)
+    nd = sum(int(np.count_nonzero(x != y)) for x, y in zip(interiors(prob, o_on), interiors(prob, o_off)))
+    print(f"{form}: differing interior values {nd}", flush=True)
     assert nd == 0
 
 
