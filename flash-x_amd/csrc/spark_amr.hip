@@ -200,10 +200,11 @@ __global__ void __launch_bounds__(256) k_amr_fill(DevParams P, const BlockMeta* 
 // block's own face flux (Grid_getFluxCorrData_block.F90:182-216); every
 // other boundary face gets 0.  fb / fc: per block (NFLUX, e0, e1, e2) face
 // arrays, one flux at a time (stride e0*e1*e2), at BlockMeta::bbase.
-__global__ void __launch_bounds__(256) k_flux_exchange(DevParams P, const BlockMeta* __restrict__ meta, AmrGrid G,
+__global__ void __launch_bounds__(256) k_flux_exchange(DevParams P, const BlockMeta* __restrict__ meta,
+                                                       const int* __restrict__ blist, AmrGrid G,
                                                        const double* __restrict__ fb, double* __restrict__ fc,
                                                        int dir) {
-  const int bid = blockIdx.y;
+  const int bid = blist ? blist[blockIdx.y] : (int)blockIdx.y;
   const BlockMeta m = meta[bid];
   const int n = G.n, L = m.level;
   const int o[3] = {G.org[3 * bid], G.org[3 * bid + 1], G.org[3 * bid + 2]};
@@ -281,7 +282,7 @@ hipError_t amr_flux_exchange(const PlainLaunch& L, const AmrGrid& G, double* con
   if (L.prof) L.prof->begin(L.stream, "k_flux_exchange", 8.0 * faces * nf * 3, faces * nf * 4);
   for (int dir = 0; dir < 3; dir++) {
     hipLaunchKernelGGL(k_flux_exchange, dim3(grid_x(2 * G.n * G.n, 64), L.nblk), dim3(256), 0, L.stream, L.P, L.meta,
-                       G, fb[dir], fc[dir], dir);
+                       L.blist, G, fb[dir], fc[dir], dir);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

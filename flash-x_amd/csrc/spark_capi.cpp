@@ -112,6 +112,11 @@ struct spark_batch {
   unsigned char* amr_fmask_d = nullptr;
   int* corr_ids_d = nullptr;
   int n_corr = 0;
+  // fc is zero on every face without a finer leaf across (set at
+  // allocation and kept by the exchange): the exchange then only visits
+  // the blocks that have such a face.  Host corrections or a new hierarchy
+  // clear it until the next full exchange.
+  bool fc_clean = false;
   spark::AmrGrid amr_g{};
   int* amr_map_d = nullptr;
   int* amr_org_d = nullptr;
@@ -253,13 +258,29 @@ int run_fill(spark_batch* b, double* pool, bool full, int depth) {
   return SPARK_OK;
 }
 
-// the fine/coarse flux-register exchange and hy_rk_correctFluxes on every
-// block (Spark/Hydro.F90-mc:205-236), dt from the device
-int run_amr_correct(spark_batch* b) {
+// the fine/coarse flux-register exchange: over every block until fc is
+// known to be zero off the faces toward finer leaves, then only over the
+// blocks that have such a face
+int run_exchange(spark_batch* b) {
   PlainLaunch L = make_launch(b, b->classes[0]);
-  L.blist = nullptr;
-  L.nblk = b->nblk;
+  if (b->fc_clean) {
+    if (b->n_corr == 0) return SPARK_OK;
+    L.blist = b->corr_ids_d;
+    L.nblk = b->n_corr;
+  } else {
+    L.blist = nullptr;
+    L.nblk = b->nblk;
+  }
   HIPCHK(spark::amr_flux_exchange(L, b->amr_g, b->fb, b->fc));
+  b->fc_clean = true;
+  return SPARK_OK;
+}
+
+// the exchange and hy_rk_correctFluxes on every block
+// (Spark/Hydro.F90-mc:205-236), dt from the device
+int run_amr_correct(spark_batch* b) {
+  int rc;
+  if ((rc = run_exchange(b))) return rc;
   if (b->fused && !b->fused64) {
     // FP32-flux form: only the cells on a face toward a finer leaf (their
     // whole x, y, z sequence).  Everywhere else the correction is zero and
@@ -844,14 +865,17 @@ int spark_batch_create(const spark_params* p, int nblocks, const spark_block_des
   if (p->flux_correct) {
     for (int q = 0; q < 3; q++) ALLOC(b->fb[q], sizeof(double) * b->bpool);
     for (int q = 0; q < 3; q++) ALLOC(b->fc[q], sizeof(double) * b->bpool);
-    // only boundary faces are ever accumulated: the interior ones read 0
+    // only boundary faces are ever accumulated: the interior ones read 0;
+    // corrections start at 0 everywhere
     for (int q = 0; q < 3; q++) {
       hipError_t e_ = hipMemset(b->fb[q], 0, sizeof(double) * b->bpool);
+      if (e_ == hipSuccess) e_ = hipMemset(b->fc[q], 0, sizeof(double) * b->bpool);
       if (e_ != hipSuccess) {
         free_batch(b);
         return fail(SPARK_ERR_HIP, "hipMemset: %s", hipGetErrorString(e_));
       }
     }
+    b->fc_clean = true;
   }
   ALLOC(b->dtp, sizeof(double));
   ALLOC(b->loc, sizeof(long long) * 2);
@@ -1099,6 +1123,12 @@ int spark_advance(spark_batch* b, int nsteps, double* dts) {
   if (b->fused && !b->planned && (rc = fused_plan(b))) return rc;
   HIPCHK(spark::plain_set_int(g_stream, b->hist_slot, 0));
   const bool graph = b->use_graph && !b->prof.on && nsteps > 0;
+  if (graph && b->gexec == nullptr && b->amr && b->p.flux_correct && !b->fc_clean) {
+    // zero the corrections outside the graph so that the captured exchange
+    // visits only the blocks with a finer face
+    for (int d = 0; d < 3; d++) HIPCHK(hipMemsetAsync(b->fc[d], 0, sizeof(double) * b->bpool, g_stream));
+    b->fc_clean = true;
+  }
   if (graph && b->gexec == nullptr) {
     // capture one device-resident step; every parameter of it lives on the
     // device (dt, the dt-history slot, the queue heads), so replays chain
@@ -1199,6 +1229,8 @@ int spark_set_flux_corrections(spark_batch* b, const double* const* fbx, const d
                                const double* const* fbz) {
   if (!b) return fail(SPARK_ERR_ARG, "spark_set_flux_corrections: null batch");
   if (!b->p.flux_correct) return fail(SPARK_ERR_STATE, "flux corrections need flux_correct = 1");
+  b->fc_clean = false;
+  drop_graph(b);
   const double* const* ins[3] = {fbx, fby, fbz};
   const int nf = 5 + b->p.nscalars;
   HIPCHK(hipStreamSynchronize(g_stream));
@@ -1234,11 +1266,7 @@ int spark_flux_exchange(spark_batch* b) {
   if (!b) return fail(SPARK_ERR_ARG, "spark_flux_exchange: null batch");
   if (!b->p.flux_correct) return fail(SPARK_ERR_STATE, "spark_flux_exchange needs flux_correct = 1");
   if (!b->amr) return fail(SPARK_ERR_STATE, "spark_flux_exchange: no block hierarchy declared");
-  PlainLaunch L = make_launch(b, b->classes[0]);
-  L.blist = nullptr;
-  L.nblk = b->nblk;
-  HIPCHK(spark::amr_flux_exchange(L, b->amr_g, b->fb, b->fc));
-  return SPARK_OK;
+  return run_exchange(b);
 }
 
 int spark_set_amr_grid(spark_batch* b, int nbx0, int nby0, int nbz0, const int* origins, const int* bc) {
@@ -1405,6 +1433,7 @@ int spark_set_amr_grid(spark_batch* b, int nbx0, int nby0, int nbz0, const int* 
     HIPCHK(hipMalloc((void**)&b->amr_fmask_d, b->nblk));
     HIPCHK(hipMemcpy(b->amr_fmask_d, fm.data(), b->nblk, hipMemcpyHostToDevice));
     b->n_corr = (int)cl.size();
+    b->fc_clean = false;
     if (!cl.empty()) {
       HIPCHK(hipMalloc((void**)&b->corr_ids_d, sizeof(int) * cl.size()));
       HIPCHK(hipMemcpy(b->corr_ids_d, cl.data(), sizeof(int) * cl.size(), hipMemcpyHostToDevice));
