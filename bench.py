@@ -4,7 +4,10 @@
 Workloads (--workload):
   sedov128     (default) 3D Sedov blast, uniform 8^3 blocks of 16^3 interior
                cells (128^3 zones), 6 guard cells refilled on the device
-               between stages, WENO5 + HLLC, RK3, CFL 0.8, gamma 1.4.
+               between stages, WENO5 + HLLC, RK3, CFL 0.45 (the reference's
+               own 3D Sedov test value: at 0.8 the time step, the largest
+               (|u_d| + c)/dx_d over directions, lets the unsplit 3D update
+               go unstable within ~15 steps), gamma 1.4.
   rt_amr       3D Rayleigh-Taylor with 8 mass scalars on a 3-level block
                hierarchy (4384 leaves of 16^3: 32 / 256 / 4096 per level,
                256^3-equivalent resolution over x, y in [0, 1], z in
@@ -38,11 +41,13 @@ sys.path.insert(0, str(ROOT / "flash-x_amd"))
 
 METRIC = "zone-updates/s per RK3 step: plain FP64 vs batched FP32-flux; % HBM roofline"
 HBM_PEAK_GBS = 8000.0
+# the reference's 3D Sedov test runs at 0.45 (Sedov/tests/test_amr_unsplit_3d.par:47)
+SEDOV_CFL = 0.45
 FP64_PEAK_TFS = 78.6
 
 WORKLOADS = {
     "sedov128": "3D Sedov blast, 8^3 blocks of 16^3 (128^3 zones), 6 guard cells refilled on the device between "
-                "stages, WENO5-Z + HLLC, RK3, CFL 0.8, gamma 1.4",
+                "stages, WENO5-Z + HLLC, RK3, CFL 0.45, gamma 1.4",
     "rt_amr": "3D Rayleigh-Taylor, 8 mass scalars, 3-level block hierarchy of 4384 16^3 leaves (32/256/4096 per "
               "level, 256^3-equivalent), rho 2 over 1, g = -1 source, hydrostatic p (2.5 at the interface), "
               "seeded interface modes of amplitude 0.01, periodic x/y and reflecting z, device guard fill over levels "
@@ -99,7 +104,7 @@ def make_problem(workload):
         return problems.batched(nblocks=4096, nguard=10)
     if workload == "rt_amr":
         return problems.rt_amr()
-    return problems.sedov3d(nb=8, n=16, nguard=6, guard_mode="refill")
+    return problems.sedov3d(nb=8, n=16, nguard=6, guard_mode="refill", cfl=SEDOV_CFL)
 
 
 def cpu_baseline(workload, target_s: float):
@@ -121,7 +126,7 @@ def cpu_baseline(workload, target_s: float):
         what = (f"the same Rayleigh-Taylor setup on a 1 x 1 x 2 level-1 grid ({len(prob.blocks)} leaves, "
                 f"{prob.zones} zones, guard fill and flux correction included)")
     else:
-        prob = problems.sedov3d(nb=8, n=16, nguard=6, guard_mode="refill")
+        prob = problems.sedov3d(nb=8, n=16, nguard=6, guard_mode="refill", cfl=SEDOV_CFL)
         run = lambda n: pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, prob.arrays, n, refill=True,
                                             nthreads=threads)
         what = "3D Sedov 128^3 (8^3 blocks of 16^3, 6 guards, refill)"
