@@ -725,6 +725,12 @@ int spark_params_default(spark_params* p) {
   p->guard_mode = SPARK_GUARD_TELESCOPE;
   p->nunk_vars = SPARK_NPROP;
   for (int v = 0; v < SPARK_NPROP + SPARK_MAX_SCALARS; v++) p->var_index[v] = v;
+  p->nspecies = 0;
+  p->eos = SPARK_EOS_GAMMA;
+  for (int s = 0; s < SPARK_MAX_SCALARS; s++) {
+    p->species_a[s] = 1.0;
+    p->species_gamma[s] = p->gamma;
+  }
   return SPARK_OK;
 }
 
@@ -773,8 +779,20 @@ int spark_batch_create(const spark_params* p, int nblocks, const spark_block_des
   for (int v = 0; v < nvar; v++)
     if (p->var_index[v] < 0 || p->var_index[v] >= p->nunk_vars)
       return fail(SPARK_ERR_ARG, "var_index[%d] = %d out of range", v, p->var_index[v]);
+  if (p->nspecies < 0 || p->nspecies > p->nscalars)
+    return fail(SPARK_ERR_ARG, "nspecies = %d (0..nscalars = %d)", p->nspecies, p->nscalars);
+  if (p->eos != SPARK_EOS_GAMMA && p->eos != SPARK_EOS_MULTIGAMMA) return fail(SPARK_ERR_ARG, "eos = %d", p->eos);
+  if (p->eos == SPARK_EOS_MULTIGAMMA) {
+    if (p->nspecies < 1) return fail(SPARK_ERR_ARG, "the Multigamma EOS needs nspecies >= 1");
+    for (int s = 0; s < p->nspecies; s++)
+      if (!(p->species_a[s] > 0.0) || !(p->species_gamma[s] > 1.0))
+        return fail(SPARK_ERR_ARG, "species %d: A = %g, gamma = %g", s, p->species_a[s], p->species_gamma[s]);
+  }
   if (p->form == SPARK_FORM_FP32_FLUX || p->form == SPARK_FORM_BATCHED_FP64) {
     if (p->ndim != 3) return fail(SPARK_ERR_ARG, "the batched form is 3D only (ndim = %d)", p->ndim);
+    if (p->nspecies != 0 || p->eos != SPARK_EOS_GAMMA)
+      return fail(SPARK_ERR_ARG, "the batched forms implement the gamma law with mass scalars only "
+                                 "(nspecies = %d, eos = %d)", p->nspecies, p->eos);
     if (p->use_hybrid_riemann)
       return fail(SPARK_ERR_ARG, "the batched form does not implement use_hybrid_riemann");
   }
@@ -798,6 +816,14 @@ int spark_batch_create(const spark_params* p, int nblocks, const spark_block_des
   }
   d.abar = p->abar;
   d.grav[0] = p->grav[0]; d.grav[1] = p->grav[1]; d.grav[2] = p->grav[2];
+  d.nspec = p->nspecies;
+  d.eos = p->eos;
+  d.gasConst = p->gasConst;
+  for (int s = 0; s < SPARK_MAX_SCALARS; s++) {
+    // Multigamma/eos_initMgamma.F90:15 and the 1/A of Multispecies_getSumInv.F90:150
+    d.mg_gm1[s] = s < p->nspecies ? 1. / (p->species_gamma[s] - 1.) : 0.0;
+    d.mg_ainv[s] = s < p->nspecies ? 1.e0 / p->species_a[s] : 0.0;
+  }
   d.ndim = p->ndim; d.ng = p->nguard; d.ns = p->nscalars;
   d.use_grav = p->use_gravity; d.use_flat = p->use_flattening; d.hybrid = p->use_hybrid_riemann;
   d.recon = p->recon; d.riemann = p->riemann;

@@ -325,6 +325,10 @@ struct DevParams {
   double gam1inv, ggprodinv, abar;  // EOS constants (host-computed like Gamma/Eos.F90:177,240-241)
   double grav[3];
   int ndim, ng, ns, use_grav, use_flat, hybrid, recon, riemann;
+  int nspec, eos;       // species among the scalars; 0 gamma law, 1 Multigamma
+  double gasConst;
+  double mg_gm1[8];     // 1/(gamma_s - 1) of each species
+  double mg_ainv[8];    // 1/A_s
 };
 
 // Per-block table entry (device resident)
@@ -341,17 +345,36 @@ struct BlockMeta {
   int level, id;
 };
 
-// Gamma-law EOS, MODE_DENS_EI, on one cell held in registers.
-struct EosOut { double pres, eint, ener, temp, game; };
+// EOS in MODE_DENS_EI on one cell held in registers (Eos_getData.F90:114-169
+// -> Eos -> Eos_putData.F90:207-224).  The gamma law (Gamma/Eos.F90:238-247)
+// or, with species mass fractions X, the Multigamma mixture
+// (Multigamma/Eos.F90: abar = 1/sum(X/A), gamc = 1 + 1/(abar sum(X/(gamma-1)/A)),
+// p = rho e (gamc - 1), T = e (gamc - 1) abar / R).
+struct EosOut { double pres, eint, ener, temp, game, gamc; };
 __device__ __forceinline__ EosOut eos_dens_ei(const DevParams& P, double dens, double vx, double vy, double vz,
-                                               double ener, double eint_in) {
+                                               double ener, double eint_in, const double* X = nullptr) {
   double ekin = 0.5 * (vx * vx + vy * vy + vz * vz);
   double eint = eint_in;
   if (ener - ekin > fmax2(P.smalle, P.eintSwitch * ekin)) eint = ener - ekin;
   eint = fmax2(eint, P.smalle);
   EosOut o;
-  o.pres = dens * eint * P.gam1inv;
-  o.temp = eint * P.ggprodinv * P.abar;
+  if (P.eos == 1 && X != nullptr) {
+    double abarInv = 0.0, rt = 0.0;
+#pragma unroll
+    for (int s = 0; s < 8; s++)
+      if (s < P.nspec) abarInv = abarInv + X[s] * P.mg_ainv[s];
+    const double abar = 1.e0 / abarInv;
+#pragma unroll
+    for (int s = 0; s < 8; s++)
+      if (s < P.nspec) rt = rt + (X[s] * P.mg_gm1[s]) * P.mg_ainv[s];
+    o.gamc = 1.0e0 + 1.0e0 / (rt * abar);
+    o.pres = dens * eint * (o.gamc - 1.e0);
+    o.temp = eint * (o.gamc - 1.e0) * abar / P.gasConst;
+  } else {
+    o.gamc = P.gamma;
+    o.pres = dens * eint * P.gam1inv;
+    o.temp = eint * P.ggprodinv * P.abar;
+  }
   o.eint = eint;
   o.ener = eint + ekin;
   o.game = o.pres / (eint * dens) + 1;

@@ -179,6 +179,27 @@ __global__ void __launch_bounds__(1024) k_flux(DevParams P, const BlockMeta* __r
   left[R_PRES] = fmax2(P.smallp, left[R_PRES]);
   right[R_DENS] = fmax2(P.smlrho, right[R_DENS]);
   right[R_PRES] = fmax2(P.smallp, right[R_PRES]);
+  if (NSM > 0 && P.nspec > 0) {
+    // species: clamp to [smallx, 1] and renormalise (Spark/Hydro.ini:227-235)
+    double sl = 0.0, sr = 0.0;
+#pragma unroll
+    for (int v = 0; v < NSM; v++) {
+      if (v < P.nspec) {
+        left[R_NUM + v] = fmax2(P.smallx, fmin2(1.0, left[R_NUM + v]));
+        right[R_NUM + v] = fmax2(P.smallx, fmin2(1.0, right[R_NUM + v]));
+        sl = sl + left[R_NUM + v];
+        sr = sr + right[R_NUM + v];
+      }
+    }
+    const double il = 1. / sl, ir = 1. / sr;
+#pragma unroll
+    for (int v = 0; v < NSM; v++) {
+      if (v < P.nspec) {
+        left[R_NUM + v] = left[R_NUM + v] * il;
+        right[R_NUM + v] = right[R_NUM + v] * ir;
+      }
+    }
+  }
   const bool inShock = (sm[(NRM + 5) * NT + tn] != 0.0) || (sm[(NRM + 5) * NT + tid] != 0.0);
   double F[NFM];
   int ierr;
@@ -327,16 +348,20 @@ __global__ void __launch_bounds__(256) k_update(DevParams P, const BlockMeta* __
     const double ener = fmax2(P.smalle, B[F_ENER] / dens);
     const double ekin = .5 * (vx * vx + vy * vy + vz * vz);
     const double eint = fmax2(P.smalle, ener - ekin);
+    double Xs[NSM > 0 ? NSM : 1];  // the mass fractions (species first) the EOS mixes
 #pragma unroll
     for (int v = 0; v < NSM; v++)
-      if (v < ns) us[(V_NPROP + v) * cs + c] = B[F_NUM + v] / dens;
-    EosOut e = eos_dens_ei(P, dens, vx, vy, vz, ener, eint);
+      if (v < ns) {
+        Xs[v] = B[F_NUM + v] / dens;
+        us[(V_NPROP + v) * cs + c] = Xs[v];
+      }
+    EosOut e = eos_dens_ei(P, dens, vx, vy, vz, ener, eint, NSM > 0 ? Xs : nullptr);
     us[V_DENS * cs + c] = dens;
     us[V_VELX * cs + c] = vx;
     us[V_VELY * cs + c] = vy;
     us[V_VELZ * cs + c] = vz;
     us[V_PRES * cs + c] = e.pres;
-    us[V_GAMC * cs + c] = P.gamma;
+    us[V_GAMC * cs + c] = e.gamc;
     us[V_EINT * cs + c] = e.eint;
     us[V_ENER * cs + c] = e.ener;
     if (final_stage) {
@@ -618,7 +643,7 @@ __global__ void __launch_bounds__(256) k_correct(DevParams P, const BlockMeta* _
         const double ekin = .5 * (vx * vx + vy * vy + vz * vz);
         const double eint = fmax2(P.smalle, en - ekin);
         for (int v = 0; v < ns; v++) X[v] = Us[F_NUM + v] / dens;
-        o = eos_dens_ei(P, dens, vx, vy, vz, en, eint);
+        o = eos_dens_ei(P, dens, vx, vy, vz, en, eint, X);
         ener = o.ener;
       }
     }
@@ -628,7 +653,7 @@ __global__ void __launch_bounds__(256) k_correct(DevParams P, const BlockMeta* _
     up[V_VELY * cs + cc] = vy;
     up[V_VELZ * cs + cc] = vz;
     up[V_PRES * cs + cc] = o.pres;
-    up[V_GAMC * cs + cc] = P.gamma;
+    up[V_GAMC * cs + cc] = o.gamc;
     up[V_EINT * cs + cc] = o.eint;
     up[V_ENER * cs + cc] = o.ener;
     up[V_TEMP * cs + cc] = o.temp;

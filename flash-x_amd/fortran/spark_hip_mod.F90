@@ -24,6 +24,7 @@ module spark_hip
   integer, parameter, public :: SPARK_GUARD_TELESCOPE = 0, SPARK_GUARD_REFILL = 1
   integer, parameter, public :: SPARK_OPT_GRAPH = 1, SPARK_OPT_AUTOTUNE = 2, SPARK_OPT_SAME_LEVEL_HALOS = 3
   integer, parameter, public :: SPARK_BC_OUTFLOW = 0, SPARK_BC_PERIODIC = 1, SPARK_BC_REFLECT = 2
+  integer, parameter, public :: SPARK_EOS_GAMMA = 0, SPARK_EOS_MULTIGAMMA = 1
 
   !! struct spark_params (same member order and types)
   type, bind(C), public :: spark_params_t
@@ -34,6 +35,8 @@ module spark_hip
      integer(c_int) :: use_flattening, use_hybrid_riemann, nscalars, use_gravity, flux_correct
      integer(c_int) :: form, guard_mode, nunk_vars
      integer(c_int) :: var_index(SPARK_NPROP + SPARK_MAX_SCALARS)
+     integer(c_int) :: nspecies, eos, reserved
+     real(c_double) :: species_a(SPARK_MAX_SCALARS), species_gamma(SPARK_MAX_SCALARS)
   end type spark_params_t
 
   !! struct spark_kernel_stat (per-kernel HIP-event timing)

@@ -26,6 +26,7 @@ BC = {"outflow": 0, "periodic": 1, "reflect": 2}
 RIEMANN = {"hllc": 0, "hll": 1}
 FORM = {"plain_fp64": 0, "fp32_flux": 1, "batched_fp64": 2}
 GUARD = {"telescope": 0, "refill": 1}
+EOS = {"gamma": 0, "multigamma": 1}
 
 # canonical variable order of the device state (alphabetical unk order of
 # Spark + Gamma, then mass scalars)
@@ -45,6 +46,8 @@ class SparkParams(ctypes.Structure):
         ("nscalars", ctypes.c_int), ("use_gravity", ctypes.c_int), ("flux_correct", ctypes.c_int),
         ("form", ctypes.c_int), ("guard_mode", ctypes.c_int), ("nunk_vars", ctypes.c_int),
         ("var_index", ctypes.c_int * (SPARK_NPROP + SPARK_MAX_SCALARS)),
+        ("nspecies", ctypes.c_int), ("eos", ctypes.c_int), ("reserved", ctypes.c_int),
+        ("species_a", ctypes.c_double * SPARK_MAX_SCALARS), ("species_gamma", ctypes.c_double * SPARK_MAX_SCALARS),
     ]
 
 
@@ -154,6 +157,13 @@ def make_params(cfg: dict) -> SparkParams:
             for i, x in enumerate(v):
                 p.var_index[i] = int(x)
             continue
+        if k in ("species_a", "species_gamma"):
+            arr = getattr(p, k)
+            for i, x in enumerate(v):
+                arr[i] = float(x)
+            continue
+        if k == "eos":
+            v = EOS[v] if isinstance(v, str) else v
         if not hasattr(p, k):
             raise KeyError(k)
         setattr(p, k, v)

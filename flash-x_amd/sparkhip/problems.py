@@ -115,6 +115,58 @@ def sod2d(nbx=16, nby=16, n=16, nguard=10, gamma=1.4, **kw) -> Problem:
     return Problem("sod2d", cfg, blocks, arrays, ug=(nbx, nby, 1))
 
 
+def mixture_thermo(X, species_a, species_gamma):
+    """abar and gamc of a Multigamma mixture of mass fractions X (..., ns):
+    c_v adds over the species (c_v,s = R / (A_s (gamma_s - 1))), so
+    gamc = 1 + (R / abar) / c_v with 1/abar = sum X_s / A_s."""
+    a = np.asarray(species_a, dtype=float)
+    g = np.asarray(species_gamma, dtype=float)
+    abar = 1.0 / np.sum(X / a, axis=-1)
+    cv_over_r = np.sum(X / (a * (g - 1.0)), axis=-1)
+    return abar, 1.0 + 1.0 / (abar * cv_over_r)
+
+
+def sod2d_species(nbx=4, nby=4, n=16, nguard=10, species_a=(1.0, 4.0), species_gamma=(1.4, 5.0 / 3.0),
+                  width=0.02, **kw) -> Problem:
+    """Two-gas shock tube on [0,1]^2 with the Multigamma EOS: the Sod states
+    (rho, p) = (1, 1) | (0.125, 0.1) across x + y = 1, gas 1 (A = 1, gamma
+    1.4) on the left and gas 2 (A = 4, gamma 5/3) on the right, the mass
+    fractions blended by a tanh of the given width; gamc, eint and T follow
+    the mixture."""
+    ns = len(species_a)
+    cfg = base_cfg(ndim=2, nguard=nguard, gamma=float(species_gamma[0]), nscalars=ns, nspecies=ns, eos="multigamma",
+                   species_a=tuple(species_a), species_gamma=tuple(species_gamma), **kw)
+    dx = 1.0 / (nbx * n)
+    dy = 1.0 / (nby * n)
+    blocks, arrays = [], []
+    nunk = SPARK_NPROP + cfg["nscalars"]
+    for bj in range(nby):
+        for bi in range(nbx):
+            b = dict(nx=n, ny=n, nz=1, level=1, dx=dx, dy=dy, dz=1.0)
+            NZ, NY, NX = gc_shape(cfg, b)
+            Z, Y, X = _centers(cfg, b, (bi * n * dx, bj * n * dy, 0.0), NZ, NY, NX)
+            s = (X + Y - 1.0) / np.sqrt(2.0)
+            right = 0.5 * (1.0 + np.tanh(s / width))
+            dens = 1.0 + (0.125 - 1.0) * right
+            pres = 1.0 + (0.1 - 1.0) * right
+            Xs = np.stack([1.0 - right, right] + [np.zeros_like(right)] * (ns - 2), axis=-1)
+            abar, gamc = mixture_thermo(Xs, species_a, species_gamma)
+            U = np.zeros((NZ, NY, NX, nunk))
+            eint = pres / ((gamc - 1.0) * dens)
+            U[..., VI["dens"]] = dens
+            U[..., VI["pres"]] = pres
+            U[..., VI["eint"]] = eint
+            U[..., VI["ener"]] = eint
+            U[..., VI["gamc"]] = gamc
+            U[..., VI["game"]] = pres / (eint * dens) + 1.0
+            U[..., VI["temp"]] = eint * (gamc - 1.0) * abar / GAS_CONST
+            for q in range(ns):
+                U[..., SPARK_NPROP + q] = Xs[..., q]
+            blocks.append(b)
+            arrays.append(U)
+    return Problem("sod2d_species", cfg, blocks, arrays, ug=(nbx, nby, 1))
+
+
 def sedov3d(nb=8, n=16, nguard=6, gamma=1.4, E=1.0, r_cells=3.5, p_amb=1e-5, rho=1.0, **kw) -> Problem:
     """3D Sedov blast on [0,1]^3: E deposited as pressure in the cells whose
     centres lie within r_cells cell widths of the domain centre (p scaled so

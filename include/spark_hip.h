@@ -44,6 +44,9 @@ enum spark_form { SPARK_FORM_PLAIN_FP64 = 0, SPARK_FORM_FP32_FLUX = 1, SPARK_FOR
  * (the reference's own scheme, Spark/Hydro.ini:19,37), or refilling the
  * guards of a uniform block grid on the device between stages */
 enum spark_guard_mode { SPARK_GUARD_TELESCOPE = 0, SPARK_GUARD_REFILL = 1 };
+/* equation of state: the gamma law (Eos/EosMain/Gamma) or a mixture of
+ * gamma-law species (Eos/EosMain/Multigamma/Eos.F90) */
+enum spark_eos { SPARK_EOS_GAMMA = 0, SPARK_EOS_MULTIGAMMA = 1 };
 
 /* Runtime parameters (Hydro_init.F90 / Eos_init.F90 subset).
  * Replaces the RuntimeParameters_get calls of Spark/Hydro_init.F90:60-104
@@ -78,6 +81,18 @@ typedef struct spark_params {
   /* 0-based unk slot of dens eint ener gamc game pres shok temp velx vely
    * velz, then the mass scalars (the *_VAR map of Simulation.h minus 1) */
   int var_index[SPARK_NPROP + SPARK_MAX_SCALARS];
+  /* Species: the first nspecies of the nscalars mass scalars are species
+   * mass fractions (SPECIES_BEGIN precedes MASS_SCALARS_BEGIN in unk).  Their
+   * face states are clamped to [smallx, 1] and renormalised
+   * (Spark/Hydro.ini:227-235).  eos = SPARK_EOS_MULTIGAMMA mixes them with
+   * per-species gamma and mass number A (Multigamma/Eos.F90, ORIGINAL_GAMC_
+   * AVERAGE; eos_initMgamma.F90:12-18); needs nspecies >= 1.  Species and the
+   * Multigamma EOS are implemented in the plain FP64 form. */
+  int nspecies;
+  int eos;             /* enum spark_eos                                    */
+  int reserved;        /* 0; keeps the arrays below 8-byte aligned with no implicit padding */
+  double species_a[SPARK_MAX_SCALARS];      /* A of each species           */
+  double species_gamma[SPARK_MAX_SCALARS];  /* gamma of each species       */
 } spark_params;
 
 /* One LEAF block (tileDesc%limits, %deltas, %level). */

@@ -26,6 +26,8 @@ class OrcParams(ctypes.Structure):
         ("ndim", ctypes.c_int), ("nguard", ctypes.c_int), ("nstencil", ctypes.c_int), ("rk_order", ctypes.c_int),
         ("recon", ctypes.c_int), ("riemann", ctypes.c_int), ("use_flattening", ctypes.c_int),
         ("use_hybrid_riemann", ctypes.c_int), ("nscalars", ctypes.c_int), ("use_gravity", ctypes.c_int),
+        ("nspecies", ctypes.c_int), ("eos", ctypes.c_int),
+        ("species_a", ctypes.c_double * 8), ("species_gamma", ctypes.c_double * 8),
     ]
 
 
@@ -86,6 +88,17 @@ def params(cfg: dict) -> OrcParams:
             g = cfg.get("grav", (0.0, 0.0, 0.0))
             for i in range(3):
                 p.grav[i] = float(g[i])
+            continue
+        if name in ("species_a", "species_gamma"):
+            arr = getattr(p, name)
+            for i, x in enumerate(cfg.get(name, ())):
+                arr[i] = float(x)
+            continue
+        if name in ("nspecies", "eos"):
+            v = cfg.get(name, 0)
+            if name == "eos" and isinstance(v, str):
+                v = {"gamma": 0, "multigamma": 1}[v]
+            setattr(p, name, v)
             continue
         v = cfg[name]
         if name == "recon" and isinstance(v, str):

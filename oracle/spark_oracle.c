@@ -433,11 +433,23 @@ void orc_get_face_flux(const orc_params* P, const orc_block* b, const double* S,
           for (int v = 0; v < nrec; v++) left[v] = uP[v];
           orc_recon(P, nrec, pen, i1, plen, pflat[i1], uP, uM);
           for (int v = 0; v < nrec; v++) right[v] = uM[v];
-          /* hy_ensurePhysicalState, Spark/Hydro.ini:222-235 (no species) */
+          /* hy_ensurePhysicalState, Spark/Hydro.ini:222-235 */
           left[HY_DENS] = fmax2(P->smlrho, left[HY_DENS]);
           left[HY_PRES] = fmax2(P->smallp, left[HY_PRES]);
           right[HY_DENS] = fmax2(P->smlrho, right[HY_DENS]);
           right[HY_PRES] = fmax2(P->smallp, right[HY_PRES]);
+          if (P->nspecies > 0) {
+            /* limit and renormalise the species (:227-235) */
+            double* st[2] = {left, right};
+            for (int q = 0; q < 2; q++) {
+              double* spc = st[q] + HY_NUM_VARS;
+              double sum = 0.0;
+              for (int s = 0; s < P->nspecies; s++) spc[s] = fmax2(P->smallx, fmin2(1.0, spc[s]));
+              for (int s = 0; s < P->nspecies; s++) sum = sum + spc[s];
+              const double spcSumInv = 1. / sum;
+              for (int s = 0; s < P->nspecies; s++) spc[s] = spc[s] * spcSumInv;
+            }
+          }
           int inShock = (pshck[i1 - 1] != 0.0) || (pshck[i1] != 0.0);
           int ierr;
           if (P->riemann == ORC_RIEMANN_HLL) ierr = orc_riemann_hll(P, dir, left, right, F);
@@ -484,24 +496,45 @@ void orc_get_face_flux(const orc_params* P, const orc_block* b, const double* S,
 }
 
 /* ------------------------------------------------------------------ */
-/* Gamma EOS, MODE_DENS_EI, on one cell of an unk-layout state:
+/* EOS (Gamma, or Multigamma over the species), MODE_DENS_EI, on one cell of an unk-layout state:
  * Eos_getData.F90:114-169 -> Gamma/Eos.F90:238-247 -> Eos_putData.F90:207-224.
  * full=0 skips the TEMP/GAME outputs (used when only the stage chain is
  * needed; both are pure outputs that nothing on the path reads). */
 void orc_eos_dens_ei_cell(const orc_params* P, double* v, int full) {
-  double eos_gammam1 = 1.0 / (P->gamma - 1.0);  /* Gamma/Eos_init.F90:86 */
-  double ggprod = eos_gammam1 * P->gasConst;
-  double ggprodinv = 1. / ggprod;
-  double gam1inv = 1. / eos_gammam1;
   double ekin = 0.5 * (v[ORC_VELX] * v[ORC_VELX] + v[ORC_VELY] * v[ORC_VELY] + v[ORC_VELZ] * v[ORC_VELZ]);
   double eint = v[ORC_EINT];
   if (v[ORC_ENER] - ekin > fmax2(P->smalle, P->eintSwitch * ekin)) eint = v[ORC_ENER] - ekin;
   eint = fmax2(eint, P->smalle);
   double dens = v[ORC_DENS];
-  double pres = dens * eint * gam1inv;
-  double temp = eint * ggprodinv * P->abar;
+  double pres, temp, gamc;
+  if (P->eos == 1) {
+    /* Multigamma/Eos.F90 (ORIGINAL_GAMC_AVERAGE): abar from
+     * Multispecies_getSumInv(A, X) (Multispecies_getSumInv.F90:143-152),
+     * rt from the weights X/(gamma-1) (eos_initMgamma.F90:15), then
+     * MODE_DENS_EI */
+    const double* X = v + ORC_NPROP;
+    double abarInv = 0.0, rt = 0.0;
+    for (int s = 0; s < P->nspecies; s++) abarInv = abarInv + X[s] * (1.e0 / P->species_a[s]);
+    const double abar = 1.e0 / abarInv;
+    for (int s = 0; s < P->nspecies; s++) {
+      const double gammam1j = 1. / (P->species_gamma[s] - 1.);
+      const double weight = X[s] * gammam1j;
+      rt = rt + weight * (1.e0 / P->species_a[s]);
+    }
+    gamc = 1.0e0 + 1.0e0 / (rt * abar);
+    pres = dens * eint * (gamc - 1.e0);
+    temp = eint * (gamc - 1.e0) * abar / P->gasConst;
+  } else {
+    double eos_gammam1 = 1.0 / (P->gamma - 1.0);  /* Gamma/Eos_init.F90:86 */
+    double ggprod = eos_gammam1 * P->gasConst;
+    double ggprodinv = 1. / ggprod;
+    double gam1inv = 1. / eos_gammam1;
+    gamc = P->gamma;
+    pres = dens * eint * gam1inv;
+    temp = eint * ggprodinv * P->abar;
+  }
   v[ORC_PRES] = pres;
-  v[ORC_GAMC] = P->gamma;
+  v[ORC_GAMC] = gamc;
   v[ORC_EINT] = eint;
   v[ORC_ENER] = eint + ekin;
   if (full) {

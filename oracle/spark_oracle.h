@@ -43,6 +43,12 @@ typedef struct orc_params {
   int use_flattening, use_hybrid_riemann;
   int nscalars;
   int use_gravity;
+  /* species: the first nspecies scalars (Spark/Hydro.ini:227-235); eos 1 =
+   * Multigamma (Eos/EosMain/Multigamma/Eos.F90) with per-species A, gamma */
+  int nspecies;
+  int eos;
+  double species_a[8];
+  double species_gamma[8];
 } orc_params;
 
 /* One block: interior sizes (1 for unused dims) and its deltas. */

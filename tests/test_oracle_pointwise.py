@@ -227,3 +227,45 @@ def test_eos_internal_energy_source_and_floor():
     v[2] = 0.5 * (0.09 + 0.16 + 0.25) - 1.0                  # E - ekin < 0
     out = pyoracle.eos_cell(CFG, v)
     assert out[1] == CFG["smalle"]
+
+
+def mg_cfg(a, g):
+    return dict(CFG, nscalars=len(a), nspecies=len(a), eos="multigamma", species_a=a, species_gamma=g)
+
+
+def test_multigamma_one_species_is_the_gamma_law():
+    """A single species of mass number abar and ratio gamma is the gamma law
+    (pressure, temperature, gamc, game to a few ulp)."""
+    ref = pyoracle.eos_cell(dict(CFG, abar=2.5), state())
+    v = state(ns=1)
+    v[11] = 1.0
+    out = pyoracle.eos_cell(mg_cfg((2.5,), (G,)), v)
+    for i in (1, 2, 3, 4, 5, 7):
+        np.testing.assert_allclose(out[i], ref[i], rtol=4e-15)
+
+
+def test_multigamma_mixture_thermodynamics():
+    """Mixture of ideal gases: c_v adds over the species by mass,
+    c_v = sum X_s R / (A_s (gamma_s - 1)), the mixture obeys p = rho R T / abar
+    with 1/abar = sum X_s / A_s, and gamc = 1 + (R / abar) / c_v.  Two
+    identical species behave as that one gas whatever the split."""
+    rng = np.random.default_rng(11)
+    R = CFG["gasConst"]
+    a, g = (1.0, 4.0, 12.0), (1.4, 5.0 / 3.0, 1.3)
+    for _ in range(40):
+        X = rng.dirichlet(np.ones(3))
+        v = state(d=rng.uniform(0.1, 5.0), e=rng.uniform(0.1, 5.0), ns=3)
+        v[11:14] = X
+        out = pyoracle.eos_cell(mg_cfg(a, g), v)
+        abar = 1.0 / np.sum(X / np.array(a))
+        cv = np.sum(X * R / (np.array(a) * (np.array(g) - 1.0)))
+        np.testing.assert_allclose(out[3], 1.0 + (R / abar) / cv, rtol=1e-14)
+        np.testing.assert_allclose(out[5], out[0] * R * out[7] / abar, rtol=1e-14)
+        np.testing.assert_allclose(out[4], out[5] / (out[1] * out[0]) + 1.0, rtol=1e-15)
+    v = state(ns=2)
+    one = pyoracle.eos_cell(mg_cfg((2.0,), (1.5,)), np.concatenate([state(), [1.0]]))
+    for split in (0.0, 0.3, 0.5, 1.0):
+        v[11], v[12] = split, 1.0 - split
+        two = pyoracle.eos_cell(mg_cfg((2.0, 2.0), (1.5, 1.5)), v)
+        for i in (3, 5, 7):
+            np.testing.assert_allclose(two[i], one[i], rtol=1e-15)

@@ -276,3 +276,24 @@ def test_sedov_against_the_reference_analytic_profile():
     assert abs(e1 - e0) / e0 < 1e-12
     assert abs(m1 - m0) / m0 < 1e-9
 
+
+
+def test_species_stay_normalised_and_gamc_follows_the_mixture():
+    """Two-gas shock tube with the Multigamma EOS: the face clamp and
+    renormalisation of the species (Spark/Hydro.ini:227-235) keep the mass
+    fractions of every cell summing to 1 to rounding over steps (each
+    species flux is X_face * mass flux, so sum X_face = 1 makes the species
+    fluxes add up to the mass flux); gamc and p stay those of the cell's
+    mixture."""
+    prob = problems.sod2d_species(nbx=2, nby=2, n=16)
+    arr = [a.copy() for a in prob.arrays]
+    g = prob.cfg["nguard"]
+    dts, nerr = pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, arr, 6)
+    assert nerr == 0
+    for a in arr:
+        c = a[:, g:-g, g:-g]
+        X = c[..., 11:13]
+        assert np.max(np.abs(X.sum(axis=-1) - 1.0)) < 1e-14
+        abar, gamc = problems.mixture_thermo(X, prob.cfg["species_a"], prob.cfg["species_gamma"])
+        np.testing.assert_allclose(c[..., 3], gamc, rtol=1e-14)
+        np.testing.assert_allclose(c[..., 5], c[..., 0] * c[..., 1] * (gamc - 1.0), rtol=1e-13)
