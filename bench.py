@@ -15,6 +15,10 @@ Workloads (--workload):
                the levels and fine/coarse flux correction every step,
                10 guard cells (telescoping RK3).  FP32-flux accuracy after
                --accuracy-steps (default 200) against the plain FP64 form.
+  species2d    2D two-gas shock tube (256^2 zones, 16^2 blocks, 10 guard
+               cells): A 1 / gamma 1.4 against A 4 / gamma 5/3 across
+               x + y = 1 with the Sod states, Multigamma EOS, species clamped
+               and renormalised at faces; the plain FP64 form only.
   batched4096  4096 independent 3D blocks of 8^3 / 16^3 / 32^3 cells
                (p = 0.25 / 0.625 / 0.125), levels 0-4 (dx = 2^-l / 16),
                smooth random fields plus a Mach 2-5 planar shock in every
@@ -52,6 +56,9 @@ WORKLOADS = {
               "level, 256^3-equivalent), rho 2 over 1, g = -1 source, hydrostatic p (2.5 at the interface), "
               "seeded interface modes of amplitude 0.01, periodic x/y and reflecting z, device guard fill over levels "
               "(limited linear prolongation) and fine/coarse flux correction every step, 10 guard cells, telescoping RK3, WENO5-Z + HLLC, CFL 0.4, gamma 1.4",
+    "species2d": "2D two-gas shock tube, 16x16 blocks of 16^2 (256^2 zones), 10 guard cells, telescoping RK3, "
+                 "gas 1 (A 1, gamma 1.4) | gas 2 (A 4, gamma 5/3) across x + y = 1 with the Sod states, Multigamma "
+                 "EOS, species clamped and renormalised at faces, WENO5-Z + HLLC, CFL 0.8 (plain form only)",
     "batched4096": "4096 independent 3D blocks of 8^3/16^3/32^3 (p .25/.625/.125, seeds 20261202/20261203), "
                    "levels 0-4, Fourier fields + Mach 2-5 shocks every 16th block, 10 guard cells, telescoping RK3, "
                    "WENO5-Z + HLLC, CFL 0.8, gamma 1.4",
@@ -74,13 +81,13 @@ def parse():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration) and exit")
     a = ap.parse_args()
-    big = a.workload != "sedov128"
+    big = a.workload in ("batched4096", "rt_amr")
     if a.steps is None:
         a.steps = 10 if big else 50
     if a.warmup is None:
         a.warmup = 2 if big else 5
     if a.accuracy_steps is None:
-        a.accuracy_steps = {"sedov128": 0, "batched4096": 100, "rt_amr": 200}[a.workload]
+        a.accuracy_steps = {"sedov128": 0, "species2d": 0, "batched4096": 100, "rt_amr": 200}[a.workload]
     a.profile_steps = max(1, a.profile_steps)
     return a
 
@@ -104,6 +111,8 @@ def make_problem(workload):
         return problems.batched(nblocks=4096, nguard=10)
     if workload == "rt_amr":
         return problems.rt_amr()
+    if workload == "species2d":
+        return problems.sod2d_species(nbx=16, nby=16, n=16)
     return problems.sedov3d(nb=8, n=16, nguard=6, guard_mode="refill", cfl=SEDOV_CFL)
 
 
@@ -115,7 +124,11 @@ def cpu_baseline(workload, target_s: float):
     from sparkhip import problems
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    if workload == "batched4096":
+    if workload == "species2d":
+        prob = problems.sod2d_species(nbx=16, nby=16, n=16)
+        run = lambda n: pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, prob.arrays, n, nthreads=threads)
+        what = "the same two-gas shock tube (256^2 zones)"
+    elif workload == "batched4096":
         prob = problems.batched(nblocks=256, nguard=10)
         run = lambda n: pyoracle.batch_advance(prob.cfg, prob.blocks, prob.arrays, n, nthreads=threads)
         what = f"the first 256 of the 4096 mixed blocks ({prob.zones} zones)"
@@ -242,7 +255,8 @@ def main():
     forms = {"plain_fp64": {"zone_updates_per_s": round(value, 1), "ms_per_step": round(el / args.steps * 1e3, 4),
                             "kernel_ms_per_step": {x["name"]: round(x["ms"] / args.profile_steps, 4)
                                                    for x in sorted(stats, key=lambda x: -x["ms"])}}}
-    for form in [f for f in args.forms.split(",") if f]:
+    batched_ok = prob.cfg.get("nspecies", 0) == 0 and prob.cfg.get("eos", "gamma") in ("gamma", 0)
+    for form in [f for f in args.forms.split(",") if f and batched_ok]:
         bf, elf = timed(form)
         bf.profile(True)
         bf.advance(args.profile_steps)
@@ -265,7 +279,7 @@ def main():
         note(f"{form} {entry['ms_per_step']} ms/step")
 
     accuracy = None
-    if args.accuracy_steps > 0:
+    if args.accuracy_steps > 0 and batched_ok:
         # FP32-flux against the plain FP64 form (bit-exact with the FP64
         # oracle) after the same number of steps from the same state
         res = {}
