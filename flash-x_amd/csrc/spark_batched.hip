@@ -796,4 +796,18 @@ hipError_t FUSED_NAME(fused_occupancy, FUSED_SFX)(int ns, size_t lds, int* block
       blocks_per_cu, FUSED_NAME(k_fused, FUSED_SFX)<8, RECON_WENO5, RIEMANN_HLLC>, FUSED_THREADS, lds);
 }
 
+// static LDS of the kernel (work-item index, neighbour table): a launch's
+// dynamic LDS must leave room for it under the per-workgroup limit
+size_t FUSED_NAME(fused_static_lds, FUSED_SFX)(int ns) {
+  hipFuncAttributes fa;
+  hipError_t e = ns == 0
+                     ? hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(
+                                                     FUSED_NAME(k_fused, FUSED_SFX)<0, RECON_WENO5, RIEMANN_HLLC>))
+                     : hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(
+                                                     FUSED_NAME(k_fused, FUSED_SFX)<8, RECON_WENO5, RIEMANN_HLLC>));
+  const size_t own = sizeof(int) + 27 * sizeof(int64_t);
+  if (e != hipSuccess) return 1024;  // a safe upper bound of the kernel's own variables
+  return fa.sharedSizeBytes > own ? fa.sharedSizeBytes : own;
+}
+
 }  // namespace spark

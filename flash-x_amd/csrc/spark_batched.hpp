@@ -51,9 +51,11 @@ size_t fused_lds_bytes_f32(int ns, int tx, int ty);
 hipError_t fused_reset_f32(hipStream_t st, int* counters);
 hipError_t fused_stage_f32(const FusedArgs& A, int grid, size_t lds, hipStream_t st);
 hipError_t fused_occupancy_f32(int ns, size_t lds, int* blocks_per_cu);
+size_t fused_static_lds_f32(int ns);  // the kernel's own __shared__ bytes beside the dynamic LDS
 size_t fused_lds_bytes_f64(int ns, int tx, int ty);
 hipError_t fused_reset_f64(hipStream_t st, int* counters);
 hipError_t fused_stage_f64(const FusedArgs& A, int grid, size_t lds, hipStream_t st);
 hipError_t fused_occupancy_f64(int ns, size_t lds, int* blocks_per_cu);
+size_t fused_static_lds_f64(int ns);
 
 }  // namespace spark

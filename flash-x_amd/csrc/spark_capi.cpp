@@ -938,7 +938,9 @@ int spark_batch_create(const spark_params* p, int nblocks, const spark_block_des
     int lim = 0;
     HIPCHK(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, g_device));
     HIPCHK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, g_device));
-    b->lds_limit = (size_t)lim;
+    // the dynamic LDS of a launch plus the kernel's own __shared__ variables
+    const size_t own = b->fused64 ? spark::fused_static_lds_f64(b->p.nscalars) : spark::fused_static_lds_f32(b->p.nscalars);
+    b->lds_limit = (size_t)lim > own + 16 ? (size_t)lim - own - 16 : 0;
     if (fused_lds(b, 4, 4) > b->lds_limit) {
       const size_t need = fused_lds(b, 4, 4), have = b->lds_limit;
       free_batch(b);
