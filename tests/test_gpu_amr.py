@@ -67,6 +67,28 @@ def test_guard_fill_matches_oracle():
     assert nd == 0
 
 
+@pytest.mark.parametrize("bc", [(0, 0, 0), (1, 2, 0), (2, 1, 1)])
+def test_guard_fill_boundary_codes_match_oracle(bc):
+    """Device guard fill over the 3-level hierarchy with each boundary code
+    (outflow, periodic, reflecting) on some dimension."""
+    prob = three_level(nscalars=2)
+    b = capi.Batch(prob.cfg, prob.blocks)
+    try:
+        b.pack(prob.arrays)
+        b.set_amr_grid(prob.info["nb0"], prob.info["origins"], bc)
+        b.fill_guardcells()
+        out = [a.copy() for a in prob.arrays]
+        b.unpack(out)
+    finally:
+        b.close()
+    ref = [a.copy() for a in prob.arrays]
+    pyoracle.amr_fill(prob.cfg, prob.blocks, prob.info["nb0"], prob.info["origins"], ref, depth=prob.cfg["nguard"],
+                      bc=bc)
+    nd = sum(int(np.count_nonzero(x != y)) for x, y in zip(out, ref))
+    print(f"bc {bc}: differing values {nd}", flush=True)
+    assert nd == 0
+
+
 @pytest.mark.parametrize("flux_correct", [1, 0])
 def test_plain_fp64_amr_steps_match_oracle(flux_correct):
     prob = two_level(nscalars=8, flux_correct=flux_correct)
