@@ -163,12 +163,26 @@ struct RParams {
   Real gamma, cvisc, smlrho, smallp, tiny;
 };
 
+// Riemann template value of the fused kernel beside RIEMANN_HLLC / _HLL:
+// HLLC with the hybrid switch to HLL at faces next to a shock-flagged cell
+// (use_hybridRiemann, hy_rk_getFaceFlux.F90-mc:258-274)
+constexpr int RIEMANN_HLLC_HYBRID = 2;
+
+// SHOK of the two cells on either side of a face, from the step's input
+// state; the cells are at most one layer outside the stage region, inside
+// the block's own box, where the step's shock detection has written SHOK
+__device__ __forceinline__ bool face_in_shock(const double* __restrict__ U0, const BlockMeta& m, int64_t lo,
+                                              int64_t hi) {
+  const double* sh = U0 + m.base + (int64_t)V_SHOK * m.cstride;
+  return sh[lo] != 0.0 || sh[hi] != 0.0;
+}
+
 // hydro face flux from the left/right face states (ensurePhysicalState,
 // Riemann solve, artificial viscosity; Hydro.ini:222-235,285-301).  c*
 // hold the two cells' centred dens, velx, vely, velz, rhoe.
 template <int DIR, int RIEMANN>
 __device__ __forceinline__ void hydro_face(const RParams& rp, Real* L, Real* R, const Real* cl, const Real* cr, Real* F,
-                                           int* err) {
+                                           int* err, bool inShock) {
   L[R_DENS] = fmax2(rp.smlrho, L[R_DENS]);
   L[R_PRES] = fmax2(rp.smallp, L[R_PRES]);
   R[R_DENS] = fmax2(rp.smlrho, R[R_DENS]);
@@ -176,7 +190,7 @@ __device__ __forceinline__ void hydro_face(const RParams& rp, Real* L, Real* R, 
   if (!kF64) { L[R_GAMC] = rp.gamma; R[R_GAMC] = rp.gamma; }
   int ierr;
   if (RIEMANN == RIEMANN_HLL) ierr = hll<DIR, Real>(L, R, F);
-  else ierr = hllc<DIR, Real>(L, R, false, false, rp.tiny, F);
+  else ierr = hllc<DIR, Real>(L, R, RIEMANN == RIEMANN_HLLC_HYBRID, inShock, rp.tiny, F);
   if (ierr) {
 #pragma unroll
     for (int n = 0; n < F_NUM; n++) F[n] = Real(0.);
@@ -234,7 +248,12 @@ __device__ __forceinline__ void xy_face(const FusedArgs& A, const BlockMeta& m, 
   const int pc = (fj + 3) * PW + (fi + 3);
   centred(plane, psz, pc - (DIR == 0 ? 1 : PW), cl);
   centred(plane, psz, pc, cr);
-  hydro_face<DIR, RIEMANN>(rp, Ls, Rs, cl, cr, F, A.err);
+  bool shk = false;
+  if (RIEMANN == RIEMANN_HLLC_HYBRID) {
+    const int64_t hi = bcix(m, I.x0 + fi, I.y0 + fj, k);
+    shk = face_in_shock(A.U0, m, hi - (DIR == 0 ? 1 : m.NX), hi);
+  }
+  hydro_face<DIR, RIEMANN>(rp, Ls, Rs, cl, cr, F, A.err, shk);
 #pragma unroll
   for (int n = 0; n < F_NUM; n++) fo[n * nf + w] = F[n];
   mo[w] = F[F_MASS];
@@ -497,7 +516,12 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
               if (!kF64) { Ls[R_GAMC] = Real(0.); Rs[R_GAMC] = Real(0.); }
               cl[0] = win[0][1]; cl[1] = win[1][1]; cl[2] = win[2][1]; cl[3] = win[3][1]; cl[4] = win[SL_RHOE][1];
               cr[0] = win[0][2]; cr[1] = win[1][2]; cr[2] = win[2][2]; cr[3] = win[3][2]; cr[4] = win[SL_RHOE][2];
-              hydro_face<2, RIEMANN>(rp, Ls, Rs, cl, cr, Hz, A.err);
+              bool shk = false;
+              if (RIEMANN == RIEMANN_HLLC_HYBRID) {
+                const int64_t hi = colc + (int64_t)k * sz;
+                shk = face_in_shock(A.U0, m, hi - sz, hi);
+              }
+              hydro_face<2, RIEMANN>(rp, Ls, Rs, cl, cr, Hz, A.err, shk);
               if (regs && colin && (k == g || k == g + m.nz)) {
                 const int64_t tot = (int64_t)m.nx * m.ny * (m.nz + 1);
                 const int64_t o = m.bbase + (ci - g) + (int64_t)m.nx * ((cj - g) + (int64_t)m.ny * (k - g));
@@ -760,17 +784,19 @@ hipError_t launch_t(const FusedArgs& A, int grid, size_t lds, hipStream_t st) {
 
 template <int NSM>
 hipError_t launch_r(const FusedArgs& A, int grid, size_t lds, hipStream_t st) {
-  const bool hll = A.P.riemann == RIEMANN_HLL;
+  // the hybrid switch only changes HLLC (HLL is what it switches to)
+  const int rs = A.P.riemann == RIEMANN_HLL ? RIEMANN_HLL : A.P.hybrid ? RIEMANN_HLLC_HYBRID : RIEMANN_HLLC;
+#define FUSED_R(RC)                                                                   \
+  (rs == RIEMANN_HLL    ? launch_t<NSM, RC, RIEMANN_HLL>(A, grid, lds, st)           \
+   : rs == RIEMANN_HLLC ? launch_t<NSM, RC, RIEMANN_HLLC>(A, grid, lds, st)          \
+                        : launch_t<NSM, RC, RIEMANN_HLLC_HYBRID>(A, grid, lds, st))
   switch (A.P.recon) {
-    case RECON_MP5: return hll ? launch_t<NSM, RECON_MP5, RIEMANN_HLL>(A, grid, lds, st)
-                               : launch_t<NSM, RECON_MP5, RIEMANN_HLLC>(A, grid, lds, st);
-    case RECON_PLM: return hll ? launch_t<NSM, RECON_PLM, RIEMANN_HLL>(A, grid, lds, st)
-                               : launch_t<NSM, RECON_PLM, RIEMANN_HLLC>(A, grid, lds, st);
-    case RECON_FOG: return hll ? launch_t<NSM, RECON_FOG, RIEMANN_HLL>(A, grid, lds, st)
-                               : launch_t<NSM, RECON_FOG, RIEMANN_HLLC>(A, grid, lds, st);
-    default: return hll ? launch_t<NSM, RECON_WENO5, RIEMANN_HLL>(A, grid, lds, st)
-                        : launch_t<NSM, RECON_WENO5, RIEMANN_HLLC>(A, grid, lds, st);
+    case RECON_MP5: return FUSED_R(RECON_MP5);
+    case RECON_PLM: return FUSED_R(RECON_PLM);
+    case RECON_FOG: return FUSED_R(RECON_FOG);
+    default: return FUSED_R(RECON_WENO5);
   }
+#undef FUSED_R
 }
 
 }  // namespace

@@ -237,7 +237,8 @@ int run_fill(spark_batch* b, double* pool, bool full, int depth) {
   }
   if (b->amr && !full && amr_remap_on(b)) {
     // remapping blocks read no guard cell of their own in the stages: fill
-    // only what shockDetect of their interior reads
+    // only what shockDetect of their interior reads (and of the layer
+    // around it, which the hybrid Riemann solver reads SHOK of)
     if (b->n_tele > 0) {
       L.blist = b->tele_ids_d;
       L.nblk = b->n_tele;
@@ -249,7 +250,7 @@ int run_fill(spark_batch* b, double* pool, bool full, int depth) {
       vs.v[vs.n++] = v;
     L.blist = b->remap_ids_d;
     L.nblk = b->amr_remap_count;
-    HIPCHK(spark::amr_fill(L, b->amr_g, pool, vs, std::min(depth, 1)));
+    HIPCHK(spark::amr_fill(L, b->amr_g, pool, vs, std::min(depth, b->p.use_hybrid_riemann ? 2 : 1)));
   } else if (b->amr) {
     HIPCHK(spark::amr_fill(L, b->amr_g, pool, vl, depth));
   } else {
@@ -306,20 +307,24 @@ int run_amr_correct(spark_batch* b) {
 
 // first-stage guard reach: stencil extent of stage 1 beyond the interior
 int stage1_reach(const spark_batch* b) {
-  // the remapping batched forms only need the layer shockDetect reads
-  if (fused_remap(b)) return 1;
+  // the remapping batched forms only need the layers shockDetect reads: one
+  // for the interior, two when the hybrid solver reads SHOK one layer out
+  if (fused_remap(b)) return b->p.use_hybrid_riemann ? 2 : 1;
   return (b->p.guard_mode == SPARK_GUARD_REFILL ? 0 : b->limits_array[0]) + b->p.nstencil;
 }
 
 int run_step_begin(spark_batch* b) {
-  // SHOK over the cells stage 1 reads (the reference computes NGUARD-1
-  // layers; the outer layers are never read before the next refill)
-  // (the batched forms have no hybrid Riemann solver and read SHOK nowhere:
-  // they compute it for the interior, where it is part of the solution)
-  const int band = b->fused ? 0 : (b->p.guard_mode == SPARK_GUARD_REFILL ? 0 : b->limits_array[0]) + 1;
+  // SHOK over the cells the stages' hybrid Riemann solves read: the stage-1
+  // region plus one layer (the reference computes NGUARD-1 layers; the
+  // outer layers are never read before the next refill).  Without the
+  // hybrid solver the batched forms read SHOK nowhere and compute it for
+  // the interior, where it is part of the solution.  The batched forms read
+  // SHOK from the block's own box, so remapping blocks (no guard cells of
+  // their own in the stages) get it one layer out.
+  const bool hyb = b->p.use_hybrid_riemann != 0;
+  const int band = b->fused && !hyb ? 0 : (b->p.guard_mode == SPARK_GUARD_REFILL ? 0 : b->limits_array[0]) + 1;
   if (amr_remap_on(b)) {
-    // remapping blocks: SHOK of the interior only (nothing reads theirs
-    // outside it); a hierarchy has one block shape
+    // a hierarchy has one block shape
     PlainLaunch L = make_launch(b, b->classes[0]);
     if (b->n_tele > 0) {
       L.blist = b->tele_ids_d;
@@ -328,7 +333,7 @@ int run_step_begin(spark_batch* b) {
     }
     L.blist = b->remap_ids_d;
     L.nblk = b->amr_remap_count;
-    HIPCHK(spark::plain_shock_detect(L, b->U, b->flat, 0));
+    HIPCHK(spark::plain_shock_detect(L, b->U, b->flat, hyb ? 1 : 0));
   } else {
     for (auto& c : b->classes) {
       PlainLaunch L = make_launch(b, c);
@@ -793,8 +798,6 @@ int spark_batch_create(const spark_params* p, int nblocks, const spark_block_des
     if (p->nspecies != 0 || p->eos != SPARK_EOS_GAMMA)
       return fail(SPARK_ERR_ARG, "the batched forms implement the gamma law with mass scalars only "
                                  "(nspecies = %d, eos = %d)", p->nspecies, p->eos);
-    if (p->use_hybrid_riemann)
-      return fail(SPARK_ERR_ARG, "the batched form does not implement use_hybrid_riemann");
   }
 
   spark_batch* b = new spark_batch();

@@ -167,16 +167,17 @@ def sod2d_species(nbx=4, nby=4, n=16, nguard=10, species_a=(1.0, 4.0), species_g
     return Problem("sod2d_species", cfg, blocks, arrays, ug=(nbx, nby, 1))
 
 
-def sedov3d(nb=8, n=16, nguard=6, gamma=1.4, E=1.0, r_cells=3.5, p_amb=1e-5, rho=1.0, **kw) -> Problem:
+def sedov3d(nb=8, n=16, nguard=6, gamma=1.4, E=1.0, r_cells=3.5, p_amb=1e-5, rho=1.0, center=(0.5, 0.5, 0.5),
+            **kw) -> Problem:
     """3D Sedov blast on [0,1]^3: E deposited as pressure in the cells whose
-    centres lie within r_cells cell widths of the domain centre (p scaled so
-    the deposited energy is exactly E)."""
+    centres lie within r_cells cell widths of `center` (the domain centre by
+    default; p scaled so the deposited energy is exactly E)."""
     cfg = base_cfg(ndim=3, nguard=nguard, gamma=gamma, **kw)
     N = nb * n
     dx = 1.0 / N
     r0 = r_cells * dx
-    idx = (np.arange(N) + 0.5) * dx - 0.5
-    r2 = idx[:, None, None] ** 2 + idx[None, :, None] ** 2 + idx[None, None, :] ** 2
+    idx = [(np.arange(N) + 0.5) * dx - center[d] for d in range(3)]
+    r2 = idx[2][:, None, None] ** 2 + idx[1][None, :, None] ** 2 + idx[0][None, None, :] ** 2
     n_in = int(np.count_nonzero(r2 < r0 * r0))
     p_in = (gamma - 1.0) * E / (n_in * dx ** 3) if n_in else p_amb
     blocks, arrays = [], []
@@ -187,7 +188,7 @@ def sedov3d(nb=8, n=16, nguard=6, gamma=1.4, E=1.0, r_cells=3.5, p_amb=1e-5, rho
                 b = dict(nx=n, ny=n, nz=n, level=1, dx=dx, dy=dx, dz=dx)
                 NZ, NY, NX = gc_shape(cfg, b)
                 Z, Y, X = _centers(cfg, b, (bi * n * dx, bj * n * dx, bk * n * dx), NZ, NY, NX)
-                rr = (X - 0.5) ** 2 + (Y - 0.5) ** 2 + (Z - 0.5) ** 2
+                rr = (X - center[0]) ** 2 + (Y - center[1]) ** 2 + (Z - center[2]) ** 2
                 pres = np.where(rr < r0 * r0, p_in, p_amb)
                 dens = np.full_like(pres, rho)
                 U = np.zeros((NZ, NY, NX, nunk))
@@ -324,7 +325,7 @@ def amr_leaves(nb0, n, lmax, refine, nguard):
 
 
 def rt_amr(nb0=(4, 4, 8), n=16, lmax=3, slabs=((0.25, 1.75), (0.5, 1.5)), nguard=10, nscalars=8, gamma=1.4,
-           seed=20261204, width=0.0, **kw) -> Problem:
+           seed=20261204, width=0.0, p_boost=None, **kw) -> Problem:
     """3D Rayleigh-Taylor on a block hierarchy: heavy fluid (rho = 2) over
     light (rho = 1) across z0 = the middle of the domain, constant gravity
     g = -1 along z as a source term, hydrostatic pressure with p = 2.5 at the
@@ -335,8 +336,10 @@ def rt_amr(nb0=(4, 4, 8), n=16, lmax=3, slabs=((0.25, 1.75), (0.5, 1.5)), nguard
     at level 1.  x and y are periodic, z is reflecting.  CFL 0.4: the CFL
     step takes the largest (|u_d| + c) / dx_d over directions, so for a
     flow at rest in 3D the unsplit RK3 / WENO5 update sees about three times
-    that Courant number and 0.8 lets a grid-scale odd-even mode grow.  Returns a Problem
-    with info nb0 / origins / bc for spark_set_amr_grid."""
+    that Courant number and 0.8 lets a grid-scale odd-even mode grow.
+    p_boost = (x_lo, x_hi, f) multiplies the pressure by f for x_lo <= x <
+    x_hi (shocks run out of both planes).  Returns a Problem with info nb0 /
+    origins / bc for spark_set_amr_grid."""
     cfg = base_cfg(**{**dict(ndim=3, nguard=nguard, gamma=gamma, nscalars=nscalars, use_gravity=1,
                              grav=(0.0, 0.0, -1.0), flux_correct=1, cfl=0.4), **kw})
     dx0 = 1.0 / (nb0[0] * n)
@@ -377,6 +380,8 @@ def rt_amr(nb0=(4, 4, 8), n=16, lmax=3, slabs=((0.25, 1.75), (0.5, 1.5)), nguard
             heavy = Z > zint
             dens = np.where(heavy, 2.0, 1.0)
             pres = np.where(heavy, 2.5 - 2.0 * (Z - zint), 2.5 + 1.0 * (zint - Z))
+        if p_boost is not None:
+            pres = np.where((X >= p_boost[0]) & (X < p_boost[1]), p_boost[2] * pres, pres)
         zero = np.zeros_like(dens)
         scal = [0.5 * (1.0 + np.tanh((Z - zint) / (0.01 * (q + 1)))) for q in range(nscalars)] if nscalars else None
         U = np.empty((NZ, NY, NX, nunk))

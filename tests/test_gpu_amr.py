@@ -89,11 +89,18 @@ def test_guard_fill_boundary_codes_match_oracle(bc):
     assert nd == 0
 
 
-@pytest.mark.parametrize("flux_correct", [1, 0])
-def test_plain_fp64_amr_steps_match_oracle(flux_correct):
-    prob = two_level(nscalars=8, flux_correct=flux_correct)
+# hybrid: a pressure jump of 10x across the planes x = 0.5 and x = 1 (= 0,
+# periodic), both on block faces of every level, with use_hybridRiemann, so
+# the faces there read SHOK across block and level boundaries
+SHOCKED = dict(use_hybrid_riemann=1, p_boost=(0.5, 1.0, 10.0))
+
+
+@pytest.mark.parametrize("flux_correct,hybrid", [(1, 0), (0, 0), (1, 1)])
+def test_plain_fp64_amr_steps_match_oracle(flux_correct, hybrid):
+    prob = two_level(nscalars=8, flux_correct=flux_correct, **(SHOCKED if hybrid else {}))
     nsteps = 3
-    print(f"RT 2 levels {prob.info['leaves_per_level']}, flux_correct={flux_correct}, {nsteps} steps", flush=True)
+    print(f"RT 2 levels {prob.info['leaves_per_level']}, flux_correct={flux_correct}, hybrid={hybrid}, "
+          f"{nsteps} steps", flush=True)
     d, out, cnt = run(prob, nsteps, "plain_fp64")
     ref = [a.copy() for a in prob.arrays]
     dref, nerr = pyoracle.amr_advance(prob.cfg, prob.blocks, prob.info["nb0"], prob.info["origins"], ref, nsteps,
@@ -152,8 +159,9 @@ def same_level_blocks(prob):
     return out
 
 
-def test_batched_fp64_amr_same_level_halos_bit_exact_with_plain():
-    prob = three_level(nscalars=8)
+@pytest.mark.parametrize("hybrid", [0, 1])
+def test_batched_fp64_amr_same_level_halos_bit_exact_with_plain(hybrid):
+    prob = three_level(nscalars=8, **(SHOCKED if hybrid else {}))
     nsame = same_level_blocks(prob)
     print(f"RT 3 levels {prob.info['leaves_per_level']}: {nsame} blocks read same-level halos", flush=True)
     assert 0 < nsame < len(prob.blocks)
@@ -166,12 +174,12 @@ def test_batched_fp64_amr_same_level_halos_bit_exact_with_plain():
     assert nd == 0
 
 
-@pytest.mark.parametrize("form", ["batched_fp64", "fp32_flux"])
-def test_same_level_halos_equal_telescoping(form):
+@pytest.mark.parametrize("form,hybrid", [("batched_fp64", 0), ("fp32_flux", 0), ("fp32_flux", 1)])
+def test_same_level_halos_equal_telescoping(form, hybrid):
     """Reading the halo of a leaf surrounded by same-level leaves from those
     leaves gives the values telescoping computes in its guard cells: the
     batched forms with and without it agree bit for bit."""
-    prob = three_level(nscalars=3)
+    prob = three_level(nscalars=3, **(SHOCKED if hybrid else {}))
     d_on, o_on, _ = run(prob, 3, form)
     d_off, o_off, _ = run(prob, 3, form, same_level_halos=0)
     assert np.array_equal(d_on, d_off)

@@ -90,6 +90,38 @@ def test_batched_fp64_is_bit_exact_with_the_plain_form(case):
     assert nd == 0
 
 
+@pytest.mark.parametrize("case", ["refill_edge", "telescope_edge"])
+def test_batched_fp64_hybrid_riemann_is_bit_exact_with_the_plain_form(case):
+    """use_hybridRiemann: HLL at faces next to a shock-flagged cell.  The
+    blast sits at the x = 0 domain edge (outflow), so the faces the batched
+    form solves there read SHOK of the guard layer the step's shock
+    detection fills, as the plain form does."""
+    if case == "refill_edge":
+        prob = problems.sedov3d(nb=4, n=16, nguard=6, guard_mode="refill", use_hybrid_riemann=1,
+                                center=(1.0 / 64, 0.5, 0.5), r_cells=4.0)
+    else:
+        prob = problems.sedov3d(nb=2, n=16, nguard=10, use_hybrid_riemann=1, center=(1.0 / 32, 0.5, 0.5),
+                                r_cells=4.0)
+    d64, o64, _, _ = run_gpu(prob, 4, "plain_fp64")
+    db, ob, _, cnt = run_gpu(prob, 4, "batched_fp64")
+    g = prob.cfg["nguard"]
+    V = capi.VI
+    nshk = sum(int(np.count_nonzero(a[..., V["shok"]])) for a in interiors(prob, o64))
+    # flagged cells in the first interior layer at x = 0 (blocks numbered x fastest)
+    edge = sum(int(np.count_nonzero(a[g:-g, g:-g, g, V["shok"]])) for i, a in enumerate(o64) if i % prob.ug[0] == 0)
+    print(f"{case}: shock-flagged interior cells {nshk}, at the x = 0 edge {edge}", flush=True)
+    assert edge > 0 and cnt[0] == 0
+    assert np.array_equal(d64, db)
+    nd = sum(int(np.count_nonzero(a != c)) for a, c in zip(interiors(prob, o64), interiors(prob, ob)))
+    print(f"{case}: differing interior values {nd}", flush=True)
+    assert nd == 0
+    # the switch is live: without it the plain form gives other values
+    cfg0 = dict(prob.cfg, use_hybrid_riemann=0)
+    _, o0, _, _ = run_gpu(problems.Problem(prob.name, cfg0, prob.blocks, prob.arrays, prob.ug, prob.info), 4,
+                          "plain_fp64")
+    assert any(not np.array_equal(a, c) for a, c in zip(interiors(prob, o64), interiors(prob, o0)))
+
+
 def test_fp32_flux_accuracy_mixed_blocks_vs_oracle():
     prob = problems.batched(nblocks=24, nguard=10, shock_every=4)
     nsteps = 10
