@@ -105,11 +105,13 @@ __device__ __forceinline__ int64_t bcix(const BlockMeta& m, int i, int j, int k)
 }
 
 // pool offset of cell (i, j, k) of a block's guard-celled box.  With a
-// neighbour table (uniform grid, no guard refill) a cell outside the
-// interior is taken from the adjacent block, or, past the domain edge, from
-// the block's own boundary layer (outflow: the guard copies the first
-// interior layer, Grid_bcApplyToRegion.F90:526-532), one dimension at a
-// time as the guard fill clamps the global index.
+// neighbour table a cell outside the interior is taken from the adjacent
+// block of its guard sector, or, past a uniform grid's domain edge
+// (NBR_EDGE), from the block's own boundary layer (outflow: the guard copies
+// the first interior layer, Grid_bcApplyToRegion.F90:526-532), one
+// dimension at a time as the guard fill clamps the global index.  A sector
+// the block computes itself holds its own base shifted by the sector's
+// offset, so the same arithmetic lands in its own guard cells.
 __device__ __forceinline__ int64_t cell_at(const BlockMeta& m, const int64_t* __restrict__ nbt, int g, int i, int j,
                                            int k) {
   if (nbt == nullptr) return m.base + bcix(m, i, j, k);
@@ -119,15 +121,15 @@ __device__ __forceinline__ int64_t cell_at(const BlockMeta& m, const int64_t* __
   if (k < g) oz = -1; else if (k >= g + m.nz) oz = 1;
   if ((ox | oy | oz) == 0) return m.base + bcix(m, i, j, k);
   if (ox != 0) {
-    if (nbt[13 + ox] < 0) { i = ox < 0 ? g : g + m.nx - 1; ox = 0; }
+    if (nbt[13 + ox] == NBR_EDGE) { i = ox < 0 ? g : g + m.nx - 1; ox = 0; }
     else i -= ox * m.nx;
   }
   if (oy != 0) {
-    if (nbt[13 + 3 * oy] < 0) { j = oy < 0 ? g : g + m.ny - 1; oy = 0; }
+    if (nbt[13 + 3 * oy] == NBR_EDGE) { j = oy < 0 ? g : g + m.ny - 1; oy = 0; }
     else j -= oy * m.ny;
   }
   if (oz != 0) {
-    if (nbt[13 + 9 * oz] < 0) { k = oz < 0 ? g : g + m.nz - 1; oz = 0; }
+    if (nbt[13 + 9 * oz] == NBR_EDGE) { k = oz < 0 ? g : g + m.nz - 1; oz = 0; }
     else k -= oz * m.nz;
   }
   return nbt[13 + ox + 3 * oy + 9 * oz] + bcix(m, i, j, k);
@@ -259,7 +261,8 @@ __device__ __forceinline__ void xy_face(const FusedArgs& A, const BlockMeta& m, 
   mo[w] = F[F_MASS];
   if (A.fb[0] != nullptr) {
     const int i = I.x0 + fi, j = I.y0 + fj;
-    const bool own = DIR == 0 ? (fi < tx || i == I.xhi + 1) : (fj < ty || j == I.yhi + 1);
+    // a boundary face belongs to the item holding the interior cell beside it
+    const bool own = DIR == 0 ? (i == g ? fi < tx : fi > 0) : (j == g ? fj < ty : fj > 0);
     // the block's boundary faces only (the register hand-off reads no other)
     const bool in = DIR == 0 ? ((i == g || i == g + m.nx) && j >= g && j < g + m.ny)
                              : (i >= g && i < g + m.nx && (j == g || j == g + m.ny));
@@ -711,7 +714,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
               if (regs) {
                 const int i = I.x0 + fi, jj = I.y0 + fj;
                 if (isx) {
-                  if ((fi < tx || i == I.xhi + 1) && (i == g || i == g + m.nx) && jj >= g && jj < g + m.ny &&
+                  if ((i == g ? fi < tx : fi > 0) && (i == g || i == g + m.nx) && jj >= g && jj < g + m.ny &&
                       k >= g && k < g + m.nz) {
                     const int ex = m.nx + 1, ey = m.ny;
                     const int64_t tot = (int64_t)ex * ey * m.nz;
@@ -720,7 +723,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
                       reg_add(A.fb[0], o, tot, F_NUM + c0 + j, A.weight, A.add_flux, fo[j * nf + w]);
                   }
                 } else {
-                  if ((fj < ty || jj == I.yhi + 1) && i >= g && i < g + m.nx && (jj == g || jj == g + m.ny) &&
+                  if ((jj == g ? fj < ty : fj > 0) && i >= g && i < g + m.nx && (jj == g || jj == g + m.ny) &&
                       k >= g && k < g + m.nz) {
                     const int ex = m.nx, ey = m.ny + 1;
                     const int64_t tot = (int64_t)ex * ey * m.nz;

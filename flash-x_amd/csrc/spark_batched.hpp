@@ -16,9 +16,11 @@ struct FusedItem {
   int blk;
   int x0, y0, tx, ty;
   int z0, z1;
-  int xhi, yhi;  // last cell of the stage region in x / y (flux-register ownership)
-  int pad[3];
+  int pad[5];
 };
+
+// neighbour-table entry of a uniform grid's guard sector past the domain edge
+constexpr int64_t NBR_EDGE = INT64_MIN;
 
 struct FusedArgs {
   DevParams P;
@@ -33,9 +35,11 @@ struct FusedArgs {
   double* fb[3];           // flux registers (nullptr when not correcting)
   // per block the 27 pool offsets (BlockMeta::base) of the blocks around
   // it: cells outside the block's interior are read from those neighbours
-  // (uniform grid, guards not refilled between stages: -1 where the domain
-  // ends, outflow).  A row whose centre entry is -1, or nullptr: the block
-  // reads its own guard cells.
+  // (uniform grid, guards not refilled between stages: NBR_EDGE where the
+  // domain ends, outflow).  A sector the block computes itself (hierarchy:
+  // no leaf of its level across) holds base + ox nx + NX (oy ny + NY oz nz),
+  // which cell_at() turns into the block's own guard cell.  A row whose
+  // centre entry is -1, or nullptr: the block reads its own guard cells.
   const int64_t* nbr;
   double weight;
   int add_flux;

@@ -100,9 +100,14 @@ struct spark_batch {
   bool amr = false;          // block hierarchy declared (spark_set_amr_grid)
   // batched forms on a block hierarchy: blocks whose 26 neighbours are leaves
   // of their own level read their halos from them (nbr_d row), every stage
-  // over the interior only; the others telescope over their guard cells
+  // over the interior only (amr_remap_blk 1); blocks with some such
+  // neighbours read those and telescope over the other guard sectors
+  // (amr_remap_blk 2, amr_local: bit 13 + ox + 3 oy + 9 oz set for a sector
+  // they compute); the others telescope over all their guard cells (0)
   std::vector<char> amr_remap_blk;
+  std::vector<uint32_t> amr_local;
   int amr_remap_count = 0;
+  int amr_partial_count = 0;
   bool same_level = true;      // SPARK_OPT_SAME_LEVEL_HALOS
   int* remap_ids_d = nullptr;  // the remapping blocks
   int* tele_ids_d = nullptr;   // the telescoping blocks
@@ -204,7 +209,9 @@ bool fused_remap(const spark_batch* b);
 void drop_graph(spark_batch* b);
 
 // batched forms on a block hierarchy: same-level halos in use
-bool amr_remap_on(const spark_batch* b) { return b->same_level && b->amr_remap_count > 0; }
+bool amr_remap_on(const spark_batch* b) {
+  return b->same_level && b->amr_remap_count + b->amr_partial_count > 0;
+}
 
 int run_dt(spark_batch* b, int hist_slot) {
   for (auto& c : b->classes) {
@@ -248,9 +255,11 @@ int run_fill(spark_batch* b, double* pool, bool full, int depth) {
     vs.n = 0;
     for (int v : {spark::V_DENS, spark::V_GAMC, spark::V_PRES, spark::V_VELX, spark::V_VELY, spark::V_VELZ})
       vs.v[vs.n++] = v;
-    L.blist = b->remap_ids_d;
-    L.nblk = b->amr_remap_count;
-    HIPCHK(spark::amr_fill(L, b->amr_g, pool, vs, std::min(depth, b->p.use_hybrid_riemann ? 2 : 1)));
+    if (b->amr_remap_count > 0) {
+      L.blist = b->remap_ids_d;
+      L.nblk = b->amr_remap_count;
+      HIPCHK(spark::amr_fill(L, b->amr_g, pool, vs, std::min(depth, b->p.use_hybrid_riemann ? 2 : 1)));
+    }
   } else if (b->amr) {
     HIPCHK(spark::amr_fill(L, b->amr_g, pool, vl, depth));
   } else {
@@ -331,9 +340,11 @@ int run_step_begin(spark_batch* b) {
       L.nblk = b->n_tele;
       HIPCHK(spark::plain_shock_detect(L, b->U, b->flat, band));
     }
-    L.blist = b->remap_ids_d;
-    L.nblk = b->amr_remap_count;
-    HIPCHK(spark::plain_shock_detect(L, b->U, b->flat, hyb ? 1 : 0));
+    if (b->amr_remap_count > 0) {
+      L.blist = b->remap_ids_d;
+      L.nblk = b->amr_remap_count;
+      HIPCHK(spark::plain_shock_detect(L, b->U, b->flat, hyb ? 1 : 0));
+    }
   } else {
     for (auto& c : b->classes) {
       PlainLaunch L = make_launch(b, c);
@@ -362,10 +373,11 @@ int stage_ngoff(const spark_batch* b, int stage) {
   return b->p.guard_mode == SPARK_GUARD_REFILL ? 0 : b->limits_array[stage - 1];
 }
 
-// stage region of one block in the batched forms: blocks that read their
-// halos from same-level neighbours update only their interior
+// stage region of one block in the batched forms: blocks that read all
+// their halos from same-level neighbours update only their interior (the
+// region of a block that reads some is built by partial_boxes())
 int blk_ngoff(const spark_batch* b, int blk, int stage) {
-  if (amr_remap_on(b) && b->amr_remap_blk[blk]) return 0;
+  if (amr_remap_on(b) && b->amr_remap_blk[blk] == 1) return 0;
   return stage_ngoff(b, stage);
 }
 
@@ -546,24 +558,86 @@ std::vector<std::pair<int, int>> tile_candidates(const spark_batch* b, int ex, i
   return out;
 }
 
-void tile_items(const spark_batch* b, int blk, int ngoff, int tx, int ty, std::vector<spark::FusedItem>& out) {
-  const BlockMeta& m = b->meta_h[blk];
-  const int g = b->p.nguard;
-  const int xlo = g - ngoff, xhi = g + m.nx - 1 + ngoff;
-  const int ylo = g - ngoff, yhi = g + m.ny - 1 + ngoff;
-  const int zlo = g - ngoff, zhi = g + m.nz - 1 + ngoff;
-  for (int y0 = ylo; y0 <= yhi; y0 += ty)
-    for (int x0 = xlo; x0 <= xhi; x0 += tx) {
+// a box of a block's guard-celled index space, bounds inclusive
+struct Box {
+  int x0, x1, y0, y1, z0, z1;
+  double cost() const { return (double)(x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 3); }
+};
+
+// the work items of a box: (tx x ty) column tiles marched over its z range
+void box_items(int blk, const Box& r, int tx, int ty, std::vector<spark::FusedItem>& out) {
+  for (int y0 = r.y0; y0 <= r.y1; y0 += ty)
+    for (int x0 = r.x0; x0 <= r.x1; x0 += tx) {
       spark::FusedItem it;
       memset(&it, 0, sizeof(it));
       it.blk = blk;
       it.x0 = x0; it.y0 = y0;
-      it.tx = std::min(tx, xhi - x0 + 1);
-      it.ty = std::min(ty, yhi - y0 + 1);
-      it.z0 = zlo; it.z1 = zhi;
-      it.xhi = xhi; it.yhi = yhi;
+      it.tx = std::min(tx, r.x1 - x0 + 1);
+      it.ty = std::min(ty, r.y1 - y0 + 1);
+      it.z0 = r.z0; it.z1 = r.z1;
       out.push_back(it);
     }
+}
+
+Box region_box(const BlockMeta& m, int g, int ngoff) {
+  return Box{g - ngoff, g + m.nx - 1 + ngoff, g - ngoff, g + m.ny - 1 + ngoff, g - ngoff, g + m.nz - 1 + ngoff};
+}
+
+// stage region of a block that reads some guard sectors from same-level
+// leaves: its interior and the sectors it computes, each to depth e, as
+// disjoint boxes.  Per column sector (ox, oy) the z extent is the union of
+// its computed sectors (two runs when only the middle one is read from a
+// leaf); column sectors of one row with equal z runs merge along x, and
+// rows with equal boxes merge along y.
+void partial_boxes(const spark_batch* b, int blk, int e, std::vector<Box>& out) {
+  const BlockMeta& m = b->meta_h[blk];
+  const int g = b->p.nguard;
+  const uint32_t local = b->amr_local[blk];
+  auto span = [&](int o, int nn, int& lo, int& hi) {
+    if (o < 0) { lo = g - e; hi = g - 1; }
+    else if (o == 0) { lo = g; hi = g + nn - 1; }
+    else { lo = g + nn; hi = g + nn + e - 1; }
+  };
+  std::vector<Box> prev;
+  auto flush = [&](std::vector<Box>& row) { for (auto& r : row) out.push_back(r); row.clear(); };
+  for (int oy = -1; oy <= 1; oy++) {
+    int y0, y1;
+    span(oy, m.ny, y0, y1);
+    if (y1 < y0) continue;
+    std::vector<Box> row;
+    std::vector<std::pair<int, int>> runs_prev;
+    int run_x0 = 0, run_x1 = -1;
+    for (int ox = -1; ox <= 1; ox++) {
+      int x0, x1;
+      span(ox, m.nx, x0, x1);
+      if (x1 < x0) continue;
+      std::vector<std::pair<int, int>> runs;
+      for (int oz = -1; oz <= 1; oz++) {
+        const bool comp = (ox == 0 && oy == 0 && oz == 0) || ((local >> (13 + ox + 3 * oy + 9 * oz)) & 1u);
+        int z0, z1;
+        span(oz, m.nz, z0, z1);
+        if (!comp || z1 < z0) continue;
+        if (!runs.empty() && runs.back().second + 1 == z0) runs.back().second = z1;
+        else runs.push_back({z0, z1});
+      }
+      if (run_x1 >= run_x0 && runs == runs_prev) { run_x1 = x1; continue; }
+      for (auto& zr : runs_prev) row.push_back(Box{run_x0, run_x1, y0, y1, zr.first, zr.second});
+      runs_prev = runs;
+      run_x0 = x0; run_x1 = x1;
+    }
+    for (auto& zr : runs_prev) row.push_back(Box{run_x0, run_x1, y0, y1, zr.first, zr.second});
+    // merge with the previous row when the boxes match in x and z
+    bool same = !prev.empty() && prev.size() == row.size();
+    for (size_t t = 0; same && t < row.size(); t++)
+      same = row[t].x0 == prev[t].x0 && row[t].x1 == prev[t].x1 && row[t].z0 == prev[t].z0 && row[t].z1 == prev[t].z1;
+    if (same) {
+      for (auto& r : prev) r.y1 = y1;
+    } else {
+      flush(prev);
+      prev = row;
+    }
+  }
+  flush(prev);
 }
 
 // build the per-XCD queues of one stage from the chosen tiles.  A block's
@@ -578,11 +652,20 @@ int build_stage_plan(spark_batch* b, int stage, const std::vector<int>* only_blo
   for (size_t ci = 0; ci < b->classes.size(); ci++)
     for (int id : b->classes[ci].ids) cls_of[id] = (int)ci;
   std::vector<int> ngoff_of(b->nblk, 0);
+  std::map<int, std::vector<Box>> boxes;  // blocks that read some sectors from same-level leaves
   for (int id : blocks) {
     const BlockMeta& m = b->meta_h[id];
+    if (force_ngoff < 0 && amr_remap_on(b) && b->amr_remap_blk[id] == 2) {
+      auto& bx = boxes[id];
+      partial_boxes(b, id, stage_ngoff(b, stage), bx);
+      double c = 0.;
+      for (const auto& r : bx) c += r.cost();
+      cost.push_back({c, id});
+      continue;
+    }
     const int ngoff = force_ngoff >= 0 ? force_ngoff : blk_ngoff(b, id, stage);
     ngoff_of[id] = ngoff;
-    cost.push_back({(double)(m.nx + 2 * ngoff) * (m.ny + 2 * ngoff) * (m.nz + 2 * ngoff + 2), id});
+    cost.push_back({region_box(m, b->p.nguard, ngoff).cost(), id});
   }
   std::sort(cost.begin(), cost.end(), [](const std::pair<double, int>& a, const std::pair<double, int>& c) {
     return a.first > c.first || (a.first == c.first && a.second < c.second);
@@ -591,13 +674,32 @@ int build_stage_plan(spark_batch* b, int stage, const std::vector<int>* only_blo
   std::vector<double> load(spark::FUSED_XCDS, 0.);
   for (auto& pr : cost) {
     const int id = pr.second;
+    const BlockMeta& m = b->meta_h[id];
+    const int qi = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[qi] += pr.first;
+    auto bx = boxes.find(id);
+    if (bx != boxes.end()) {
+      for (const auto& r : bx->second) {
+        const int w = r.x1 - r.x0 + 1, h = r.y1 - r.y0 + 1;
+        // the interior columns take the tile tuned for interior regions
+        auto tt = b->tiles.find(std::make_pair(cls_of[id], 0));
+        std::pair<int, int> t;
+        if (w == m.nx && h == m.ny && tt != b->tiles.end()) {
+          t = tt->second;
+        } else {
+          auto cands = tile_candidates(b, w, h);
+          if (cands.empty()) return fail(SPARK_ERR_ARG, "no tile fits a %d x %d region", w, h);
+          t = cands[0];
+        }
+        box_items(id, r, t.first, t.second, q[qi]);
+      }
+      continue;
+    }
     const int ngoff = ngoff_of[id];
     const auto key = std::make_pair(cls_of[id], ngoff);
     auto tt = b->tiles.find(key);
     if (tt == b->tiles.end()) return fail(SPARK_ERR_STATE, "no tile chosen for class %d ngoff %d", key.first, ngoff);
-    const int qi = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-    tile_items(b, id, ngoff, tt->second.first, tt->second.second, q[qi]);
-    load[qi] += pr.first;
+    box_items(id, region_box(m, b->p.nguard, ngoff), tt->second.first, tt->second.second, q[qi]);
   }
   auto& pl = b->plan[stage - 1];
   pl.items.clear();
@@ -1049,7 +1151,7 @@ int spark_set_uniform_grid(spark_batch* b, int nbx, int nby, int nbz) {
   }
   b->ug[0] = nbx; b->ug[1] = nby; b->ug[2] = nbz;
   // neighbour table (blocks numbered x fastest)
-  std::vector<int64_t> nb((size_t)b->nblk * 27, -1);
+  std::vector<int64_t> nb((size_t)b->nblk * 27, spark::NBR_EDGE);
   for (int id = 0; id < b->nblk; id++) {
     const int bx = id % nbx, by = (id / nbx) % nby, bz = id / (nbx * nby);
     for (int oz = -1; oz <= 1; oz++)
@@ -1376,55 +1478,73 @@ int spark_set_amr_grid(spark_batch* b, int nbx0, int nby0, int nbz0, const int* 
             return fail(SPARK_ERR_ARG, "block %d (level %d): a leaf of level %d lies under its guard cells", i, L, L2);
         }
   }
-  // batched forms: a block whose 26 neighbours (periodic wrap allowed) are
-  // all leaves of its own level gets their pool offsets; its guard region
-  // then holds nothing a stage needs that the neighbours' interiors do not,
-  // so it reads them there and every stage covers only its interior.  The
-  // stage values this gives are those of telescoping (the same cells from
-  // the same inputs); blocks at a level change or a non-periodic domain
-  // edge keep telescoping over their own guard cells.
+  // batched forms: a guard sector (face, edge or corner region of the guard
+  // band) across which lies a leaf of the block's own level (periodic wrap
+  // allowed) holds nothing a stage needs that the leaf's interior does not,
+  // so the block reads that sector from the leaf, every stage.  A block
+  // whose 26 sectors all qualify updates only its interior; one with some
+  // telescopes over the others (it computes them at each stage's depth);
+  // one with none telescopes over its own guard cells.  The stage values
+  // this gives are those of telescoping: the same cells from the same
+  // inputs, since the guard fill is a function of the cell's position and
+  // the leaves' data.
   b->amr_remap_blk.assign(b->nblk, 0);
+  b->amr_local.assign(b->nblk, 0);
   b->amr_remap_count = 0;
+  b->amr_partial_count = 0;
   if (b->fused && n >= b->p.nguard) {
     std::vector<int64_t> nb((size_t)b->nblk * 27, -1);
+    const uint32_t all_sectors = ((1u << 27) - 1) & ~(1u << 13);
     for (int i = 0; i < b->nblk; i++) {
-      const int L = b->meta_h[i].level, sh = lmax - L;
+      const BlockMeta& mi = b->meta_h[i];
+      const int L = mi.level, sh = lmax - L;
       int bp[3], nbl[3];
       for (int d = 0; d < 3; d++) { bp[d] = origins[3 * i + d] / n; nbl[d] = nb0[d] << (L - 1); }
       int64_t row[27];
-      bool ok = true;
-      for (int oz = -1; oz <= 1 && ok; oz++)
-        for (int oy = -1; oy <= 1 && ok; oy++)
-          for (int ox = -1; ox <= 1 && ok; ox++) {
+      uint32_t local = 0;
+      for (int oz = -1; oz <= 1; oz++)
+        for (int oy = -1; oy <= 1; oy++)
+          for (int ox = -1; ox <= 1; ox++) {
+            const int s = 13 + ox + 3 * oy + 9 * oz;
+            if (s == 13) { row[s] = mi.base; continue; }
             const int o[3] = {ox, oy, oz};
             int q[3];
-            for (int d = 0; d < 3 && ok; d++) {
+            bool in = true;
+            for (int d = 0; d < 3; d++) {
               q[d] = bp[d] + o[d];
               if (q[d] < 0 || q[d] >= nbl[d]) {
-                if (!per[d]) ok = false;
+                if (!per[d]) in = false;
                 q[d] = (q[d] + nbl[d]) % nbl[d];
               }
             }
-            if (!ok) break;
-            const int j = map[((size_t)q[0] << sh) + (size_t)md[0] * (((size_t)q[1] << sh) + (size_t)md[1] * ((size_t)q[2] << sh))];
-            if (b->meta_h[j].level != L) { ok = false; break; }
-            row[13 + ox + 3 * oy + 9 * oz] = b->meta_h[j].base;
+            int j = -1;
+            if (in) j = map[((size_t)q[0] << sh) + (size_t)md[0] * (((size_t)q[1] << sh) + (size_t)md[1] * ((size_t)q[2] << sh))];
+            if (j >= 0 && b->meta_h[j].level == L) {
+              row[s] = b->meta_h[j].base;
+            } else {
+              // computed here: the entry cell_at() shifts back into this block
+              row[s] = mi.base + (int64_t)ox * mi.nx + (int64_t)mi.NX * ((int64_t)oy * mi.ny + (int64_t)mi.NY * oz * mi.nz);
+              local |= 1u << s;
+            }
           }
-      if (!ok) continue;
+      if (local == all_sectors) continue;  // telescopes over its own guard cells
       std::copy(row, row + 27, nb.begin() + (size_t)i * 27);
-      b->amr_remap_blk[i] = 1;
-      b->amr_remap_count++;
+      b->amr_local[i] = local;
+      b->amr_remap_blk[i] = local ? 2 : 1;
+      (local ? b->amr_partial_count : b->amr_remap_count)++;
     }
-    if (b->amr_remap_count > 0) {
+    if (b->amr_remap_count + b->amr_partial_count > 0) {
       std::vector<int> rl, tl;
-      for (int i = 0; i < b->nblk; i++) (b->amr_remap_blk[i] ? rl : tl).push_back(i);
+      for (int i = 0; i < b->nblk; i++) (b->amr_remap_blk[i] == 1 ? rl : tl).push_back(i);
       auto F = [](void* p) { if (p) (void)hipFree(p); };
       F(b->nbr_d); F(b->remap_ids_d); F(b->tele_ids_d);
       b->nbr_d = nullptr; b->remap_ids_d = nullptr; b->tele_ids_d = nullptr;
       HIPCHK(hipMalloc((void**)&b->nbr_d, sizeof(int64_t) * nb.size()));
       HIPCHK(hipMemcpy(b->nbr_d, nb.data(), sizeof(int64_t) * nb.size(), hipMemcpyHostToDevice));
-      HIPCHK(hipMalloc((void**)&b->remap_ids_d, sizeof(int) * rl.size()));
-      HIPCHK(hipMemcpy(b->remap_ids_d, rl.data(), sizeof(int) * rl.size(), hipMemcpyHostToDevice));
+      if (!rl.empty()) {
+        HIPCHK(hipMalloc((void**)&b->remap_ids_d, sizeof(int) * rl.size()));
+        HIPCHK(hipMemcpy(b->remap_ids_d, rl.data(), sizeof(int) * rl.size(), hipMemcpyHostToDevice));
+      }
       b->n_tele = (int)tl.size();
       if (!tl.empty()) {
         HIPCHK(hipMalloc((void**)&b->tele_ids_d, sizeof(int) * tl.size()));

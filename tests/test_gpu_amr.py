@@ -124,13 +124,14 @@ def three_level(**kw):
     # 1 x 1 x 2 level-1 blocks of 16^3, 3 levels: 32 of the 72 leaves (levels
     # 2 and 3) have 26 leaves of their own level around them, so the batched
     # forms read their halos from those neighbours and update only their
-    # interior every stage; the other 40 telescope over their guard cells
+    # interior every stage; the other 40 read the guard sectors that have a
+    # leaf of their level across from it and telescope over the rest
     return problems.rt_amr(nb0=(1, 1, 2), n=16, lmax=3, **kw)
 
 
 def same_level_blocks(prob):
-    """Leaves whose 26 neighbours (periodic wrap, no other domain edge) are
-    leaves of their own level."""
+    """(leaves whose 26 guard sectors all have a leaf of their own level
+    across (periodic wrap, no other domain edge), leaves with some)."""
     info = prob.info
     n = prob.blocks[0]["nx"]
     lv = [b["level"] for b in prob.blocks]
@@ -143,28 +144,33 @@ def same_level_blocks(prob):
         lo = [(int(o[d]) // n) << sh for d in range(3)]
         slot[lo[2]:lo[2] + (1 << sh), lo[1]:lo[1] + (1 << sh), lo[0]:lo[0] + (1 << sh)] = i
     per = [b == problems.BC_PERIODIC for b in info["bc"]]
-    out = 0
+    full = some = 0
     for o, L in zip(org, lv):
         sh = lmax - L
         nbl = [info["nb0"][d] << (L - 1) for d in range(3)]
-        ok = True
+        nsame = 0
         for off in np.ndindex(3, 3, 3):
+            if off == (1, 1, 1):
+                continue
             q = [int(o[d]) // n + off[d] - 1 for d in range(3)]
+            ok = True
             for d in range(3):
                 if not 0 <= q[d] < nbl[d]:
                     ok = ok and per[d]
                     q[d] %= nbl[d]
-            ok = ok and lv[slot[q[2] << sh, q[1] << sh, q[0] << sh]] == L
-        out += ok
-    return out
+            nsame += ok and lv[slot[q[2] << sh, q[1] << sh, q[0] << sh]] == L
+        full += nsame == 26
+        some += 0 < nsame < 26
+    return full, some
 
 
 @pytest.mark.parametrize("hybrid", [0, 1])
 def test_batched_fp64_amr_same_level_halos_bit_exact_with_plain(hybrid):
     prob = three_level(nscalars=8, **(SHOCKED if hybrid else {}))
-    nsame = same_level_blocks(prob)
-    print(f"RT 3 levels {prob.info['leaves_per_level']}: {nsame} blocks read same-level halos", flush=True)
-    assert 0 < nsame < len(prob.blocks)
+    full, some = same_level_blocks(prob)
+    print(f"RT 3 levels {prob.info['leaves_per_level']}: {full} blocks read all their halos from same-level "
+          f"leaves, {some} some of them", flush=True)
+    assert full > 0 and some > 0
     d64, o64, _ = run(prob, 3, "plain_fp64")
     db, ob, cnt = run(prob, 3, "batched_fp64")
     assert cnt[0] == 0
