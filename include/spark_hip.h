@@ -238,9 +238,11 @@ int spark_profile_read(spark_batch* b, spark_kernel_stat* stats, int max, int* n
  * batched FP32-flux form times candidate tile shapes per block shape and
  * stage region on the first step and keeps the fastest.
  * SPARK_OPT_SAME_LEVEL_HALOS (default 1): on a block hierarchy the batched
- * forms let leaves surrounded by same-level leaves read their halos from
- * them; 0 makes every leaf telescope over its own guard cells (the same
- * values, more work).  No counterpart in the reference. */
+ * forms let a leaf read each guard sector (face, edge or corner region of
+ * its guard band) that has a leaf of its own level across from that leaf,
+ * and telescope only over the others; 0 makes every leaf telescope over
+ * all its guard cells (the same values, more work).  No counterpart in the
+ * reference. */
 #define SPARK_OPT_GRAPH 1
 #define SPARK_OPT_AUTOTUNE 2
 #define SPARK_OPT_SAME_LEVEL_HALOS 3
