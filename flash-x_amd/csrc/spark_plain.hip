@@ -711,17 +711,20 @@ static hipError_t launch_flux_t(const PlainLaunch& L, int ngoff) {
   const int T1 = (DIR == 0) ? 1 : 0, T2 = (DIR == 2) ? 1 : 2;
   const int Lr = hi[DIR] - lo[DIR] + 3;
   const int nt1 = hi[T1] - lo[T1] + 1, nt2 = hi[T2] - lo[T2] + 1;
-  int lanes = 256 / Lr;
-  if (DIR != 0 && lanes < 16) lanes = 16;
-  if (lanes < 1) lanes = 1;
-  if (lanes * Lr > 1024) lanes = 1024 / Lr;
-  if (lanes > nt1) lanes = nt1;
   constexpr int NRM = R_NUM + NSM;
-  {
-    // keep the LDS exchange buffer within 64 KiB per workgroup
-    const int cap = 65536 / (int)(sizeof(double) * (NRM + 6) * Lr);
-    if (lanes > cap) lanes = cap;
-    if (lanes < 1) lanes = 1;
+  // pencils per workgroup: the LDS exchange buffer stays within 80 KiB (two
+  // workgroups per CU) and the workgroup within 1024 threads; among those
+  // the count that launches the fewest waves per plane of pencils (an
+  // uneven split leaves a chunk of mostly idle lanes), the largest on
+  // ties.  y/z sweeps keep >= 16 x-columns per row where the plane has them,
+  // so each row of a wave's loads fills a 128-byte line.
+  const int cap = 81920 / (int)(sizeof(double) * (NRM + 6) * Lr);
+  const int lmax = std::max(1, std::min(std::min(1024 / Lr, cap), nt1));
+  const int lmin = DIR == 0 ? 1 : std::min(std::min(16, nt1), lmax);
+  int lanes = lmax, best = 1 << 30;
+  for (int l = lmin; l <= lmax; l++) {
+    const int waves = ((nt1 + l - 1) / l) * ((l * Lr + 63) / 64);
+    if (waves <= best) { best = waves; lanes = l; }
   }
   const int chunks = (nt1 + lanes - 1) / lanes;
   dim3 grid(chunks * nt2, L.nblk);
