@@ -123,9 +123,12 @@ __device__ __forceinline__ double mc_slope(double a, double b) {
 // per dimension, the monotonized-central slope of its face neighbours times
 // +-1/4; piecewise-constant injection feeds an odd-even mode into the fine
 // blocks that grows at fine/coarse faces under gravity).
+// smask (optional): per block the guard sectors (bit 13 + ox + 3 oy + 9 oz)
+// to fill to `depth`; the other sectors are filled only to `keep` layers
 __global__ void __launch_bounds__(256) k_amr_fill(DevParams P, const BlockMeta* __restrict__ meta,
                                                   const int* __restrict__ blist, AmrGrid G,
-                                                  double* __restrict__ U, VarList vars, int depth) {
+                                                  double* __restrict__ U, VarList vars, int depth,
+                                                  const uint32_t* __restrict__ smask, int keep) {
   const int bid = blist ? blist[blockIdx.y] : (int)blockIdx.y;
   const BlockMeta m = meta[bid];
   const int n = G.n, g = P.ng, L = m.level;
@@ -133,16 +136,20 @@ __global__ void __launch_bounds__(256) k_amr_fill(DevParams P, const BlockMeta* 
   const int total = ext * ext * ext;
   const int o[3] = {G.org[3 * bid], G.org[3 * bid + 1], G.org[3 * bid + 2]};
   const Shape S{m.NX, (int64_t)m.NX * m.NX, m.cstride};
+  const uint32_t mk = smask ? smask[bid] : ~0u;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
     const int loc[3] = {g - depth + t % ext, g - depth + (t / ext) % ext, g - depth + t / (ext * ext)};
     bool inside = true;
-    int gi[3];
+    int gi[3], sec = 13, dep = 0;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      if (loc[d] < g || loc[d] >= g + n) inside = false;
+      const int sd = d == 0 ? 1 : d == 1 ? 3 : 9;
+      if (loc[d] < g) { inside = false; sec -= sd; dep = max(dep, g - loc[d]); }
+      else if (loc[d] >= g + n) { inside = false; sec += sd; dep = max(dep, loc[d] - g - n + 1); }
       gi[d] = o[d] + loc[d] - g;
     }
     if (inside) continue;
+    if (dep > keep && !((mk >> sec) & 1u)) continue;
     const int flip = amr_domain(G, L, gi);
     const int64_t dst = m.base + acix(m.NX, loc[0], loc[1], loc[2]);
     const int s = leaf_at(G, L, gi);
@@ -265,13 +272,14 @@ inline int grid_x(int total, int cap) {
 
 }  // namespace
 
-hipError_t amr_fill(const PlainLaunch& L, const AmrGrid& G, double* U, const VarList& vars, int depth) {
+hipError_t amr_fill(const PlainLaunch& L, const AmrGrid& G, double* U, const VarList& vars, int depth,
+                    const uint32_t* smask, int keep, const char* prof_name) {
   const int ext = G.n + 2 * depth;
   const int box = ext * ext * ext;
   const int nin = G.n * G.n * G.n;
-  if (L.prof) L.prof->begin(L.stream, "k_amr_fill", 16.0 * L.nblk * (double)(box - nin) * vars.n, 0.0);
+  if (L.prof) L.prof->begin(L.stream, prof_name, 16.0 * L.nblk * (double)(box - nin) * vars.n, 0.0);
   hipLaunchKernelGGL(k_amr_fill, dim3(grid_x(box, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta, L.blist, G, U,
-                     vars, depth);
+                     vars, depth, smask, keep);
   if (L.prof) L.prof->end(L.stream);
   return hipGetLastError();
 }

@@ -106,6 +106,7 @@ struct spark_batch {
   // they compute); the others telescope over all their guard cells (0)
   std::vector<char> amr_remap_blk;
   std::vector<uint32_t> amr_local;
+  uint32_t* amr_smask_d = nullptr;  // per block the guard sectors it computes (all for telescoping blocks)
   int amr_remap_count = 0;
   int amr_partial_count = 0;
   bool same_level = true;      // SPARK_OPT_SAME_LEVEL_HALOS
@@ -247,9 +248,12 @@ int run_fill(spark_batch* b, double* pool, bool full, int depth) {
     // only what shockDetect of their interior reads (and of the layer
     // around it, which the hybrid Riemann solver reads SHOK of)
     if (b->n_tele > 0) {
+      // blocks that read some sectors from same-level leaves: those sectors
+      // only to the layer shockDetect of the interior reads (the hybrid
+      // solver reads SHOK of computed cells next to them: all layers then)
       L.blist = b->tele_ids_d;
       L.nblk = b->n_tele;
-      HIPCHK(spark::amr_fill(L, b->amr_g, pool, vl, depth));
+      HIPCHK(spark::amr_fill(L, b->amr_g, pool, vl, depth, b->p.use_hybrid_riemann ? nullptr : b->amr_smask_d, 1));
     }
     spark::VarList vs;  // shockDetect: dens, gamc, pres and the velocities one layer out
     vs.n = 0;
@@ -258,7 +262,8 @@ int run_fill(spark_batch* b, double* pool, bool full, int depth) {
     if (b->amr_remap_count > 0) {
       L.blist = b->remap_ids_d;
       L.nblk = b->amr_remap_count;
-      HIPCHK(spark::amr_fill(L, b->amr_g, pool, vs, std::min(depth, b->p.use_hybrid_riemann ? 2 : 1)));
+      HIPCHK(spark::amr_fill(L, b->amr_g, pool, vs, std::min(depth, b->p.use_hybrid_riemann ? 2 : 1), nullptr, 0,
+                             "k_amr_fill_shock_layer"));
     }
   } else if (b->amr) {
     HIPCHK(spark::amr_fill(L, b->amr_g, pool, vl, depth));
@@ -493,6 +498,7 @@ void free_batch(spark_batch* b) {
   if (b->nbr_d) (void)hipFree(b->nbr_d);
   if (b->remap_ids_d) (void)hipFree(b->remap_ids_d);
   if (b->tele_ids_d) (void)hipFree(b->tele_ids_d);
+  if (b->amr_smask_d) (void)hipFree(b->amr_smask_d);
   if (b->amr_fmask_d) (void)hipFree(b->amr_fmask_d);
   if (b->corr_ids_d) (void)hipFree(b->corr_ids_d);
   if (b->amr_map_d) (void)hipFree(b->amr_map_d);
@@ -1550,6 +1556,12 @@ int spark_set_amr_grid(spark_batch* b, int nbx0, int nby0, int nbz0, const int* 
         HIPCHK(hipMalloc((void**)&b->tele_ids_d, sizeof(int) * tl.size()));
         HIPCHK(hipMemcpy(b->tele_ids_d, tl.data(), sizeof(int) * tl.size(), hipMemcpyHostToDevice));
       }
+      std::vector<uint32_t> sm(b->nblk);
+      for (int i = 0; i < b->nblk; i++) sm[i] = b->amr_remap_blk[i] == 2 ? b->amr_local[i] : all_sectors;
+      F(b->amr_smask_d);
+      b->amr_smask_d = nullptr;
+      HIPCHK(hipMalloc((void**)&b->amr_smask_d, sizeof(uint32_t) * sm.size()));
+      HIPCHK(hipMemcpy(b->amr_smask_d, sm.data(), sizeof(uint32_t) * sm.size(), hipMemcpyHostToDevice));
     }
     b->planned = false;
   }
