@@ -260,10 +260,11 @@ int run_fill(spark_batch* b, double* pool, bool full, int depth) {
     for (int v : {spark::V_DENS, spark::V_GAMC, spark::V_PRES, spark::V_VELX, spark::V_VELY, spark::V_VELZ})
       vs.v[vs.n++] = v;
     if (b->amr_remap_count > 0) {
+      // every guard sector faces a same-level leaf: copies through the
+      // neighbour table
       L.blist = b->remap_ids_d;
       L.nblk = b->amr_remap_count;
-      HIPCHK(spark::amr_fill(L, b->amr_g, pool, vs, std::min(depth, b->p.use_hybrid_riemann ? 2 : 1), nullptr, 0,
-                             "k_amr_fill_shock_layer"));
+      HIPCHK(spark::amr_halo_copy(L, b->nbr_d, pool, vs, std::min(depth, b->p.use_hybrid_riemann ? 2 : 1)));
     }
   } else if (b->amr) {
     HIPCHK(spark::amr_fill(L, b->amr_g, pool, vl, depth));

@@ -89,8 +89,11 @@ struct AmrGrid {
 enum { AMR_BC_OUTFLOW = 0, AMR_BC_PERIODIC = 1, AMR_BC_REFLECT = 2 };
 // smask (optional, per block): guard sectors filled to `depth`; the others
 // only to `keep` layers
+// guard cells of the listed blocks (cubic, all 26 neighbour-table entries
+// same-level leaves) copied from their neighbours' interiors, `depth` layers
+hipError_t amr_halo_copy(const PlainLaunch& L, const int64_t* nbr, double* U, const VarList& vars, int depth);
 hipError_t amr_fill(const PlainLaunch& L, const AmrGrid& G, double* U, const VarList& vars, int depth,
-                    const uint32_t* smask = nullptr, int keep = 0, const char* prof_name = "k_amr_fill");
+                    const uint32_t* smask = nullptr, int keep = 0);
 hipError_t amr_flux_exchange(const PlainLaunch& L, const AmrGrid& G, double* const fb[3], double* const fc[3]);
 hipError_t plain_aos_to_soa(hipStream_t st, const BlockMeta* meta, int nblk, int maxcells, const double* aos,
                             const int64_t* aos_off, int nunk, const int* var_index, int nvar, double* U);
