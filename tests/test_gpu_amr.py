@@ -194,8 +194,9 @@ def test_same_level_halos_equal_telescoping(form, hybrid):
     assert nd == 0
 
 
-def test_fp32_flux_amr_same_level_halos_vs_plain():
-    prob = three_level(nscalars=8)
+@pytest.mark.parametrize("hybrid", [0, 1])
+def test_fp32_flux_amr_same_level_halos_vs_plain(hybrid):
+    prob = three_level(nscalars=8, **(SHOCKED if hybrid else {}))
     nsteps = 10
     d64, o64, _ = run(prob, nsteps, "plain_fp64")
     d32, o32, cnt = run(prob, nsteps, "fp32_flux")
