@@ -300,10 +300,11 @@ def main():
         del res
 
     tot = sum(s["ms"] for s in stats)
-    flux = [s for s in stats if s["name"].startswith("k_flux")]
+    sweeps = ("k_flux_x", "k_flux_y", "k_flux_z")  # one kernel template (not k_flux_exchange)
+    flux = [s for s in stats if s["name"] in sweeps]
     fam = {}
-    for s_ in stats:   # the three sweeps are one kernel template
-        k = "k_flux" if s_["name"].startswith("k_flux") else s_["name"]
+    for s_ in stats:
+        k = "k_flux" if s_["name"] in sweeps else s_["name"]
         fam[k] = fam.get(k, 0.0) + s_["ms"]
     dom = max(fam, key=fam.get)
     fl_ms = sum(s["ms"] for s in flux)
