@@ -56,7 +56,7 @@ constexpr int NT = FUSED_THREADS;
 // FP32 form takes gamc = gamma (the Gamma EOS sets every cell's gamc to
 // gamma), so it neither stages nor reconstructs it.
 constexpr int NH = kF64 ? 7 : 6;
-constexpr int CH = 4;  // mass scalars per chunk
+constexpr int CH = FUSED_SCALAR_CHUNK;  // mass scalars per chunk (the species all in the first)
 constexpr int SL_RHOE = NH - 1;
 
 __host__ __device__ __forceinline__ constexpr int hslot(int s) { return (!kF64 && s >= R_GAMC) ? s + 1 : s; }
@@ -218,6 +218,31 @@ __device__ __forceinline__ void face_states(const Real* __restrict__ bp, const R
     R[hslot(s)] = bm[s * nr + ir];
   }
   if (!kF64) { L[R_GAMC] = Real(0.); R[R_GAMC] = Real(0.); }
+}
+
+// species face states: clamp to [smallx, 1] and renormalise, left and right
+// (Hydro.ini:227-235); x[j], j < nspec (all in the first scalar chunk)
+__device__ __forceinline__ void species_faces(const DevParams& P, Real* xl, Real* xr) {
+  if (P.nspec <= 0) return;
+  const Real lo = (Real)P.smallx;
+  Real sl = Real(0.), sr = Real(0.);
+#pragma unroll
+  for (int j = 0; j < CH; j++) {
+    if (j < P.nspec) {
+      xl[j] = fmax2(lo, fmin2(Real(1.), xl[j]));
+      xr[j] = fmax2(lo, fmin2(Real(1.), xr[j]));
+      sl = sl + xl[j];
+      sr = sr + xr[j];
+    }
+  }
+  const Real il = Num<Real>::rcp(sl), ir = Num<Real>::rcp(sr);
+#pragma unroll
+  for (int j = 0; j < CH; j++) {
+    if (j < P.nspec) {
+      xl[j] = xl[j] * il;
+      xr[j] = xr[j] * ir;
+    }
+  }
 }
 
 __device__ __forceinline__ void centred(const Real* __restrict__ plane, int psz, int pc, Real* c) {
@@ -549,6 +574,10 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
       // ---- P5: update of level k-1 (hydro part), x/y differences of level k
       const bool upd = col && k - 1 >= I.z0 && k - 1 <= I.z1;
       double d0 = 0., ds = 0., dens1 = 1.;
+      // Multigamma (FP64 form): the EOS of level k-1 waits for its updated
+      // species (first scalar chunk); dens vx vy vz ener eint meanwhile
+      const bool defer_eos = kF64 && P.eos == 1;
+      double mg[6] = {0., 0., 0., 0., 0., 0.};
       if (upd) {
         // FP64 (hy_rk_updateSoln.F90-mc:227-287)
         const int64_t c = colc + (int64_t)(k - 1) * sz;
@@ -585,18 +614,22 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
         const double ener = fmax2(P.smalle, Bu[F_ENER] / dens);
         const double ekin = .5 * (vx * vx + vy * vy + vz * vz);
         const double eint = fmax2(P.smalle, ener - ekin);
-        EosOut e = eos_dens_ei(P, dens, vx, vy, vz, ener, eint);
         sout[V_DENS * cs + c] = dens;
         sout[V_VELX * cs + c] = vx;
         sout[V_VELY * cs + c] = vy;
         sout[V_VELZ * cs + c] = vz;
-        sout[V_PRES * cs + c] = e.pres;
-        sout[V_GAMC * cs + c] = P.gamma;
-        sout[V_EINT * cs + c] = e.eint;
-        sout[V_ENER * cs + c] = e.ener;
-        if (A.final_stage) {
-          sout[V_TEMP * cs + c] = e.temp;
-          sout[V_GAME * cs + c] = e.game;
+        if (defer_eos) {
+          mg[0] = dens; mg[1] = vx; mg[2] = vy; mg[3] = vz; mg[4] = ener; mg[5] = eint;
+        } else {
+          EosOut e = eos_dens_ei(P, dens, vx, vy, vz, ener, eint);
+          sout[V_PRES * cs + c] = e.pres;
+          sout[V_GAMC * cs + c] = P.gamma;
+          sout[V_EINT * cs + c] = e.eint;
+          sout[V_ENER * cs + c] = e.ener;
+          if (A.final_stage) {
+            sout[V_TEMP * cs + c] = e.temp;
+            sout[V_GAME * cs + c] = e.game;
+          }
         }
         dens1 = dens;
       }
@@ -674,11 +707,15 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
 #pragma unroll
           for (int j = 0; j < CH; j++) Hs[j] = Real(0.);
           if (col && k >= I.z0) {
+            Real xl[CH], xr[CH];
+#pragma unroll
+            for (int j = 0; j < CH; j++) { xl[j] = upzs[c0 + j]; xr[j] = ums[j]; }
+            if (c0 == 0) species_faces(P, xl, xr);
 #pragma unroll
             for (int j = 0; j < CH; j++) {
               if (j < nc) {
-                if (Hz[F_MASS] > Real(0.)) Hs[j] = upzs[c0 + j] * Hz[F_MASS];
-                else Hs[j] = ums[j] * Hz[F_MASS];
+                if (Hz[F_MASS] > Real(0.)) Hs[j] = xl[j] * Hz[F_MASS];
+                else Hs[j] = xr[j] * Hz[F_MASS];
               }
             }
             if (regs && colin && (k == g || k == g + m.nz)) {
@@ -705,11 +742,24 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
               }
               Real* fo = isx ? fxb : fyb;
               const int nf = isx ? nfx : nfy;
-              for (int j = 0; j < nc; j++) {
-                Real f;
-                if (fm > Real(0.)) f = bp[j * nr + il] * fm;
-                else f = bm[j * nr + ir] * fm;
-                fo[j * nf + w] = f;
+              if (c0 == 0 && P.nspec > 0) {
+                Real xl[CH], xr[CH];
+#pragma unroll
+                for (int j = 0; j < CH; j++) {
+                  xl[j] = j < nc ? bp[j * nr + il] : Real(0.);
+                  xr[j] = j < nc ? bm[j * nr + ir] : Real(0.);
+                }
+                species_faces(P, xl, xr);
+#pragma unroll
+                for (int j = 0; j < CH; j++)
+                  if (j < nc) fo[j * nf + w] = fm > Real(0.) ? xl[j] * fm : xr[j] * fm;
+              } else {
+                for (int j = 0; j < nc; j++) {
+                  Real f;
+                  if (fm > Real(0.)) f = bp[j * nr + il] * fm;
+                  else f = bm[j * nr + ir] * fm;
+                  fo[j * nf + w] = f;
+                }
               }
               if (regs) {
                 const int i = I.x0 + fi, jj = I.y0 + fj;
@@ -739,6 +789,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
           // S3: scalar update of level k-1, x/y differences of level k
           if (upd) {
             const int64_t c = colc + (int64_t)(k - 1) * sz;
+            double Xn[8] = {0., 0., 0., 0., 0., 0., 0., 0.};  // updated species of the first chunk
 #pragma unroll
             for (int j = 0; j < CH; j++) {
               if (j < nc) {
@@ -748,7 +799,22 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
                 double inner_n = dUxy[n] - dtdz * ((double)Hs[j] - (double)Hprev[n]);
                 inner_n = inner_n + dt * (0.0 + 0.0);
                 Bu = A.c1 * Au + A.c2 * Bu + A.c3 * inner_n;
-                sout[(V_NPROP + c0 + j) * cs + c] = Bu / dens1;
+                const double x = Bu / dens1;
+                sout[(V_NPROP + c0 + j) * cs + c] = x;
+                if (c0 == 0) Xn[j] = x;
+              }
+            }
+            if (defer_eos && c0 == 0) {
+              // the EOS of level k-1 with its updated species (Eos_wrapped
+              // after updateSoln)
+              const EosOut e = eos_dens_ei(P, mg[0], mg[1], mg[2], mg[3], mg[4], mg[5], Xn);
+              sout[V_PRES * cs + c] = e.pres;
+              sout[V_GAMC * cs + c] = e.gamc;
+              sout[V_EINT * cs + c] = e.eint;
+              sout[V_ENER * cs + c] = e.ener;
+              if (A.final_stage) {
+                sout[V_TEMP * cs + c] = e.temp;
+                sout[V_GAME * cs + c] = e.game;
               }
             }
           }

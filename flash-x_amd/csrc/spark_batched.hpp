@@ -9,6 +9,9 @@ namespace spark {
 
 constexpr int FUSED_THREADS = 256;    // workgroup size of the fused stage kernel
 constexpr int FUSED_XCDS = 8;         // MI355X: 8 XCDs, workgroups dispatched round-robin
+// mass scalars staged together; species (clamped and renormalised
+// together at faces) must all fall in the first chunk
+constexpr int FUSED_SCALAR_CHUNK = 4;
 
 // one work item: a (tx x ty) column tile of one block's stage region,
 // marched over z0..z1 (guard-celled box coordinates)

@@ -904,9 +904,14 @@ int spark_batch_create(const spark_params* p, int nblocks, const spark_block_des
   }
   if (p->form == SPARK_FORM_FP32_FLUX || p->form == SPARK_FORM_BATCHED_FP64) {
     if (p->ndim != 3) return fail(SPARK_ERR_ARG, "the batched form is 3D only (ndim = %d)", p->ndim);
-    if (p->nspecies != 0 || p->eos != SPARK_EOS_GAMMA)
-      return fail(SPARK_ERR_ARG, "the batched forms implement the gamma law with mass scalars only "
-                                 "(nspecies = %d, eos = %d)", p->nspecies, p->eos);
+    // species are clamped and renormalised together, within the first
+    // chunk of scalars the fused kernel stages
+    if (p->nspecies > spark::FUSED_SCALAR_CHUNK)
+      return fail(SPARK_ERR_ARG, "the batched forms take at most %d species (nspecies = %d)",
+                  spark::FUSED_SCALAR_CHUNK, p->nspecies);
+    // the FP32 reconstruction takes gamc = gamma and so needs the gamma law
+    if (p->form == SPARK_FORM_FP32_FLUX && p->eos != SPARK_EOS_GAMMA)
+      return fail(SPARK_ERR_ARG, "the FP32-flux form implements the gamma law only (eos = %d)", p->eos);
   }
 
   spark_batch* b = new spark_batch();

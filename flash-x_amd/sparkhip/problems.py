@@ -150,21 +150,67 @@ def sod2d_species(nbx=4, nby=4, n=16, nguard=10, species_a=(1.0, 4.0), species_g
             dens = 1.0 + (0.125 - 1.0) * right
             pres = 1.0 + (0.1 - 1.0) * right
             Xs = np.stack([1.0 - right, right] + [np.zeros_like(right)] * (ns - 2), axis=-1)
-            abar, gamc = mixture_thermo(Xs, species_a, species_gamma)
             U = np.zeros((NZ, NY, NX, nunk))
-            eint = pres / ((gamc - 1.0) * dens)
-            U[..., VI["dens"]] = dens
-            U[..., VI["pres"]] = pres
-            U[..., VI["eint"]] = eint
-            U[..., VI["ener"]] = eint
-            U[..., VI["gamc"]] = gamc
-            U[..., VI["game"]] = pres / (eint * dens) + 1.0
-            U[..., VI["temp"]] = eint * (gamc - 1.0) * abar / GAS_CONST
-            for q in range(ns):
-                U[..., SPARK_NPROP + q] = Xs[..., q]
+            _fill_mixture(U, dens, pres, Xs, species_a, species_gamma)
             blocks.append(b)
             arrays.append(U)
     return Problem("sod2d_species", cfg, blocks, arrays, ug=(nbx, nby, 1))
+
+
+def _fill_mixture(U, dens, pres, Xs, species_a, species_gamma, scalars=None):
+    """A fluid at rest with a Multigamma mixture Xs (..., ns): eint, gamc,
+    game and T from the mixture; further mass scalars after the species."""
+    abar, gamc = mixture_thermo(Xs, species_a, species_gamma)
+    eint = pres / ((gamc - 1.0) * dens)
+    U[..., VI["dens"]] = dens
+    U[..., VI["pres"]] = pres
+    U[..., VI["eint"]] = eint
+    U[..., VI["ener"]] = eint
+    U[..., VI["gamc"]] = gamc
+    U[..., VI["game"]] = pres / (eint * dens) + 1.0
+    U[..., VI["temp"]] = eint * (gamc - 1.0) * abar / GAS_CONST
+    ns = Xs.shape[-1]
+    for q in range(ns):
+        U[..., SPARK_NPROP + q] = Xs[..., q]
+    for q, Xq in enumerate(scalars or []):
+        U[..., SPARK_NPROP + ns + q] = Xq
+
+
+def species_tube3d(nb=2, n=16, nguard=10, species_a=(1.0, 4.0), species_gamma=(1.4, 5.0 / 3.0), width=0.03,
+                   nscalars=None, eos="multigamma", **kw) -> Problem:
+    """Two-gas shock tube on [0,1]^3: the Sod states (rho, p) = (1, 1) |
+    (0.125, 0.1) across x + y + z = 1.5, gas 1 on the low side and gas 2 on
+    the high side, the mass fractions blended by a tanh of the given width.
+    nscalars (default: the species count) adds passive mass scalars after
+    the species (smooth profiles in z).  eos "multigamma" takes gamc, eint
+    and T from the mixture; "gamma" advects the species under the gamma law
+    of gas 1."""
+    ns = len(species_a)
+    nsc = ns if nscalars is None else nscalars
+    cfg = base_cfg(ndim=3, nguard=nguard, gamma=float(species_gamma[0]), nscalars=nsc, nspecies=ns, eos=eos,
+                   species_a=tuple(species_a), species_gamma=tuple(species_gamma), **kw)
+    if eos == "gamma":
+        species_gamma = (float(species_gamma[0]),) * ns
+        species_a = (1.0,) * ns
+    dx = 1.0 / (nb * n)
+    blocks, arrays = [], []
+    nunk = SPARK_NPROP + nsc
+    for bk in range(nb):
+        for bj in range(nb):
+            for bi in range(nb):
+                b = dict(nx=n, ny=n, nz=n, level=1, dx=dx, dy=dx, dz=dx)
+                NZ, NY, NX = gc_shape(cfg, b)
+                Z, Y, X = _centers(cfg, b, (bi * n * dx, bj * n * dx, bk * n * dx), NZ, NY, NX)
+                right = 0.5 * (1.0 + np.tanh((X + Y + Z - 1.5) / np.sqrt(3.0) / width))
+                dens = 1.0 + (0.125 - 1.0) * right
+                pres = 1.0 + (0.1 - 1.0) * right
+                Xs = np.stack([1.0 - right, right] + [np.zeros_like(right)] * (ns - 2), axis=-1)
+                extra = [0.5 * (1.0 + np.tanh((Z - 0.5) / (0.05 * (q + 1)))) for q in range(nsc - ns)]
+                U = np.zeros((NZ, NY, NX, nunk))
+                _fill_mixture(U, dens, pres, Xs, species_a, species_gamma, extra)
+                blocks.append(b)
+                arrays.append(U)
+    return Problem("species_tube3d", cfg, blocks, arrays, ug=(nb, nb, nb))
 
 
 def sedov3d(nb=8, n=16, nguard=6, gamma=1.4, E=1.0, r_cells=3.5, p_amb=1e-5, rho=1.0, center=(0.5, 0.5, 0.5),

@@ -122,6 +122,62 @@ def test_batched_fp64_hybrid_riemann_is_bit_exact_with_the_plain_form(case):
     assert any(not np.array_equal(a, c) for a, c in zip(interiors(prob, o64), interiors(prob, o0)))
 
 
+SPECIES_CASES = {
+    # two gases, Multigamma EOS: gamc per cell from the mixture; telescoping
+    "multigamma": dict(),
+    # the same on a uniform grid with device refills (halos read from the
+    # neighbouring blocks in the batched forms)
+    "multigamma_refill": dict(nguard=6, guard_mode="refill"),
+    # species under the gamma law with 4 more mass scalars: the first scalar
+    # chunk mixes species and passive scalars
+    "gamma_plus_scalars": dict(eos="gamma", nscalars=6),
+}
+
+
+@pytest.mark.parametrize("case", sorted(SPECIES_CASES))
+def test_species_batched_fp64_bit_exact_with_plain_and_oracle(case):
+    """Species faces clamped to [smallx, 1] and renormalised (Hydro.ini:
+    227-235) and, with Multigamma, the EOS of the updated species: the plain
+    form equals the oracle and the batched FP64 form the plain form, bit for
+    bit."""
+    prob = problems.species_tube3d(nb=2, n=16, **SPECIES_CASES[case])
+    nsteps = 3
+    d64, o64, _, c64 = run_gpu(prob, nsteps, "plain_fp64")
+    db, ob, _, cb = run_gpu(prob, nsteps, "batched_fp64")
+    ref = [a.copy() for a in prob.arrays]
+    dref, nerr = pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, ref, nsteps,
+                                     refill=prob.cfg["guard_mode"] == "refill")
+    assert nerr == 0 and c64[0] == 0 and cb[0] == 0
+    assert np.array_equal(d64, dref), (d64, dref)
+    assert np.array_equal(d64, db), (d64, db)
+    n_or = sum(int(np.count_nonzero(a != c)) for a, c in zip(interiors(prob, o64), interiors(prob, ref)))
+    n_b = sum(int(np.count_nonzero(a != c)) for a, c in zip(interiors(prob, o64), interiors(prob, ob)))
+    V = capi.VI
+    gc = [float(np.ptp(a[..., V["gamc"]])) for a in interiors(prob, o64)]
+    print(f"{case}: differing interior values plain/oracle {n_or}, batched/plain {n_b}; gamc spread {max(gc):.3f}",
+          flush=True)
+    assert n_or == 0 and n_b == 0
+
+
+def test_fp32_flux_species_gamma_law_vs_oracle():
+    prob = problems.species_tube3d(nb=2, n=16, eos="gamma", nscalars=6)
+    nsteps = 5
+    d32, out32, _, cnt = run_gpu(prob, nsteps, "fp32_flux")
+    ref = [a.copy() for a in prob.arrays]
+    pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, ref, nsteps)
+    nm = norms(prob, out32, ref)
+    g = prob.cfg["nguard"]
+    xs = [np.concatenate([a[g:-g, g:-g, g:-g, 11 + s].ravel() for a in out32]) for s in range(2)]
+    xr = [np.concatenate([a[g:-g, g:-g, g:-g, 11 + s].ravel() for a in ref]) for s in range(2)]
+    lx = [float(np.mean(np.abs(x - r)) / np.mean(np.abs(r))) for x, r in zip(xs, xr)]
+    tot = np.abs(xs[0] + xs[1] - 1.0).max()
+    print(", ".join(f"{k}: L1 {v[0]:.2e}" for k, v in nm.items()) + f"; species L1 {lx}, max |X1+X2-1| {tot:.1e}",
+          flush=True)
+    assert cnt[0] == 0
+    assert nm["rho"][0] <= 1e-5 and nm["E"][0] <= 1e-5
+    assert max(lx) <= 1e-5
+
+
 def test_fp32_flux_accuracy_mixed_blocks_vs_oracle():
     prob = problems.batched(nblocks=24, nguard=10, shock_every=4)
     nsteps = 10

@@ -239,9 +239,13 @@ def test_species_multigamma_matches_oracle(guard_mode):
     compare(interior_views(prob, out), interior_views(prob, ref), f"multigamma {guard_mode}")
 
 
-def test_batched_forms_reject_species():
-    prob = problems.sod2d_species(nbx=1, nby=1, n=16)
-    cfg = dict(prob.cfg, ndim=3, form="fp32_flux")
-    blocks = [dict(b, nz=16, dz=b["dx"]) for b in prob.blocks]
-    with pytest.raises(capi.SparkError, match="gamma law with mass scalars only"):
-        capi.Batch(cfg, blocks)
+def test_batched_forms_reject_what_they_do_not_implement():
+    """The FP32 reconstruction takes gamc = gamma (no Multigamma), and the
+    species must fit the fused kernel's first scalar chunk."""
+    prob = problems.species_tube3d(nb=1, n=16)
+    with pytest.raises(capi.SparkError, match="FP32-flux form implements the gamma law only"):
+        capi.Batch(dict(prob.cfg, form="fp32_flux"), prob.blocks)
+    five = problems.species_tube3d(nb=1, n=16, species_a=(1.0, 4.0, 2.0, 3.0, 5.0),
+                                   species_gamma=(1.4, 5.0 / 3.0, 1.4, 1.4, 1.4))
+    with pytest.raises(capi.SparkError, match="at most 4 species"):
+        capi.Batch(dict(five.cfg, form="batched_fp64"), five.blocks)
