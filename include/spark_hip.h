@@ -86,8 +86,9 @@ typedef struct spark_params {
    * face states are clamped to [smallx, 1] and renormalised
    * (Spark/Hydro.ini:227-235).  eos = SPARK_EOS_MULTIGAMMA mixes them with
    * per-species gamma and mass number A (Multigamma/Eos.F90, ORIGINAL_GAMC_
-   * AVERAGE; eos_initMgamma.F90:12-18); needs nspecies >= 1.  Species and the
-   * Multigamma EOS are implemented in the plain FP64 form. */
+   * AVERAGE; eos_initMgamma.F90:12-18); needs nspecies >= 1.  The batched
+   * forms take at most 4 species, and the FP32-flux form the gamma law only
+   * (Multigamma runs in the plain and batched FP64 forms). */
   int nspecies;
   int eos;             /* enum spark_eos                                    */
   int reserved;        /* 0; keeps the arrays below 8-byte aligned with no implicit padding */
