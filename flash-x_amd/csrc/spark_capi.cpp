@@ -24,6 +24,7 @@
 
 #include "../../include/spark_hip.h"
 #include "spark_batched.hpp"
+#include "spark_regions.hpp"
 #include "spark_launch.hpp"
 
 using spark::BlockMeta;
@@ -565,11 +566,7 @@ std::vector<std::pair<int, int>> tile_candidates(const spark_batch* b, int ex, i
   return out;
 }
 
-// a box of a block's guard-celled index space, bounds inclusive
-struct Box {
-  int x0, x1, y0, y1, z0, z1;
-  double cost() const { return (double)(x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 3); }
-};
+using spark::Box;
 
 // the work items of a box: (tx x ty) column tiles marched over its z range
 void box_items(int blk, const Box& r, int tx, int ty, std::vector<spark::FusedItem>& out) {
@@ -591,60 +588,10 @@ Box region_box(const BlockMeta& m, int g, int ngoff) {
 }
 
 // stage region of a block that reads some guard sectors from same-level
-// leaves: its interior and the sectors it computes, each to depth e, as
-// disjoint boxes.  Per column sector (ox, oy) the z extent is the union of
-// its computed sectors (two runs when only the middle one is read from a
-// leaf); column sectors of one row with equal z runs merge along x, and
-// rows with equal boxes merge along y.
+// leaves (sector_boxes over its computed-sector mask)
 void partial_boxes(const spark_batch* b, int blk, int e, std::vector<Box>& out) {
   const BlockMeta& m = b->meta_h[blk];
-  const int g = b->p.nguard;
-  const uint32_t local = b->amr_local[blk];
-  auto span = [&](int o, int nn, int& lo, int& hi) {
-    if (o < 0) { lo = g - e; hi = g - 1; }
-    else if (o == 0) { lo = g; hi = g + nn - 1; }
-    else { lo = g + nn; hi = g + nn + e - 1; }
-  };
-  std::vector<Box> prev;
-  auto flush = [&](std::vector<Box>& row) { for (auto& r : row) out.push_back(r); row.clear(); };
-  for (int oy = -1; oy <= 1; oy++) {
-    int y0, y1;
-    span(oy, m.ny, y0, y1);
-    if (y1 < y0) continue;
-    std::vector<Box> row;
-    std::vector<std::pair<int, int>> runs_prev;
-    int run_x0 = 0, run_x1 = -1;
-    for (int ox = -1; ox <= 1; ox++) {
-      int x0, x1;
-      span(ox, m.nx, x0, x1);
-      if (x1 < x0) continue;
-      std::vector<std::pair<int, int>> runs;
-      for (int oz = -1; oz <= 1; oz++) {
-        const bool comp = (ox == 0 && oy == 0 && oz == 0) || ((local >> (13 + ox + 3 * oy + 9 * oz)) & 1u);
-        int z0, z1;
-        span(oz, m.nz, z0, z1);
-        if (!comp || z1 < z0) continue;
-        if (!runs.empty() && runs.back().second + 1 == z0) runs.back().second = z1;
-        else runs.push_back({z0, z1});
-      }
-      if (run_x1 >= run_x0 && runs == runs_prev) { run_x1 = x1; continue; }
-      for (auto& zr : runs_prev) row.push_back(Box{run_x0, run_x1, y0, y1, zr.first, zr.second});
-      runs_prev = runs;
-      run_x0 = x0; run_x1 = x1;
-    }
-    for (auto& zr : runs_prev) row.push_back(Box{run_x0, run_x1, y0, y1, zr.first, zr.second});
-    // merge with the previous row when the boxes match in x and z
-    bool same = !prev.empty() && prev.size() == row.size();
-    for (size_t t = 0; same && t < row.size(); t++)
-      same = row[t].x0 == prev[t].x0 && row[t].x1 == prev[t].x1 && row[t].z0 == prev[t].z0 && row[t].z1 == prev[t].z1;
-    if (same) {
-      for (auto& r : prev) r.y1 = y1;
-    } else {
-      flush(prev);
-      prev = row;
-    }
-  }
-  flush(prev);
+  spark::sector_boxes(m.nx, m.ny, m.nz, b->p.nguard, e, b->amr_local[blk], out);
 }
 
 // build the per-XCD queues of one stage from the chosen tiles.  A block's
