@@ -86,9 +86,9 @@ def run_case(prob, nsteps, label):
 
 
 def test_sod2d_full_config_telescope():
-    """2D Sod, 16x16 blocks of 16^2, WENO5+HLLC, RK3, CFL 0.8, 20 steps (NGUARD 10 telescoping)."""
+    """The full 2D Sod shock tube: 16x16 blocks of 16^2, WENO5+HLLC, RK3, CFL 0.8, 20 steps (NGUARD 10 telescoping)."""
     prob = problems.sod2d(nbx=16, nby=16, n=16, nguard=10)
-    run_case(prob, 20, "sod2d c0")
+    run_case(prob, 20, "sod2d 256^2")
 
 
 def test_sedov3d_small_telescope():
