@@ -148,7 +148,7 @@ def cpu_baseline(workload, target_s: float):
     t0 = time.perf_counter()
     run(1)
     one = time.perf_counter() - t0
-    nsteps = int(max(1, min(30, (target_s - one) / max(one, 1e-3))))
+    nsteps = int(max(1, min(100, (target_s - one) / max(one, 1e-3))))
     t0 = time.perf_counter()
     run(nsteps)
     el = time.perf_counter() - t0
