@@ -93,11 +93,14 @@ def test_guard_fill_boundary_codes_match_oracle(bc):
 # periodic), both on block faces of every level, with use_hybridRiemann, so
 # the faces there read SHOK across block and level boundaries
 SHOCKED = dict(use_hybrid_riemann=1, p_boost=(0.5, 1.0, 10.0))
+# the first two mass scalars as species (clamped and renormalised at faces)
+SPECIES = dict(nspecies=2, eos="gamma")
 
 
-@pytest.mark.parametrize("flux_correct,hybrid", [(1, 0), (0, 0), (1, 1)])
+@pytest.mark.parametrize("flux_correct,hybrid", [(1, 0), (0, 0), (1, 1), (1, "species")])
 def test_plain_fp64_amr_steps_match_oracle(flux_correct, hybrid):
-    prob = two_level(nscalars=8, flux_correct=flux_correct, **(SHOCKED if hybrid else {}))
+    extra = SHOCKED if hybrid == 1 else SPECIES if hybrid == "species" else {}
+    prob = two_level(nscalars=8, flux_correct=flux_correct, **extra)
     nsteps = 3
     print(f"RT 2 levels {prob.info['leaves_per_level']}, flux_correct={flux_correct}, hybrid={hybrid}, "
           f"{nsteps} steps", flush=True)
@@ -164,9 +167,9 @@ def same_level_blocks(prob):
     return full, some
 
 
-@pytest.mark.parametrize("hybrid", [0, 1])
-def test_batched_fp64_amr_same_level_halos_bit_exact_with_plain(hybrid):
-    prob = three_level(nscalars=8, **(SHOCKED if hybrid else {}))
+@pytest.mark.parametrize("case", ["gamma", "hybrid", "species"])
+def test_batched_fp64_amr_same_level_halos_bit_exact_with_plain(case):
+    prob = three_level(nscalars=8, **{"gamma": {}, "hybrid": SHOCKED, "species": SPECIES}[case])
     full, some = same_level_blocks(prob)
     print(f"RT 3 levels {prob.info['leaves_per_level']}: {full} blocks read all their halos from same-level "
           f"leaves, {some} some of them", flush=True)
