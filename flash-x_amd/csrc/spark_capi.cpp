@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cfloat>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -108,6 +109,8 @@ struct spark_batch {
   std::vector<char> amr_remap_blk;
   std::vector<uint32_t> amr_local;
   uint32_t* amr_smask_d = nullptr;  // per block the guard sectors it computes (all for telescoping blocks)
+  spark::DtCtl* dtctl_d = nullptr;  // the driver's dt limits for spark_advance (spark_set_dt_control)
+  bool dt_control = false;
   int amr_remap_count = 0;
   int amr_partial_count = 0;
   bool same_level = true;      // SPARK_OPT_SAME_LEVEL_HALOS
@@ -220,8 +223,11 @@ int run_dt(spark_batch* b, int hist_slot) {
     PlainLaunch L = make_launch(b, c);
     HIPCHK(spark::plain_dt_part(L, b->U, b->dtparts, b->dtchunks));
   }
+  // the driver's limits apply to the steps of spark_advance only (hist_slot
+  // >= 0); spark_compute_dt returns the CFL dt as Hydro_computeDt does
   HIPCHK(spark::plain_dt_final(g_stream, b->dp, b->dtparts, b->nblk, b->dtchunks, b->dtp, b->loc,
-                               hist_slot >= 0 ? b->dt_hist : nullptr, b->hist_slot));
+                               hist_slot >= 0 ? b->dt_hist : nullptr, b->hist_slot,
+                               hist_slot >= 0 && b->dt_control ? b->dtctl_d : nullptr));
   return SPARK_OK;
 }
 
@@ -501,6 +507,7 @@ void free_batch(spark_batch* b) {
   if (b->remap_ids_d) (void)hipFree(b->remap_ids_d);
   if (b->tele_ids_d) (void)hipFree(b->tele_ids_d);
   if (b->amr_smask_d) (void)hipFree(b->amr_smask_d);
+  if (b->dtctl_d) (void)hipFree(b->dtctl_d);
   if (b->amr_fmask_d) (void)hipFree(b->amr_fmask_d);
   if (b->corr_ids_d) (void)hipFree(b->corr_ids_d);
   if (b->amr_map_d) (void)hipFree(b->amr_map_d);
@@ -1245,6 +1252,28 @@ int spark_advance(spark_batch* b, int nsteps, double* dts) {
   }
   if (dts && nsteps > 0) {
     HIPCHK(hipMemcpyAsync(dts, b->dt_hist, sizeof(double) * nsteps, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+  }
+  return SPARK_OK;
+}
+
+int spark_set_dt_control(spark_batch* b, double dt_init, double change_factor, double dt_min, double dt_max) {
+  if (!b) return fail(SPARK_ERR_ARG, "spark_set_dt_control: null batch");
+  const double v[4] = {dt_init, change_factor, dt_min, dt_max};
+  for (double x : v)
+    if (!(x >= 0.0) || !std::isfinite(x)) return fail(SPARK_ERR_ARG, "spark_set_dt_control: %g is not a finite value >= 0", x);
+  if (dt_max > 0.0 && dt_min > dt_max) return fail(SPARK_ERR_ARG, "spark_set_dt_control: dt_min %g > dt_max %g", dt_min, dt_max);
+  const bool on = dt_init > 0.0 || change_factor > 0.0 || dt_min > 0.0 || dt_max > 0.0;
+  if (on && !b->dtctl_d) {
+    HIPCHK(hipMalloc((void**)&b->dtctl_d, sizeof(spark::DtCtl)));
+    drop_graph(b);  // the captured dt kernel takes the control block's address
+  }
+  if (on != b->dt_control) drop_graph(b);
+  b->dt_control = on;
+  if (on) {
+    // no previous dt before the first step: no growth limit on it
+    const spark::DtCtl c{dt_init > 0.0 ? dt_init : DBL_MAX, dt_init, change_factor, dt_min, dt_max};
+    HIPCHK(hipMemcpyAsync(b->dtctl_d, &c, sizeof(c), hipMemcpyHostToDevice, g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
   }
   return SPARK_OK;

@@ -62,8 +62,15 @@ struct PlainLaunch {
 hipError_t plain_stage(const PlainLaunch& L, int ngoff, double c1, double c2, double c3, int final_stage);
 hipError_t plain_shock_detect(const PlainLaunch& L, double* U, double* scratch, int band);
 hipError_t plain_dt_part(const PlainLaunch& L, const double* U, void* parts, int chunks);
+// the driver's limits on the step's dt (Driver_computeDt.F90:450-460 and
+// the first step's dtinit, Driver_evolveAll), kept on the device: `first` > 0
+// is taken as the next dt (then cleared), later dts are the CFL dt limited
+// to prev * factor (factor > 0) and clamped to [dtmin, dtmax] (each when > 0)
+struct DtCtl {
+  double prev, first, factor, dtmin, dtmax;
+};
 hipError_t plain_dt_final(hipStream_t st, const DevParams& P, const void* parts, int nblk_total, int chunks,
-                          double* dt_out, long long* loc_out, double* dt_hist, int* hist_slot);
+                          double* dt_out, long long* loc_out, double* dt_hist, int* hist_slot, DtCtl* ctl);
 hipError_t plain_set_scalar(hipStream_t st, double* p, double v);
 hipError_t plain_set_int(hipStream_t st, int* p, int v);
 size_t plain_dt_part_size();

@@ -489,7 +489,7 @@ __global__ void __launch_bounds__(256) k_dt_part(DevParams P, const BlockMeta* _
 __global__ void __launch_bounds__(256) k_dt_final(DevParams P, const DtPart* __restrict__ parts, int nblocks,
                                                   int chunks, double* __restrict__ dt_out,
                                                   long long* __restrict__ loc_out, double* __restrict__ dt_hist,
-                                                  int* __restrict__ hist_slot) {
+                                                  int* __restrict__ hist_slot, DtCtl* __restrict__ ctl) {
   __shared__ double sdt[256];
   __shared__ int sb[256];
   __shared__ long long sc[256];
@@ -518,12 +518,26 @@ __global__ void __launch_bounds__(256) k_dt_final(DevParams P, const DtPart* __r
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    dt_out[0] = sdt[0];
+    double dt = sdt[0];
+    if (ctl) {
+      // Driver_evolveAll's first dt, then Driver_computeDt.F90:450-460:
+      // dtNew = min(dtNew, dtOld * factor); dtNew = min(max(dtNew, dtMin), dtMax)
+      if (ctl->first > 0.) {
+        dt = ctl->first;
+        ctl->first = 0.;
+      } else {
+        if (ctl->factor > 0.) dt = fmin2(dt, ctl->prev * ctl->factor);
+        if (ctl->dtmin > 0.) dt = fmax2(dt, ctl->dtmin);
+        if (ctl->dtmax > 0.) dt = fmin2(dt, ctl->dtmax);
+      }
+      ctl->prev = dt;
+    }
+    dt_out[0] = dt;
     if (dt_hist) {
       // the slot counter lives on the device so that a captured step graph
       // records each replay's dt in the next slot
       const int h = *hist_slot;
-      dt_hist[h] = sdt[0];
+      dt_hist[h] = dt;
       *hist_slot = h + 1;
     }
     if (sb[0] < nblocks) {
@@ -847,9 +861,9 @@ hipError_t plain_dt_part(const PlainLaunch& L, const double* U, void* parts, int
 }
 
 hipError_t plain_dt_final(hipStream_t st, const DevParams& P, const void* parts, int nblk_total, int chunks,
-                          double* dt_out, long long* loc_out, double* dt_hist, int* hist_slot) {
+                          double* dt_out, long long* loc_out, double* dt_hist, int* hist_slot, DtCtl* ctl) {
   hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, st, P, (const DtPart*)parts, nblk_total, chunks, dt_out,
-                     loc_out, dt_hist, hist_slot);
+                     loc_out, dt_hist, hist_slot, ctl);
   return hipGetLastError();
 }
 

@@ -57,7 +57,8 @@ module spark_hip
             spark_hydro_step, spark_step_begin, spark_stage, spark_step_end, spark_advance, &
             spark_get_flux_registers, spark_set_flux_corrections, spark_flux_correct, &
             spark_set_option, spark_batched_tiles, spark_synchronize, spark_get_counters, &
-            spark_last_error, spark_profile, spark_profile_read, spark_set_amr_grid, spark_flux_exchange
+            spark_last_error, spark_profile, spark_profile_read, spark_set_amr_grid, spark_flux_exchange, &
+            spark_set_dt_control
   public :: spark_check, spark_set_abort_hook, spark_error_message
 
   interface
@@ -166,6 +167,13 @@ module spark_hip
        integer(c_int), value :: nsteps
        real(c_double), intent(out) :: dts(*)
      end function spark_advance
+
+     integer(c_int) function spark_set_dt_control(batch, dt_init, change_factor, dt_min, dt_max) &
+          bind(C, name="spark_set_dt_control")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: batch
+       real(c_double), value :: dt_init, change_factor, dt_min, dt_max
+     end function spark_set_dt_control
 
      integer(c_int) function spark_get_flux_registers(batch, fbx, fby, fbz) bind(C, name="spark_get_flux_registers")
        import :: c_int, c_ptr

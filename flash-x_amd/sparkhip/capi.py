@@ -94,6 +94,7 @@ def lib():
             "spark_stage": [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int],
             "spark_step_end": [vp],
             "spark_advance": [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
+            "spark_set_dt_control": [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double],
             "spark_get_flux_registers": [vp, dpp, dpp, dpp],
             "spark_set_flux_corrections": [vp, dpp, dpp, dpp],
             "spark_flux_correct": [vp, ctypes.c_double],
@@ -124,7 +125,7 @@ def exported_symbols():
         "spark_hydro_step", "spark_step_begin", "spark_stage", "spark_step_end", "spark_advance",
         "spark_get_flux_registers", "spark_set_flux_corrections", "spark_flux_correct", "spark_synchronize",
         "spark_get_counters", "spark_last_error", "spark_profile", "spark_profile_read", "spark_set_option",
-        "spark_batched_tiles", "spark_set_amr_grid", "spark_flux_exchange",
+        "spark_batched_tiles", "spark_set_amr_grid", "spark_flux_exchange", "spark_set_dt_control",
     ]
 
 
@@ -253,6 +254,11 @@ class Batch:
         dts = (ctypes.c_double * max(1, nsteps))()
         check(lib().spark_advance(self.h, int(nsteps), dts), "spark_advance")
         return np.array(dts[:nsteps])
+
+    def set_dt_control(self, dt_init=0.0, change_factor=0.0, dt_min=0.0, dt_max=0.0):
+        """The driver's dt limits for advance() (all 0: the CFL dt)."""
+        check(lib().spark_set_dt_control(self.h, float(dt_init), float(change_factor), float(dt_min), float(dt_max)),
+              "spark_set_dt_control")
 
     def synchronize(self):
         check(lib().spark_synchronize(self.h), "spark_synchronize")

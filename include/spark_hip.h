@@ -190,6 +190,15 @@ int spark_step_end(spark_batch* b);
  * Replaces: the Hydro / Driver_computeDt pair of the step loop,
  * Driver/DriverMain/Unsplit/Driver_evolveAll.F90:136-275. */
 int spark_advance(spark_batch* b, int nsteps, double* dts);
+/* The driver's limits on the dt of the spark_advance steps, kept on the
+ * device: the next step takes dt_init as its dt (Flash-X's dtinit, the
+ * first step of Driver_evolveAll) when dt_init > 0; every later step takes
+ * the CFL dt limited to the previous dt x change_factor (when > 0) and
+ * clamped to [dt_min, dt_max] (each when > 0), in that order.  All zero:
+ * the CFL dt (the default).  spark_compute_dt is not affected.
+ * Replaces: Driver/DriverMain/Driver_computeDt.F90:450-460 (dr_tstepChangeFactor,
+ * dr_dtMin, dr_dtMax) and dr_dtInit. */
+int spark_set_dt_control(spark_batch* b, double dt_init, double change_factor, double dt_min, double dt_max);
 
 /* Flux registers (weighted sum over stages) in the shape of hy_fluxBuf[XYZ]:
  * every face of the block's interior, (NFLUXES, nx+1, ny, nz) for x etc.

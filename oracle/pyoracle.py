@@ -5,6 +5,7 @@ the cpu_baseline leg of bench.py, always as the checker.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 from pathlib import Path
 
@@ -77,6 +78,8 @@ def lib():
         L.orc_ug_advance.restype = ctypes.c_int
         L.orc_batch_advance.argtypes = [P, B, ctypes.c_int, pp, ctypes.c_int, ctypes.c_int, _D]
         L.orc_batch_advance.restype = ctypes.c_int
+        L.orc_set_dt_control.argtypes = [ctypes.c_double] * 4
+        L.orc_set_dt_control.restype = None
         _lib = L
     return _lib
 
@@ -147,6 +150,17 @@ def ug_advance(cfg, b, ug, arrays, nsteps, refill=False, nthreads=0):
     nerr = lib().orc_ug_advance(ctypes.byref(P), ctypes.byref(B), ug[0], ug[1], ug[2], ptr_array(arrays), nsteps,
                                 int(refill), nthreads, dptr(dts))
     return dts[:nsteps], nerr
+
+
+@contextlib.contextmanager
+def dt_control(dt_init=0.0, change_factor=0.0, dt_min=0.0, dt_max=0.0):
+    """The driver's dt limits for the advance functions inside the block
+    (Driver_computeDt.F90:450-460 and dr_dtInit); reset to the CFL dt after."""
+    lib().orc_set_dt_control(float(dt_init), float(change_factor), float(dt_min), float(dt_max))
+    try:
+        yield
+    finally:
+        lib().orc_set_dt_control(0.0, 0.0, 0.0, 0.0)
 
 
 def ug_fill(cfg, b, ug, arrays):

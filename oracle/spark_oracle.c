@@ -919,6 +919,34 @@ double orc_ug_compute_dt(const orc_params* P, const orc_block* b, int nblocks, d
   return dt;
 }
 
+/* The driver's limits on each step's dt in the advance loops below
+ * (Driver_computeDt.F90:450-460, and dr_dtInit for the first step of
+ * Driver_evolveAll): dt_init > 0 is the next step's dt; later steps take
+ * min(dt_cfl, dt_prev * change_factor) (factor > 0), then max with dt_min
+ * and min with dt_max (each when > 0).  All zero: the CFL dt. */
+static double g_dt_first = 0.0, g_dt_prev = DBL_MAX, g_dt_factor = 0.0, g_dt_min = 0.0, g_dt_max = 0.0;
+
+void orc_set_dt_control(double dt_init, double change_factor, double dt_min, double dt_max) {
+  g_dt_first = dt_init;
+  g_dt_prev = dt_init > 0.0 ? dt_init : DBL_MAX;
+  g_dt_factor = change_factor;
+  g_dt_min = dt_min;
+  g_dt_max = dt_max;
+}
+
+static double driver_dt(double dt) {
+  if (g_dt_first > 0.0) {
+    dt = g_dt_first;
+    g_dt_first = 0.0;
+  } else {
+    if (g_dt_factor > 0.0) dt = fmin2(dt, g_dt_prev * g_dt_factor);
+    if (g_dt_min > 0.0) dt = fmax2(dt, g_dt_min);
+    if (g_dt_max > 0.0) dt = fmin2(dt, g_dt_max);
+  }
+  g_dt_prev = dt;
+  return dt;
+}
+
 int orc_ug_advance(const orc_params* P, const orc_block* b, int nbx, int nby, int nbz,
                    double* const* blocks, int nsteps, int refill, int nthreads, double* dts_out) {
 #ifdef _OPENMP
@@ -946,7 +974,7 @@ int orc_ug_advance(const orc_params* P, const orc_block* b, int nbx, int nby, in
     }
   }
   for (int step = 0; step < nsteps; step++) {
-    double dt = orc_ug_compute_dt(P, b, nblk, blocks);
+    double dt = driver_dt(orc_ug_compute_dt(P, b, nblk, blocks));
     if (dts_out) dts_out[step] = dt;
     orc_ug_fill_guards(P, b, nbx, nby, nbz, blocks, nunk);
     if (!refill) {
@@ -1002,6 +1030,7 @@ int orc_batch_advance(const orc_params* P, const orc_block* bl, int nblocks, dou
       double d = orc_compute_dt(P, &bl[i], blocks[i], NULL);
       if (d < dt) dt = d;
     }
+    dt = driver_dt(dt);
     if (dts_out) dts_out[step] = dt;
 #pragma omp parallel for schedule(dynamic) reduction(+ : nerr)
     for (int i = 0; i < nblocks; i++) nerr += orc_hydro_block(P, &bl[i], blocks[i], dt, NULL, NULL, NULL);
@@ -1235,6 +1264,7 @@ int orc_amr_advance(const orc_params* P, const orc_amr* A, const orc_block* bl, 
       double d = orc_compute_dt(P, &bl[i], blocks[i], NULL);
       if (d < dt) dt = d;
     }
+    dt = driver_dt(dt);
     if (dts_out) dts_out[step] = dt;
 #pragma omp parallel for schedule(dynamic) reduction(+ : nerr)
     for (int i = 0; i < nblocks; i++)

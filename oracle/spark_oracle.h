@@ -110,6 +110,9 @@ void orc_correct_fluxes(const orc_params* P, const orc_block* b, double* U, doub
  * multi-step driver in either parity (telescoping) or refill mode. */
 void orc_ug_fill_guards(const orc_params* P, const orc_block* b, int nbx, int nby, int nbz,
                         double* const* blocks, int nvar);
+/* The driver's dt limits for the advance loops (Driver_computeDt.F90:450-460
+ * and dr_dtInit); all zero: the CFL dt (the default). */
+void orc_set_dt_control(double dt_init, double change_factor, double dt_min, double dt_max);
 int orc_ug_advance(const orc_params* P, const orc_block* b, int nbx, int nby, int nbz,
                    double* const* blocks, int nsteps, int refill, int nthreads,
                    double* dts_out);

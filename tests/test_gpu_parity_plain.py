@@ -249,3 +249,63 @@ def test_batched_forms_reject_what_they_do_not_implement():
                                    species_gamma=(1.4, 5.0 / 3.0, 1.4, 1.4, 1.4))
     with pytest.raises(capi.SparkError, match="at most 4 species"):
         capi.Batch(dict(five.cfg, form="batched_fp64"), five.blocks)
+
+
+def test_advance_with_driver_dt_limits_matches_oracle():
+    """spark_set_dt_control: dtinit for the first step, then the CFL dt
+    limited to twice the previous one and clamped to dt_max
+    (Driver_computeDt.F90:450-460), kept on the device across graph
+    replays; the plain form equals the oracle with the same limits."""
+    prob = problems.sod2d(nbx=2, nby=2, n=16, nguard=10)
+    ctl = dict(dt_init=1e-5, change_factor=2.0, dt_max=1.5e-3)
+    nsteps = 12
+    b = capi.Batch(prob.cfg, prob.blocks)
+    try:
+        b.pack(prob.arrays)
+        b.set_uniform_grid(*prob.ug)
+        b.set_dt_control(**ctl)
+        d1 = b.advance(5)        # the first advance captures the step graph
+        d2 = b.advance(nsteps - 5)
+        out = clone(prob.arrays)
+        b.unpack(out)
+        b.set_dt_control()       # back to the CFL dt
+        d3 = b.advance(1)
+    finally:
+        b.close()
+    ref = clone(prob.arrays)
+    with pyoracle.dt_control(**ctl):
+        dref, nerr = pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, ref, nsteps)
+    d = np.concatenate([d1, d2])
+    print(f"dts {d}", flush=True)
+    assert nerr == 0
+    assert np.array_equal(d, dref), (d, dref)
+    assert d[0] == 1e-5 and d[1] == 2e-5 and d.max() == 1.5e-3
+    compare(interior_views(prob, out), interior_views(prob, ref), "sod2d with driver dt limits")
+    ref1 = clone(ref)
+    dcfl, _ = pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, ref1, 1)
+    assert np.array_equal(d3, dcfl)
+
+
+@pytest.mark.parametrize("form", ["batched_fp64", "fp32_flux"])
+def test_batched_forms_apply_the_driver_dt_limits(form):
+    prob = problems.sedov3d(nb=2, n=16, nguard=6, guard_mode="refill")
+    ctl = dict(dt_init=1e-6, change_factor=2.0, dt_min=0.0, dt_max=0.0)
+    res = {}
+    for f in ("plain_fp64", form):
+        b = capi.Batch(dict(prob.cfg, form=f), prob.blocks)
+        try:
+            b.pack(prob.arrays)
+            b.set_uniform_grid(*prob.ug)
+            b.set_dt_control(**ctl)
+            dts = b.advance(6)
+            out = clone(prob.arrays)
+            b.unpack(out)
+        finally:
+            b.close()
+        res[f] = (dts, out)
+    assert list(res[form][0]) == [1e-6 * 2.0 ** k for k in range(6)]
+    assert np.array_equal(res["plain_fp64"][0], res[form][0])
+    if form == "batched_fp64":
+        nd = sum(int(np.count_nonzero(a != c)) for a, c in
+                 zip(interior_views(prob, res["plain_fp64"][1]), interior_views(prob, res[form][1])))
+        assert nd == 0

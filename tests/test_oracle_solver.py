@@ -297,3 +297,31 @@ def test_species_stay_normalised_and_gamc_follows_the_mixture():
         abar, gamc = problems.mixture_thermo(X, prob.cfg["species_a"], prob.cfg["species_gamma"])
         np.testing.assert_allclose(c[..., 3], gamc, rtol=1e-14)
         np.testing.assert_allclose(c[..., 5], c[..., 0] * c[..., 1] * (gamc - 1.0), rtol=1e-13)
+
+
+def test_driver_dt_limits_follow_driver_computeDt():
+    """The oracle's driver dt limits (Driver_computeDt.F90:450-460 and
+    dr_dtInit) against their closed form on a Sod tube: from dt_init the dt
+    doubles exactly (powers of two) while far below the CFL dt; dt_max and
+    dt_min clamp every step; with no dt_init the first step takes the CFL dt
+    with no growth limit; and the control resets to the CFL dt."""
+    prob = problems.sod2d(nbx=2, nby=2, n=16, nguard=10)
+
+    def adv(n, **ctl):
+        arrs = [a.copy() for a in prob.arrays]
+        with pyoracle.dt_control(**ctl):
+            d, nerr = pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, arrs, n)
+        assert nerr == 0
+        return d
+
+    cfl = adv(3)
+    assert cfl[0] > 1e-3
+    d = adv(6, dt_init=1e-6, change_factor=2.0)
+    assert list(d) == [1e-6 * 2.0 ** k for k in range(6)]
+    d = adv(4, dt_max=0.5 * cfl[0])
+    assert np.all(d == 0.5 * cfl[0])
+    d = adv(3, dt_min=4.0 * cfl[0])
+    assert np.all(d == 4.0 * cfl[0])
+    d = adv(3, change_factor=1.5)
+    assert d[0] == cfl[0] and np.all(d[1:] <= 1.5 * d[:-1])
+    assert np.array_equal(adv(3), cfl)
