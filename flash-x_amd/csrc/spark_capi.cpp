@@ -110,6 +110,7 @@ struct spark_batch {
   std::vector<uint32_t> amr_local;
   uint32_t* amr_smask_d = nullptr;  // per block the guard sectors it computes (all for telescoping blocks)
   spark::DtCtl* dtctl_d = nullptr;  // the driver's dt limits for spark_advance (spark_set_dt_control)
+  double* geo_d = nullptr;          // per block radius / low x-face area / volume rows (spark_set_geometry)
   bool dt_control = false;
   int amr_remap_count = 0;
   int amr_partial_count = 0;
@@ -508,6 +509,7 @@ void free_batch(spark_batch* b) {
   if (b->tele_ids_d) (void)hipFree(b->tele_ids_d);
   if (b->amr_smask_d) (void)hipFree(b->amr_smask_d);
   if (b->dtctl_d) (void)hipFree(b->dtctl_d);
+  if (b->geo_d) (void)hipFree(b->geo_d);
   if (b->amr_fmask_d) (void)hipFree(b->amr_fmask_d);
   if (b->corr_ids_d) (void)hipFree(b->corr_ids_d);
   if (b->amr_map_d) (void)hipFree(b->amr_map_d);
@@ -898,6 +900,7 @@ int spark_batch_create(const spark_params* p, int nblocks, const spark_block_des
   d.ndim = p->ndim; d.ng = p->nguard; d.ns = p->nscalars;
   d.use_grav = p->use_gravity; d.use_flat = p->use_flattening; d.hybrid = p->use_hybrid_riemann;
   d.recon = p->recon; d.riemann = p->riemann;
+  d.geom = 0; d.geo_nx = 0; d.geo = nullptr;  // Cartesian until spark_set_geometry
 
   const int g = p->nguard;
   const int nf = 5 + p->nscalars;
@@ -1254,6 +1257,42 @@ int spark_advance(spark_batch* b, int nsteps, double* dts) {
     HIPCHK(hipMemcpyAsync(dts, b->dt_hist, sizeof(double) * nsteps, hipMemcpyDeviceToHost, g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
   }
+  return SPARK_OK;
+}
+
+int spark_set_geometry(spark_batch* b, int geometry, const double* const* r_center, const double* const* x_face_area,
+                       const double* const* cell_volume) {
+  if (!b) return fail(SPARK_ERR_ARG, "spark_set_geometry: null batch");
+  if (geometry == SPARK_GEOM_CARTESIAN) {
+    b->dp.geom = 0;
+    b->dp.geo = nullptr;
+    drop_graph(b);
+    return SPARK_OK;
+  }
+  if (geometry != SPARK_GEOM_CYLINDRICAL) return fail(SPARK_ERR_ARG, "spark_set_geometry: geometry %d", geometry);
+  if (b->fused) return fail(SPARK_ERR_ARG, "spark_set_geometry: the batched forms are Cartesian only");
+  // Grid_getCellVolumes supports cylindrical geometry in 2D only (:49-55)
+  if (b->p.ndim != 2) return fail(SPARK_ERR_ARG, "spark_set_geometry: cylindrical geometry is 2D (r, z) only");
+  if (b->classes.size() != 1) return fail(SPARK_ERR_ARG, "spark_set_geometry: blocks of one shape only");
+  if (!r_center || !x_face_area || !cell_volume) return fail(SPARK_ERR_ARG, "spark_set_geometry: null argument");
+  const int NX = b->meta_h[0].NX;
+  std::vector<double> h((size_t)b->nblk * 3 * NX);
+  for (int i = 0; i < b->nblk; i++) {
+    if (!r_center[i] || !x_face_area[i] || !cell_volume[i])
+      return fail(SPARK_ERR_ARG, "spark_set_geometry: block %d array is null", i);
+    for (int q = 0; q < NX; q++) {
+      if (!(cell_volume[i][q] != 0.0)) return fail(SPARK_ERR_ARG, "block %d: cell volume %d is zero", i, q);
+      h[((size_t)i * 3 + 0) * NX + q] = r_center[i][q];
+      h[((size_t)i * 3 + 1) * NX + q] = x_face_area[i][q];
+      h[((size_t)i * 3 + 2) * NX + q] = cell_volume[i][q];
+    }
+  }
+  if (!b->geo_d) HIPCHK(hipMalloc((void**)&b->geo_d, sizeof(double) * h.size()));
+  HIPCHK(hipMemcpy(b->geo_d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+  b->dp.geom = 1;
+  b->dp.geo_nx = NX;
+  b->dp.geo = b->geo_d;
+  drop_graph(b);  // the kernels take the parameters by value
   return SPARK_OK;
 }
 

@@ -329,7 +329,36 @@ struct DevParams {
   double gasConst;
   double mg_gm1[8];     // 1/(gamma_s - 1) of each species
   double mg_ainv[8];    // 1/A_s
+  // geometry: 0 Cartesian, 1 cylindrical (r, z[, phi]).  geo holds per block
+  // (BlockMeta::id) three rows of geo_nx values over the guard-celled x
+  // index: cell-centre radius, area of the cell's low x face, cell volume
+  // (the Grid's hy_xCenter, hy_farea, hy_cvol; Hydro.ini:40-56)
+  int geom, geo_nx;
+  const double* geo;
 };
+
+// geoFacs for the x faces of cell i (hy_rk_updateSoln.F90-mc:98-175,
+// hy_rk_correctFluxes.F90:341-358): facM, facP = farea * dx / cvol of the
+// low and high face, and the cell-centre radius; all factors 1 (and rc 0)
+// in Cartesian geometry.  Returns false where the reference zeroes them
+// (a cell centre at negative radius).
+__device__ __forceinline__ bool geo_facs(const DevParams& P, int id, int i, double dx, double& facM, double& facP,
+                                         double& rc) {
+  facM = 1.0;
+  facP = 1.0;
+  rc = 0.0;
+  if (P.geom == 0) return true;
+  const double* g = P.geo + (int64_t)id * 3 * P.geo_nx;
+  rc = g[i];
+  facM = g[P.geo_nx + i] * dx / g[2 * P.geo_nx + i];
+  facP = g[P.geo_nx + i + 1] * dx / g[2 * P.geo_nx + i];
+  if (rc < 0.0) {
+    facM = 0.;
+    facP = 0.;
+    return false;
+  }
+  return true;
+}
 
 // Per-block table entry (device resident)
 struct BlockMeta {

@@ -333,13 +333,21 @@ __global__ void __launch_bounds__(256) k_update(DevParams P, const BlockMeta* __
     const int64_t fpx = m.fbase + fix(m, i + 1, j, k);
     const int64_t fpy = m.fbase + fix(m, i, j + 1, k);
     const int64_t fpz = m.fbase + fix(m, i, j, k + 1);
+    // geometric factors and sources (geoFacs, hy_rk_updateSoln.F90-mc:98-175;
+    // cylindrical: phi is the z velocity, alpha = 1), from the star state V
+    double facM, facP, rc, Sgeo[F_NUM] = {0., 0., 0., 0., 0.};
+    if (geo_facs(P, m.id, i, dx, facM, facP, rc) && P.geom == 1) {
+      const double vd = ui[V_DENS * cs + c], vvx = ui[V_VELX * cs + c], vvz = ui[V_VELZ * cs + c];
+      Sgeo[F_XMOM] = (vd * vvz * vvz + 1.0 * ui[V_PRES * cs + c]) / rc;
+      Sgeo[F_ZMOM] = -(vd * vvz * vvx / rc);
+    }
 #pragma unroll
     for (int n = 0; n < NFM; n++) {
       if (n < F_NUM + ns) {
-        double inner = -(dt / dx * (flx[fpx + n * fs] - flx[fm + n * fs]));
+        double inner = -(dt / dx * (facP * flx[fpx + n * fs] - facM * flx[fm + n * fs]));
         if (P.ndim > 1) inner = inner - dt / dy * (fly[fpy + n * fs] - fly[fm + n * fs]);
         if (P.ndim == 3) inner = inner - dt / dz * (flz[fpz + n * fs] - flz[fm + n * fs]);
-        inner = inner + dt * (0.0 + Sg[n]);
+        inner = inner + dt * ((n < F_NUM ? Sgeo[n < F_NUM ? n : 0] : 0.0) + Sg[n]);
         B[n] = c1 * A[n] + c2 * B[n] + c3 * inner;
       }
     }
@@ -643,7 +651,13 @@ __global__ void __launch_bounds__(256) k_correct(DevParams P, const BlockMeta* _
         Us[F_ZMOM] = d0 * vz;
         Us[F_ENER] = d0 * ener;
         for (int v = 0; v < ns; v++) Us[F_NUM + v] = X[v] * d0;
-        const double fac = 1.0;
+        // geoFacs of the x faces (hy_rk_correctFluxes.F90:137-168, 341-358)
+        double fac = 1.0;
+        if (d == 0 && P.geom != 0) {
+          double facM, facP, rc;
+          geo_facs(P, m.id, L.lo[0] + c[0], m.dx, facM, facP, rc);
+          fac = side ? facP : facM;
+        }
         for (int q = 0; q < nf; q++) {
           const double b = fb[bo + q * bstride];
           const double dF = side ? b : -b;

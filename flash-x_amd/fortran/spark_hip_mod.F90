@@ -25,6 +25,7 @@ module spark_hip
   integer, parameter, public :: SPARK_OPT_GRAPH = 1, SPARK_OPT_AUTOTUNE = 2, SPARK_OPT_SAME_LEVEL_HALOS = 3
   integer, parameter, public :: SPARK_BC_OUTFLOW = 0, SPARK_BC_PERIODIC = 1, SPARK_BC_REFLECT = 2
   integer, parameter, public :: SPARK_EOS_GAMMA = 0, SPARK_EOS_MULTIGAMMA = 1
+  integer, parameter, public :: SPARK_GEOM_CARTESIAN = 0, SPARK_GEOM_CYLINDRICAL = 1
 
   !! struct spark_params (same member order and types)
   type, bind(C), public :: spark_params_t
@@ -58,7 +59,7 @@ module spark_hip
             spark_get_flux_registers, spark_set_flux_corrections, spark_flux_correct, &
             spark_set_option, spark_batched_tiles, spark_synchronize, spark_get_counters, &
             spark_last_error, spark_profile, spark_profile_read, spark_set_amr_grid, spark_flux_exchange, &
-            spark_set_dt_control
+            spark_set_dt_control, spark_set_geometry
   public :: spark_check, spark_set_abort_hook, spark_error_message
 
   interface
@@ -167,6 +168,14 @@ module spark_hip
        integer(c_int), value :: nsteps
        real(c_double), intent(out) :: dts(*)
      end function spark_advance
+
+     integer(c_int) function spark_set_geometry(batch, geometry, r_center, x_face_area, cell_volume) &
+          bind(C, name="spark_set_geometry")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: batch
+       integer(c_int), value :: geometry
+       type(c_ptr), intent(in) :: r_center(*), x_face_area(*), cell_volume(*)
+     end function spark_set_geometry
 
      integer(c_int) function spark_set_dt_control(batch, dt_init, change_factor, dt_min, dt_max) &
           bind(C, name="spark_set_dt_control")

@@ -22,6 +22,7 @@ SPARK_NPROP = 11
 SPARK_MAX_SCALARS = 8
 
 RECON = {"weno5": 0, "mp5": 1, "plm": 2, "fog": 3}
+GEOM = {"cartesian": 0, "cylindrical": 1}
 BC = {"outflow": 0, "periodic": 1, "reflect": 2}
 RIEMANN = {"hllc": 0, "hll": 1}
 FORM = {"plain_fp64": 0, "fp32_flux": 1, "batched_fp64": 2}
@@ -95,6 +96,7 @@ def lib():
             "spark_step_end": [vp],
             "spark_advance": [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
             "spark_set_dt_control": [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double],
+            "spark_set_geometry": [vp, ctypes.c_int, dpp, dpp, dpp],
             "spark_get_flux_registers": [vp, dpp, dpp, dpp],
             "spark_set_flux_corrections": [vp, dpp, dpp, dpp],
             "spark_flux_correct": [vp, ctypes.c_double],
@@ -126,6 +128,7 @@ def exported_symbols():
         "spark_get_flux_registers", "spark_set_flux_corrections", "spark_flux_correct", "spark_synchronize",
         "spark_get_counters", "spark_last_error", "spark_profile", "spark_profile_read", "spark_set_option",
         "spark_batched_tiles", "spark_set_amr_grid", "spark_flux_exchange", "spark_set_dt_control",
+        "spark_set_geometry",
     ]
 
 
@@ -165,6 +168,8 @@ def make_params(cfg: dict) -> SparkParams:
             continue
         if k == "eos":
             v = EOS[v] if isinstance(v, str) else v
+        if k == "geometry":
+            continue  # per-block arrays: Batch.set_geometry
         if not hasattr(p, k):
             raise KeyError(k)
         setattr(p, k, v)
@@ -254,6 +259,19 @@ class Batch:
         dts = (ctypes.c_double * max(1, nsteps))()
         check(lib().spark_advance(self.h, int(nsteps), dts), "spark_advance")
         return np.array(dts[:nsteps])
+
+    def set_geometry(self, geometry, r_center=None, x_face_area=None, cell_volume=None):
+        """geometry GEOM["cartesian"] (0) or GEOM["cylindrical"] (1); for the
+        latter per block 1D arrays over the guard-celled x index."""
+        g = GEOM[geometry] if isinstance(geometry, str) else int(geometry)
+        if g == 0:
+            check(lib().spark_set_geometry(self.h, 0, None, None, None), "spark_set_geometry")
+            return
+        keep = [[np.ascontiguousarray(a, dtype=np.float64) for a in arrs]
+                for arrs in (r_center, x_face_area, cell_volume)]
+        ptrs = [_ptrs(k) for k in keep]
+        check(lib().spark_set_geometry(self.h, g, *[ctypes.cast(p_, ctypes.POINTER(ctypes.c_void_p)) for p_ in ptrs]),
+              "spark_set_geometry")
 
     def set_dt_control(self, dt_init=0.0, change_factor=0.0, dt_min=0.0, dt_max=0.0):
         """The driver's dt limits for advance() (all 0: the CFL dt)."""

@@ -213,6 +213,54 @@ def species_tube3d(nb=2, n=16, nguard=10, species_a=(1.0, 4.0), species_gamma=(1
     return Problem("species_tube3d", cfg, blocks, arrays, ug=(nb, nb, nb))
 
 
+def cylindrical_rows(n, nguard, bi, nb, dx, dz, r0):
+    """Geometry rows of block bi of nb along r (2D cylindrical, r from r0):
+    cell-centre radius, low x-face area and cell volume over the
+    guard-celled x index, as the Grid computes them: coordinates
+    r0 + (K - shift) dx with K the 1-based global cell index plus one and
+    shift 1.5 (centre) or 2 (left face) (Grid_getCellCoords.F90:173-200),
+    area 2 pi |r_left| dz (Grid_getCellFaceAreas.F90, CYLINDRICAL, 2D),
+    volume 2 pi |r_centre| dr dz (Grid_getCellVolumes.F90, CYLINDRICAL, 2D)."""
+    NX = n + 2 * nguard
+    K = (bi * n + 1 + (np.arange(NX) - nguard) + 1).astype(np.float64)
+    rc = r0 + (K - 1.5) * dx
+    rl = r0 + (K - 2.0) * dx
+    area = 2.0 * np.pi * np.abs(rl) * dz
+    vol = 2.0 * np.pi * np.abs(rc) * dx * dz
+    return np.ascontiguousarray(np.stack([rc, area, vol]))
+
+
+def cylindrical_blast(nbx=4, nby=4, n=16, nguard=10, r0=1.0, p_in=10.0, radius=0.15, swirl=0.1, vr=0.0,
+                      gamma=1.4, **kw) -> Problem:
+    """2D cylindrical (r, z) blast: r in [r0, r0 + 1], z in [0, nby/nbx],
+    rho = 1, p = 1 with p = p_in inside a circle of the given radius at the
+    domain centre (in the r-z plane), radial velocity vr, azimuthal velocity
+    swirl * r (the z velocity component).  info["geo"]: per block the
+    geometry rows (cylindrical_rows) for spark_set_geometry and the
+    oracle."""
+    cfg = base_cfg(ndim=2, nguard=nguard, gamma=gamma, geometry="cylindrical", **kw)
+    dx = 1.0 / (nbx * n)
+    blocks, arrays, geo = [], [], []
+    nunk = SPARK_NPROP + cfg["nscalars"]
+    rmid, zmid = r0 + 0.5, 0.5 * nby * n * dx
+    for bj in range(nby):
+        for bi in range(nbx):
+            b = dict(nx=n, ny=n, nz=1, level=1, dx=dx, dy=dx, dz=1.0)
+            NZ, NY, NX = gc_shape(cfg, b)
+            Z, Y, X = _centers(cfg, b, (r0 + bi * n * dx, bj * n * dx, 0.0), NZ, NY, NX)
+            inside = (X - rmid) ** 2 + (Y - zmid) ** 2 < radius * radius
+            pres = np.where(inside, p_in, 1.0)
+            dens = np.ones_like(pres)
+            U = np.zeros((NZ, NY, NX, nunk))
+            fill_thermo(U, dens, pres, np.full_like(pres, vr), np.zeros_like(pres), swirl * X, gamma)
+            g = cylindrical_rows(n, nguard, bi, nbx, dx, dx, r0)
+            b = dict(b, geo=g)
+            blocks.append(b)
+            arrays.append(U)
+            geo.append(g)
+    return Problem("cylindrical_blast", cfg, blocks, arrays, ug=(nbx, nby, 1), info=dict(geo=geo, r0=r0))
+
+
 def sedov3d(nb=8, n=16, nguard=6, gamma=1.4, E=1.0, r_cells=3.5, p_amb=1e-5, rho=1.0, center=(0.5, 0.5, 0.5),
             **kw) -> Problem:
     """3D Sedov blast on [0,1]^3: E deposited as pressure in the cells whose

@@ -34,7 +34,8 @@ class OrcParams(ctypes.Structure):
 
 class OrcBlock(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
-                ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double)]
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+                ("geo", ctypes.c_void_p)]
 
 
 _lib = None
@@ -80,6 +81,8 @@ def lib():
         L.orc_batch_advance.restype = ctypes.c_int
         L.orc_set_dt_control.argtypes = [ctypes.c_double] * 4
         L.orc_set_dt_control.restype = None
+        L.orc_set_geometry.argtypes = [pp, ctypes.c_int]
+        L.orc_set_geometry.restype = None
         _lib = L
     return _lib
 
@@ -113,7 +116,13 @@ def params(cfg: dict) -> OrcParams:
 
 
 def block(b: dict) -> OrcBlock:
-    return OrcBlock(int(b["nx"]), int(b["ny"]), int(b["nz"]), float(b["dx"]), float(b["dy"]), float(b["dz"]))
+    """b["geo"] (optional): (3, NX) float64 rows -- cell-centre radius, low
+    x-face area, cell volume -- for cylindrical geometry."""
+    geo = b.get("geo")
+    if geo is not None:
+        assert geo.dtype == np.float64 and geo.flags["C_CONTIGUOUS"]
+    return OrcBlock(int(b["nx"]), int(b["ny"]), int(b["nz"]), float(b["dx"]), float(b["dy"]), float(b["dz"]),
+                    geo.ctypes.data if geo is not None else None)
 
 
 def dptr(a: np.ndarray):
@@ -144,11 +153,19 @@ def compute_dt(cfg, b, U):
     return dt, list(loc)
 
 
-def ug_advance(cfg, b, ug, arrays, nsteps, refill=False, nthreads=0):
+def ug_advance(cfg, b, ug, arrays, nsteps, refill=False, nthreads=0, geo=None):
+    """geo (optional): per block (3, NX) geometry rows (see block())."""
     P, B = params(cfg), block(b)
     dts = np.zeros(max(1, nsteps))
-    nerr = lib().orc_ug_advance(ctypes.byref(P), ctypes.byref(B), ug[0], ug[1], ug[2], ptr_array(arrays), nsteps,
-                                int(refill), nthreads, dptr(dts))
+    gp = ptr_array(geo) if geo is not None else None  # kept alive: the oracle holds the pointers
+    if gp is not None:
+        lib().orc_set_geometry(gp, len(geo))
+    try:
+        nerr = lib().orc_ug_advance(ctypes.byref(P), ctypes.byref(B), ug[0], ug[1], ug[2], ptr_array(arrays),
+                                    nsteps, int(refill), nthreads, dptr(dts))
+    finally:
+        if gp is not None:
+            lib().orc_set_geometry(None, 0)
     return dts[:nsteps], nerr
 
 

@@ -556,6 +556,26 @@ void orc_eos_wrapped(const orc_params* P, const orc_block* b, double* S, const i
 /* ------------------------------------------------------------------ */
 /* update_solution, Spark/hy_rk_updateSoln.F90-mc:194-304 (Cartesian),
  * gravSources :179-190 with a constant acceleration. */
+/* geoFacs (hy_rk_updateSoln.F90-mc:98-175, hy_rk_correctFluxes.F90:341-358)
+ * for cylindrical geometry: facM, facP = farea * dx / cvol of the x faces
+ * of cell i; 0 and no sources where the cell centre is at negative radius.
+ * Returns 1 where the sources apply. */
+static int geo_facs(const orc_block* b, int NX, int i, double dx, double* facM, double* facP, double* rc) {
+  *facM = 1.0;
+  *facP = 1.0;
+  *rc = 0.0;
+  if (!b->geo) return 0;
+  *rc = b->geo[i];
+  *facM = b->geo[NX + i] * dx / b->geo[2 * NX + i];
+  *facP = b->geo[NX + i + 1] * dx / b->geo[2 * NX + i];
+  if (*rc < 0.0) {
+    *facM = 0.;
+    *facP = 0.;
+    return 0;
+  }
+  return 1;
+}
+
 void orc_update_soln(const orc_params* P, const orc_block* b, const double* U0v, double* S,
                      const int lim[3][2], double dt, const double coeffs[3], const double* flx,
                      const double* fly, const double* flz) {
@@ -599,9 +619,18 @@ void orc_update_soln(const orc_params* P, const orc_block* b, const double* U0v,
           Sg[HY_ZMOM] = Us[HY_MASS] * P->grav[2];
           Sg[HY_ENER] = Us[HY_XMOM] * P->grav[0] + Us[HY_YMOM] * P->grav[1] + Us[HY_ZMOM] * P->grav[2];
         }
-        for (int n = 0; n < nf; n++) St[n] = 0.0 + Sg[n];
+        /* Stot = Sgeo + Sgrav; cylindrical: alpha = 1, the azimuthal
+         * momentum is the z one, V the star state (geoFacs :131-157) */
+        double Sgeo[MAXFLX], facM, facP, rc;
+        for (int n = 0; n < nf; n++) Sgeo[n] = 0.;
+        if (geo_facs(b, NX, i, dx, &facM, &facP, &rc)) {
+          Sgeo[HY_XMOM] = (Vs[ORC_DENS] * Vs[ORC_VELZ] * Vs[ORC_VELZ] + 1.0 * Vs[ORC_PRES]) / rc;
+          Sgeo[HY_ZMOM] = Vs[ORC_DENS] * Vs[ORC_VELZ] * Vs[ORC_VELX] / rc;
+          Sgeo[HY_ZMOM] = -Sgeo[HY_ZMOM];
+        }
+        for (int n = 0; n < nf; n++) St[n] = Sgeo[n] + Sg[n];
         for (int n = 0; n < nf; n++) {
-          double inner = -(dt / dx * (Fp[n] - Fm[n]));
+          double inner = -(dt / dx * (facP * Fp[n] - facM * Fm[n]));
           if (P->ndim > 1) inner = inner - dt / dy * (Gp[n] - Gm[n]);
           if (P->ndim == 3) inner = inner - dt / dz * (Hp[n] - Hm[n]);
           inner = inner + dt * St[n];
@@ -811,10 +840,9 @@ int orc_hydro_block(const orc_params* P, const orc_block* b, double* U, double d
 
 /* ------------------------------------------------------------------ */
 /* correctZone + per-cell EOS, Spark/hy_rk_correctFluxes.F90:126-338 */
-static void correct_zone(const orc_params* P, double* V, const double* dFlux, double dt, double dx) {
+static void correct_zone(const orc_params* P, double* V, const double* dFlux, double dt, double dx, double fac) {
   const int nf = orc_nflux(P), ns = P->nscalars;
   double Us[MAXFLX];
-  double fac = 1.0;
   Us[HY_MASS] = V[ORC_DENS];
   Us[HY_XMOM] = V[ORC_DENS] * V[ORC_VELX];
   Us[HY_YMOM] = V[ORC_DENS] * V[ORC_VELY];
@@ -861,7 +889,14 @@ void orc_correct_fluxes(const orc_params* P, const orc_block* b, double* U, doub
           const double* buf = fb[dir] + (size_t)nf * ((size_t)f[0] + (size_t)e[0] * ((size_t)f[1] + (size_t)e[1] * f[2]));
           for (int v = 0; v < nf; v++) dF[v] = side ? buf[v] : -buf[v];
           double* V = U + (size_t)nunk * cidx(NX, NY, lim[0][0] + c[0], lim[1][0] + c[1], lim[2][0] + c[2]);
-          correct_zone(P, V, dF, dt, del[dir]);
+          /* fac = facM on the low x face, facP on the high one (:137-168) */
+          double fac = 1.0;
+          if (dir == 0 && b->geo) {
+            double facM, facP, rc;
+            geo_facs(b, NX, lim[0][0] + c[0], b->dx, &facM, &facP, &rc);
+            fac = side ? facP : facM;
+          }
+          correct_zone(P, V, dF, dt, del[dir], fac);
         }
       }
   }
@@ -926,6 +961,21 @@ double orc_ug_compute_dt(const orc_params* P, const orc_block* b, int nblocks, d
  * and min with dt_max (each when > 0).  All zero: the CFL dt. */
 static double g_dt_first = 0.0, g_dt_prev = DBL_MAX, g_dt_factor = 0.0, g_dt_min = 0.0, g_dt_max = 0.0;
 
+static const double* const* g_geo = NULL;
+static int g_geo_n = 0;
+
+void orc_set_geometry(const double* const* geo, int nblocks) {
+  g_geo = geo;
+  g_geo_n = geo ? nblocks : 0;
+}
+
+/* block i's descriptor with its geometry rows */
+static orc_block with_geo(const orc_block* b, int i) {
+  orc_block c = *b;
+  if (g_geo && i < g_geo_n) c.geo = g_geo[i];
+  return c;
+}
+
 void orc_set_dt_control(double dt_init, double change_factor, double dt_min, double dt_max) {
   g_dt_first = dt_init;
   g_dt_prev = dt_init > 0.0 ? dt_init : DBL_MAX;
@@ -979,7 +1029,10 @@ int orc_ug_advance(const orc_params* P, const orc_block* b, int nbx, int nby, in
     orc_ug_fill_guards(P, b, nbx, nby, nbz, blocks, nunk);
     if (!refill) {
 #pragma omp parallel for schedule(dynamic) reduction(+ : nerr)
-      for (int i = 0; i < nblk; i++) nerr += orc_hydro_block(P, b, blocks[i], dt, NULL, NULL, NULL);
+      for (int i = 0; i < nblk; i++) {
+        const orc_block bi = with_geo(b, i);
+        nerr += orc_hydro_block(P, &bi, blocks[i], dt, NULL, NULL, NULL);
+      }
     } else {
 #pragma omp parallel for schedule(static)
       for (int i = 0; i < nblk; i++) {
@@ -991,7 +1044,8 @@ int orc_ug_advance(const orc_params* P, const orc_block* b, int nbx, int nby, in
 #pragma omp parallel for schedule(dynamic) reduction(+ : nerr)
         for (int i = 0; i < nblk; i++) {
           int e = 0;
-          orc_stage(P, b, U0[i], S[i], stage, 0, dt, NULL, NULL, NULL, &e);
+          const orc_block bi = with_geo(b, i);
+          orc_stage(P, &bi, U0[i], S[i], stage, 0, dt, NULL, NULL, NULL, &e);
           nerr += e;
         }
         if (stage < last) orc_ug_fill_guards(P, b, nbx, nby, nbz, S, nunk);

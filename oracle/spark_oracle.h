@@ -51,10 +51,15 @@ typedef struct orc_params {
   double species_gamma[8];
 } orc_params;
 
-/* One block: interior sizes (1 for unused dims) and its deltas. */
+/* One block: interior sizes (1 for unused dims) and its deltas.  geo:
+ * NULL in Cartesian geometry; in cylindrical geometry (x = radius) three
+ * rows of NX values over the guard-celled x index -- cell-centre radius,
+ * area of the cell's low x face, cell volume (the Grid arrays hy_xCenter,
+ * hy_farea, hy_cvol of Spark/Hydro.ini:40-56). */
 typedef struct orc_block {
   int nx, ny, nz;
   double dx, dy, dz;
+  const double* geo;
 } orc_block;
 
 /* helpers */
@@ -110,6 +115,9 @@ void orc_correct_fluxes(const orc_params* P, const orc_block* b, double* U, doub
  * multi-step driver in either parity (telescoping) or refill mode. */
 void orc_ug_fill_guards(const orc_params* P, const orc_block* b, int nbx, int nby, int nbz,
                         double* const* blocks, int nvar);
+/* Per-block geometry rows (orc_block.geo) for the advance loops, which
+ * share one block descriptor: geo[i] for block i, or NULL (Cartesian). */
+void orc_set_geometry(const double* const* geo, int nblocks);
 /* The driver's dt limits for the advance loops (Driver_computeDt.F90:450-460
  * and dr_dtInit); all zero: the CFL dt (the default). */
 void orc_set_dt_control(double dt_init, double change_factor, double dt_min, double dt_max);

@@ -325,3 +325,66 @@ def test_driver_dt_limits_follow_driver_computeDt():
     d = adv(3, change_factor=1.5)
     assert d[0] == cfl[0] and np.all(d[1:] <= 1.5 * d[:-1])
     assert np.array_equal(adv(3), cfl)
+
+
+def _cyl(**kw):
+    return problems.cylindrical_blast(nbx=2, nby=2, n=16, nguard=10, **kw)
+
+
+def test_cylindrical_rows_are_the_grid_geometry():
+    """Face areas and volumes of the generator against the cylindrical
+    closed forms: facM = area_low dr / vol = r_low / r_centre, facP =
+    r_high / r_centre, centres midway between faces."""
+    prob = _cyl()
+    dx = prob.blocks[0]["dx"]
+    for g in prob.info["geo"]:
+        rc, area, vol = g
+        facm, facp = area[:-1] * dx / vol[:-1], area[1:] * dx / vol[:-1]
+        rl = rc - 0.5 * dx
+        assert np.allclose(facm, rl[:-1] / rc[:-1], rtol=1e-14, atol=0)
+        assert np.allclose(facp, (rl[:-1] + dx) / rc[:-1], rtol=1e-14, atol=0)
+        assert np.allclose(np.diff(rc), dx, rtol=1e-12, atol=0)
+
+
+def test_cylindrical_rest_state_is_a_fixed_point():
+    """At rest with uniform p, the pressure flux difference of the radial
+    momentum, -(r_high - r_low) p / (r_centre dr), cancels the geometric
+    source p / r_centre: the state stays at rest to rounding.  Without the
+    source (or with facM/facP swapped) the radial velocity would reach
+    ~dt p / (rho r) ~ 1e-3 in a step."""
+    prob = _cyl(p_in=1.0, swirl=0.0)
+    arrs = [a.copy() for a in prob.arrays]
+    d, nerr = pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, arrs, 3, geo=prob.info["geo"])
+    assert nerr == 0 and d[0] > 1e-3
+    g = prob.cfg["nguard"]
+    vmax = max(float(np.abs(a[0, g:-g, g:-g, VI["velx"]]).max()) for a in arrs)
+    dmax = max(float(np.abs(a[0, g:-g, g:-g, VI["dens"]] - 1.0).max()) for a in arrs)
+    assert vmax < 1e-13 and dmax < 1e-13, (vmax, dmax)
+
+
+def test_cylindrical_uniform_flow_follows_the_axisymmetric_equations():
+    """Uniform rho, p, radial velocity u and azimuthal velocity w: over a
+    short step the conservative update in cylindrical geometry gives
+    d(rho)/dt = -rho u / r, d(rho u)/dt = rho (w^2 - u^2) / r and
+    d(rho w)/dt = -2 rho u w / r (the flux divergence (1/r) d(r F)/dr plus
+    the sources (rho w^2 + p)/r and -rho u w / r).  A dropped or mis-signed
+    source changes the rates by O(1)."""
+    u, w = 0.1, 0.2
+    prob = _cyl(p_in=1.0, swirl=0.0, vr=u)
+    for a in prob.arrays:
+        a[..., VI["velz"]] = w
+        a[..., VI["ener"]] = a[..., VI["eint"]] + 0.5 * (u * u + w * w)
+    arrs = [a.copy() for a in prob.arrays]
+    dt = 1e-5
+    with pyoracle.dt_control(dt_init=dt):
+        pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, arrs, 1, geo=prob.info["geo"])
+    g = prob.cfg["nguard"]
+    for a, geo in zip(arrs, prob.info["geo"]):
+        rc = geo[0][g + 2:-g - 2][None, :]
+        c = a[0, g + 2:-g - 2, g + 2:-g - 2]
+        rho = c[..., VI["dens"]]
+        mu = rho * c[..., VI["velx"]]
+        mw = rho * c[..., VI["velz"]]
+        assert np.allclose((rho - 1.0) / dt, -u / rc, rtol=1e-3)
+        assert np.allclose((mu - u) / dt, (w * w - u * u) / rc, rtol=1e-3)
+        assert np.allclose((mw - w) / dt, -2.0 * u * w / rc, rtol=1e-3)
