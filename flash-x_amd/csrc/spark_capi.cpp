@@ -1261,7 +1261,7 @@ int spark_advance(spark_batch* b, int nsteps, double* dts) {
 }
 
 int spark_set_geometry(spark_batch* b, int geometry, const double* const* r_center, const double* const* x_face_area,
-                       const double* const* cell_volume) {
+                       const double* const* cell_volume, const double* const* r_left, const double* const* r_right) {
   if (!b) return fail(SPARK_ERR_ARG, "spark_set_geometry: null batch");
   if (geometry == SPARK_GEOM_CARTESIAN) {
     b->dp.geom = 0;
@@ -1269,27 +1269,34 @@ int spark_set_geometry(spark_batch* b, int geometry, const double* const* r_cent
     drop_graph(b);
     return SPARK_OK;
   }
-  if (geometry != SPARK_GEOM_CYLINDRICAL) return fail(SPARK_ERR_ARG, "spark_set_geometry: geometry %d", geometry);
+  if (geometry != SPARK_GEOM_CYLINDRICAL && geometry != SPARK_GEOM_SPHERICAL)
+    return fail(SPARK_ERR_ARG, "spark_set_geometry: geometry %d", geometry);
   if (b->fused) return fail(SPARK_ERR_ARG, "spark_set_geometry: the batched forms are Cartesian only");
-  // Grid_getCellVolumes supports cylindrical geometry in 2D only (:49-55)
-  if (b->p.ndim != 2) return fail(SPARK_ERR_ARG, "spark_set_geometry: cylindrical geometry is 2D (r, z) only");
+  // the Grid's cell volumes exist for cylindrical geometry in 2D and
+  // spherical in 1D and 2D (Grid_getCellVolumes.F90:49-55); spherical 2D
+  // areas and volumes depend on theta, which these radial rows do not carry
+  if (geometry == SPARK_GEOM_CYLINDRICAL && b->p.ndim != 2)
+    return fail(SPARK_ERR_ARG, "spark_set_geometry: cylindrical geometry is 2D (r, z) only");
+  if (geometry == SPARK_GEOM_SPHERICAL && b->p.ndim != 1)
+    return fail(SPARK_ERR_ARG, "spark_set_geometry: spherical geometry is 1D (r) only");
   if (b->classes.size() != 1) return fail(SPARK_ERR_ARG, "spark_set_geometry: blocks of one shape only");
-  if (!r_center || !x_face_area || !cell_volume) return fail(SPARK_ERR_ARG, "spark_set_geometry: null argument");
+  if (!r_center || !x_face_area || !cell_volume || !r_left || !r_right)
+    return fail(SPARK_ERR_ARG, "spark_set_geometry: null argument");
   const int NX = b->meta_h[0].NX;
-  std::vector<double> h((size_t)b->nblk * 3 * NX);
+  const int R = spark::GEO_ROWS;
+  std::vector<double> h((size_t)b->nblk * R * NX);
   for (int i = 0; i < b->nblk; i++) {
-    if (!r_center[i] || !x_face_area[i] || !cell_volume[i])
-      return fail(SPARK_ERR_ARG, "spark_set_geometry: block %d array is null", i);
+    const double* rows[5] = {r_center[i], x_face_area[i], cell_volume[i], r_left[i], r_right[i]};
+    for (int r = 0; r < R; r++)
+      if (!rows[r]) return fail(SPARK_ERR_ARG, "spark_set_geometry: block %d array %d is null", i, r);
     for (int q = 0; q < NX; q++) {
       if (!(cell_volume[i][q] != 0.0)) return fail(SPARK_ERR_ARG, "block %d: cell volume %d is zero", i, q);
-      h[((size_t)i * 3 + 0) * NX + q] = r_center[i][q];
-      h[((size_t)i * 3 + 1) * NX + q] = x_face_area[i][q];
-      h[((size_t)i * 3 + 2) * NX + q] = cell_volume[i][q];
+      for (int r = 0; r < R; r++) h[((size_t)i * R + r) * NX + q] = rows[r][q];
     }
   }
   if (!b->geo_d) HIPCHK(hipMalloc((void**)&b->geo_d, sizeof(double) * h.size()));
   HIPCHK(hipMemcpy(b->geo_d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
-  b->dp.geom = 1;
+  b->dp.geom = geometry == SPARK_GEOM_CYLINDRICAL ? 1 : 2;
   b->dp.geo_nx = NX;
   b->dp.geo = b->geo_d;
   drop_graph(b);  // the kernels take the parameters by value

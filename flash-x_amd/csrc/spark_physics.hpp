@@ -329,13 +329,35 @@ struct DevParams {
   double gasConst;
   double mg_gm1[8];     // 1/(gamma_s - 1) of each species
   double mg_ainv[8];    // 1/A_s
-  // geometry: 0 Cartesian, 1 cylindrical (r, z[, phi]).  geo holds per block
-  // (BlockMeta::id) three rows of geo_nx values over the guard-celled x
-  // index: cell-centre radius, area of the cell's low x face, cell volume
-  // (the Grid's hy_xCenter, hy_farea, hy_cvol; Hydro.ini:40-56)
+  // geometry: 0 Cartesian, 1 cylindrical (r, z), 2 spherical (r).  geo holds
+  // per block (BlockMeta::id) GEO_ROWS rows of geo_nx values over the
+  // guard-celled x index: cell-centre radius, area of the cell's low x face,
+  // cell volume, left and right edge radius (the Grid's hy_xCenter,
+  // hy_farea, hy_cvol, hy_xLeft, hy_xRight; Hydro.ini:40-56)
   int geom, geo_nx;
   const double* geo;
 };
+constexpr int GEO_ROWS = 5;
+
+// geometric sources of the update (geoFacs, hy_rk_updateSoln.F90-mc:131-
+// 170) from the star state V (vd, vx, vz, p) and the star conserved theta
+// momentum uy: cylindrical alpha = 1, spherical alpha = 2 plus (rho v)^2 /
+// rho / r and the whole source scaled by dx / dx_sph,
+// dx_sph = (r_R^3 - r_L^3) / (3 r_c^2); the azimuthal momentum is the z one
+__device__ __forceinline__ void geo_sources(const DevParams& P, int id, int i, double dx, double rc, double vd,
+                                            double vx, double vz, double pres, double uy, double* Sgeo) {
+  const double alpha = P.geom == 2 ? 2. : 1.;
+  Sgeo[F_XMOM] = (vd * vz * vz + alpha * pres) / rc;
+  Sgeo[F_ZMOM] = -(vd * vz * vx / rc);
+  if (P.geom == 2) {
+    const double* g = P.geo + (int64_t)id * GEO_ROWS * P.geo_nx;
+    const double rl = g[3 * P.geo_nx + i], rr = g[4 * P.geo_nx + i];
+    const double dx_sph = (rr * rr * rr - rl * rl * rl) / (3. * (rc * rc));
+    Sgeo[F_XMOM] = Sgeo[F_XMOM] + uy * uy / vd / rc;
+#pragma unroll
+    for (int n = 0; n < F_NUM; n++) Sgeo[n] = Sgeo[n] * dx / dx_sph;
+  }
+}
 
 // geoFacs for the x faces of cell i (hy_rk_updateSoln.F90-mc:98-175,
 // hy_rk_correctFluxes.F90:341-358): facM, facP = farea * dx / cvol of the
@@ -348,7 +370,7 @@ __device__ __forceinline__ bool geo_facs(const DevParams& P, int id, int i, doub
   facP = 1.0;
   rc = 0.0;
   if (P.geom == 0) return true;
-  const double* g = P.geo + (int64_t)id * 3 * P.geo_nx;
+  const double* g = P.geo + (int64_t)id * GEO_ROWS * P.geo_nx;
   rc = g[i];
   facM = g[P.geo_nx + i] * dx / g[2 * P.geo_nx + i];
   facP = g[P.geo_nx + i + 1] * dx / g[2 * P.geo_nx + i];

@@ -333,14 +333,11 @@ __global__ void __launch_bounds__(256) k_update(DevParams P, const BlockMeta* __
     const int64_t fpx = m.fbase + fix(m, i + 1, j, k);
     const int64_t fpy = m.fbase + fix(m, i, j + 1, k);
     const int64_t fpz = m.fbase + fix(m, i, j, k + 1);
-    // geometric factors and sources (geoFacs, hy_rk_updateSoln.F90-mc:98-175;
-    // cylindrical: phi is the z velocity, alpha = 1), from the star state V
+    // geometric factors and sources (geoFacs, hy_rk_updateSoln.F90-mc:98-175)
     double facM, facP, rc, Sgeo[F_NUM] = {0., 0., 0., 0., 0.};
-    if (geo_facs(P, m.id, i, dx, facM, facP, rc) && P.geom == 1) {
-      const double vd = ui[V_DENS * cs + c], vvx = ui[V_VELX * cs + c], vvz = ui[V_VELZ * cs + c];
-      Sgeo[F_XMOM] = (vd * vvz * vvz + 1.0 * ui[V_PRES * cs + c]) / rc;
-      Sgeo[F_ZMOM] = -(vd * vvz * vvx / rc);
-    }
+    if (geo_facs(P, m.id, i, dx, facM, facP, rc) && P.geom != 0)
+      geo_sources(P, m.id, i, dx, rc, ui[V_DENS * cs + c], ui[V_VELX * cs + c], ui[V_VELZ * cs + c],
+                  ui[V_PRES * cs + c], B[F_YMOM], Sgeo);
 #pragma unroll
     for (int n = 0; n < NFM; n++) {
       if (n < F_NUM + ns) {

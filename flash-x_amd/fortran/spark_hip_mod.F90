@@ -25,7 +25,7 @@ module spark_hip
   integer, parameter, public :: SPARK_OPT_GRAPH = 1, SPARK_OPT_AUTOTUNE = 2, SPARK_OPT_SAME_LEVEL_HALOS = 3
   integer, parameter, public :: SPARK_BC_OUTFLOW = 0, SPARK_BC_PERIODIC = 1, SPARK_BC_REFLECT = 2
   integer, parameter, public :: SPARK_EOS_GAMMA = 0, SPARK_EOS_MULTIGAMMA = 1
-  integer, parameter, public :: SPARK_GEOM_CARTESIAN = 0, SPARK_GEOM_CYLINDRICAL = 1
+  integer, parameter, public :: SPARK_GEOM_CARTESIAN = 0, SPARK_GEOM_CYLINDRICAL = 1, SPARK_GEOM_SPHERICAL = 2
 
   !! struct spark_params (same member order and types)
   type, bind(C), public :: spark_params_t
@@ -169,12 +169,12 @@ module spark_hip
        real(c_double), intent(out) :: dts(*)
      end function spark_advance
 
-     integer(c_int) function spark_set_geometry(batch, geometry, r_center, x_face_area, cell_volume) &
-          bind(C, name="spark_set_geometry")
+     integer(c_int) function spark_set_geometry(batch, geometry, r_center, x_face_area, cell_volume, r_left, &
+          r_right) bind(C, name="spark_set_geometry")
        import :: c_int, c_ptr
        type(c_ptr), value :: batch
        integer(c_int), value :: geometry
-       type(c_ptr), intent(in) :: r_center(*), x_face_area(*), cell_volume(*)
+       type(c_ptr), intent(in) :: r_center(*), x_face_area(*), cell_volume(*), r_left(*), r_right(*)
      end function spark_set_geometry
 
      integer(c_int) function spark_set_dt_control(batch, dt_init, change_factor, dt_min, dt_max) &

@@ -22,7 +22,7 @@ SPARK_NPROP = 11
 SPARK_MAX_SCALARS = 8
 
 RECON = {"weno5": 0, "mp5": 1, "plm": 2, "fog": 3}
-GEOM = {"cartesian": 0, "cylindrical": 1}
+GEOM = {"cartesian": 0, "cylindrical": 1, "spherical": 2}
 BC = {"outflow": 0, "periodic": 1, "reflect": 2}
 RIEMANN = {"hllc": 0, "hll": 1}
 FORM = {"plain_fp64": 0, "fp32_flux": 1, "batched_fp64": 2}
@@ -96,7 +96,7 @@ def lib():
             "spark_step_end": [vp],
             "spark_advance": [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
             "spark_set_dt_control": [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double],
-            "spark_set_geometry": [vp, ctypes.c_int, dpp, dpp, dpp],
+            "spark_set_geometry": [vp, ctypes.c_int, dpp, dpp, dpp, dpp, dpp],
             "spark_get_flux_registers": [vp, dpp, dpp, dpp],
             "spark_set_flux_corrections": [vp, dpp, dpp, dpp],
             "spark_flux_correct": [vp, ctypes.c_double],
@@ -260,18 +260,18 @@ class Batch:
         check(lib().spark_advance(self.h, int(nsteps), dts), "spark_advance")
         return np.array(dts[:nsteps])
 
-    def set_geometry(self, geometry, r_center=None, x_face_area=None, cell_volume=None):
-        """geometry GEOM["cartesian"] (0) or GEOM["cylindrical"] (1); for the
-        latter per block 1D arrays over the guard-celled x index."""
+    def set_geometry(self, geometry, rows=None):
+        """geometry GEOM["cartesian"] (0), ["cylindrical"] (1) or
+        ["spherical"] (2); rows: per block a (5, NX) array over the
+        guard-celled x index -- cell-centre radius, low x-face area, cell
+        volume, left and right edge radius."""
         g = GEOM[geometry] if isinstance(geometry, str) else int(geometry)
         if g == 0:
-            check(lib().spark_set_geometry(self.h, 0, None, None, None), "spark_set_geometry")
+            check(lib().spark_set_geometry(self.h, 0, None, None, None, None, None), "spark_set_geometry")
             return
-        keep = [[np.ascontiguousarray(a, dtype=np.float64) for a in arrs]
-                for arrs in (r_center, x_face_area, cell_volume)]
+        keep = [[np.ascontiguousarray(r[q], dtype=np.float64) for r in rows] for q in range(5)]
         ptrs = [_ptrs(k) for k in keep]
-        check(lib().spark_set_geometry(self.h, g, *[ctypes.cast(p_, ctypes.POINTER(ctypes.c_void_p)) for p_ in ptrs]),
-              "spark_set_geometry")
+        check(lib().spark_set_geometry(self.h, g, *ptrs), "spark_set_geometry")
 
     def set_dt_control(self, dt_init=0.0, change_factor=0.0, dt_min=0.0, dt_max=0.0):
         """The driver's dt limits for advance() (all 0: the CFL dt)."""

@@ -213,21 +213,62 @@ def species_tube3d(nb=2, n=16, nguard=10, species_a=(1.0, 4.0), species_gamma=(1
     return Problem("species_tube3d", cfg, blocks, arrays, ug=(nb, nb, nb))
 
 
-def cylindrical_rows(n, nguard, bi, nb, dx, dz, r0):
-    """Geometry rows of block bi of nb along r (2D cylindrical, r from r0):
-    cell-centre radius, low x-face area and cell volume over the
-    guard-celled x index, as the Grid computes them: coordinates
+def _radii(n, nguard, bi, dx, r0):
+    """Cell-centre, left and right edge radius over the guard-celled x index
+    of block bi, as Grid_getCellCoords.F90:173-200 computes them:
     r0 + (K - shift) dx with K the 1-based global cell index plus one and
-    shift 1.5 (centre) or 2 (left face) (Grid_getCellCoords.F90:173-200),
-    area 2 pi |r_left| dz (Grid_getCellFaceAreas.F90, CYLINDRICAL, 2D),
-    volume 2 pi |r_centre| dr dz (Grid_getCellVolumes.F90, CYLINDRICAL, 2D)."""
+    shift 1.5 (centre), 2 (left edge, faces) or 1 (right edge)."""
     NX = n + 2 * nguard
     K = (bi * n + 1 + (np.arange(NX) - nguard) + 1).astype(np.float64)
-    rc = r0 + (K - 1.5) * dx
-    rl = r0 + (K - 2.0) * dx
+    return r0 + (K - 1.5) * dx, r0 + (K - 2.0) * dx, r0 + (K - 1.0) * dx
+
+
+def cylindrical_rows(n, nguard, bi, nb, dx, dz, r0):
+    """Geometry rows of block bi along r (2D cylindrical, r from r0): cell-
+    centre radius, low x-face area, cell volume, left and right edge radius
+    over the guard-celled x index, as the Grid computes them: area
+    2 pi |r_left| dz (Grid_getCellFaceAreas.F90, CYLINDRICAL, 2D), volume
+    2 pi |r_centre| dr dz (Grid_getCellVolumes.F90, CYLINDRICAL, 2D)."""
+    rc, rl, rr = _radii(n, nguard, bi, dx, r0)
     area = 2.0 * np.pi * np.abs(rl) * dz
     vol = 2.0 * np.pi * np.abs(rc) * dx * dz
-    return np.ascontiguousarray(np.stack([rc, area, vol]))
+    return np.ascontiguousarray(np.stack([rc, area, vol, rl, rr]))
+
+
+def spherical_rows(n, nguard, bi, dx, r0):
+    """Geometry rows of block bi along r (1D spherical): as cylindrical_rows
+    with area r_left^2 4 pi (Grid_getCellFaceAreas.F90, SPHERICAL, 1D) and
+    volume dr (r_l^2 + r_l r_r + r_r^2) 4 pi / 3 (Grid_getCellVolumes.F90,
+    SPHERICAL, 1D)."""
+    rc, rl, rr = _radii(n, nguard, bi, dx, r0)
+    area = rl * rl * 4.0 * np.pi
+    vol = dx * (rl * rl + rl * rr + rr * rr) * 4.0 * np.pi / 3.0
+    return np.ascontiguousarray(np.stack([rc, area, vol, rl, rr]))
+
+
+def spherical_blast1d(nb=4, n=32, nguard=10, r0=0.5, p_in=10.0, r_in=0.3, vtheta=0.05, swirl=0.1, vr=0.0,
+                      gamma=1.4, **kw) -> Problem:
+    """1D spherical blast: r in [r0, r0 + 1], rho = 1, p = p_in for
+    r < r0 + r_in and 1 beyond, radial velocity vr, polar velocity vtheta
+    (the y component) and azimuthal velocity swirl * r (the z component).
+    info["geo"]: per block the geometry rows (spherical_rows)."""
+    cfg = base_cfg(ndim=1, nguard=nguard, gamma=gamma, geometry="spherical", **kw)
+    dx = 1.0 / (nb * n)
+    blocks, arrays, geo = [], [], []
+    nunk = SPARK_NPROP + cfg["nscalars"]
+    for bi in range(nb):
+        b = dict(nx=n, ny=1, nz=1, level=1, dx=dx, dy=1.0, dz=1.0)
+        NZ, NY, NX = gc_shape(cfg, b)
+        Z, Y, X = _centers(cfg, b, (r0 + bi * n * dx, 0.0, 0.0), NZ, NY, NX)
+        pres = np.where(X < r0 + r_in, p_in, 1.0)
+        dens = np.ones_like(pres)
+        U = np.zeros((NZ, NY, NX, nunk))
+        fill_thermo(U, dens, pres, np.full_like(pres, vr), np.full_like(pres, vtheta), swirl * X, gamma)
+        g = spherical_rows(n, nguard, bi, dx, r0)
+        blocks.append(dict(b, geo=g, geometry=2))
+        arrays.append(U)
+        geo.append(g)
+    return Problem("spherical_blast1d", cfg, blocks, arrays, ug=(nb, 1, 1), info=dict(geo=geo, r0=r0))
 
 
 def cylindrical_blast(nbx=4, nby=4, n=16, nguard=10, r0=1.0, p_in=10.0, radius=0.15, swirl=0.1, vr=0.0,
@@ -254,7 +295,7 @@ def cylindrical_blast(nbx=4, nby=4, n=16, nguard=10, r0=1.0, p_in=10.0, radius=0
             U = np.zeros((NZ, NY, NX, nunk))
             fill_thermo(U, dens, pres, np.full_like(pres, vr), np.zeros_like(pres), swirl * X, gamma)
             g = cylindrical_rows(n, nguard, bi, nbx, dx, dx, r0)
-            b = dict(b, geo=g)
+            b = dict(b, geo=g, geometry=1)
             blocks.append(b)
             arrays.append(U)
             geo.append(g)

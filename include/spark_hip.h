@@ -201,18 +201,21 @@ int spark_advance(spark_batch* b, int nsteps, double* dts);
 int spark_set_dt_control(spark_batch* b, double dt_init, double change_factor, double dt_min, double dt_max);
 
 /* Geometry of the plain form (the batched forms are Cartesian only).
- * SPARK_GEOM_CYLINDRICAL: 2D (r, z) with x the radius and the z velocity
- * the azimuthal one (the Grid's cell volumes exist for 2D only).  Per block, over the guard-celled x index (NX values):
- * the cell-centre radius, the area of the cell's low x face and the cell
- * volume -- the Grid's hy_xCenter, hy_farea and hy_cvol, which the update
- * and the flux correction turn into facM/facP = farea dx / cvol and the
- * geometric sources (rho vphi^2 + p)/r and -rho vphi vr / r.
+ * SPARK_GEOM_CYLINDRICAL: 2D (r, z), x the radius, the z velocity the
+ * azimuthal one.  SPARK_GEOM_SPHERICAL: 1D (r), the y and z velocities the
+ * polar and azimuthal ones.  Per block, over the guard-celled x index (NX
+ * values): the cell-centre radius, the area of the cell's low x face, the
+ * cell volume, the left and right edge radius -- the Grid's hy_xCenter,
+ * hy_farea, hy_cvol, hy_xLeft and hy_xRight, which the update and the flux
+ * correction turn into facM/facP = farea dx / cvol and the geometric
+ * sources ((rho vphi^2 + alpha p) / r, -rho vphi vr / r, and in spherical
+ * geometry (rho vtheta)^2 / rho / r, scaled by dx / dx_sph).
  * Replaces: Spark/Hydro.ini:40-56 (geom_1d_alloc), geoFacs in
  * Spark/hy_rk_updateSoln.F90-mc:98-175 and hy_rk_correctFluxes.F90:341-358.
  * SPARK_GEOM_CARTESIAN (the default) takes no arrays. */
-enum spark_geometry { SPARK_GEOM_CARTESIAN = 0, SPARK_GEOM_CYLINDRICAL = 1 };
+enum spark_geometry { SPARK_GEOM_CARTESIAN = 0, SPARK_GEOM_CYLINDRICAL = 1, SPARK_GEOM_SPHERICAL = 2 };
 int spark_set_geometry(spark_batch* b, int geometry, const double* const* r_center, const double* const* x_face_area,
-                       const double* const* cell_volume);
+                       const double* const* cell_volume, const double* const* r_left, const double* const* r_right);
 
 /* Flux registers (weighted sum over stages) in the shape of hy_fluxBuf[XYZ]:
  * every face of the block's interior, (NFLUXES, nx+1, ny, nz) for x etc.

@@ -35,7 +35,7 @@ class OrcParams(ctypes.Structure):
 class OrcBlock(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
                 ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
-                ("geo", ctypes.c_void_p)]
+                ("geo", ctypes.c_void_p), ("geometry", ctypes.c_int)]
 
 
 _lib = None
@@ -116,13 +116,14 @@ def params(cfg: dict) -> OrcParams:
 
 
 def block(b: dict) -> OrcBlock:
-    """b["geo"] (optional): (3, NX) float64 rows -- cell-centre radius, low
-    x-face area, cell volume -- for cylindrical geometry."""
+    """b["geo"] (optional): (5, NX) float64 rows -- cell-centre radius, low
+    x-face area, cell volume, left and right edge radius -- with
+    b["geometry"] 1 (cylindrical) or 2 (spherical)."""
     geo = b.get("geo")
     if geo is not None:
-        assert geo.dtype == np.float64 and geo.flags["C_CONTIGUOUS"]
+        assert geo.dtype == np.float64 and geo.flags["C_CONTIGUOUS"] and geo.shape[0] == 5
     return OrcBlock(int(b["nx"]), int(b["ny"]), int(b["nz"]), float(b["dx"]), float(b["dy"]), float(b["dz"]),
-                    geo.ctypes.data if geo is not None else None)
+                    geo.ctypes.data if geo is not None else None, int(b.get("geometry", 0)))
 
 
 def dptr(a: np.ndarray):

@@ -624,9 +624,18 @@ void orc_update_soln(const orc_params* P, const orc_block* b, const double* U0v,
         double Sgeo[MAXFLX], facM, facP, rc;
         for (int n = 0; n < nf; n++) Sgeo[n] = 0.;
         if (geo_facs(b, NX, i, dx, &facM, &facP, &rc)) {
-          Sgeo[HY_XMOM] = (Vs[ORC_DENS] * Vs[ORC_VELZ] * Vs[ORC_VELZ] + 1.0 * Vs[ORC_PRES]) / rc;
+          const int sph = b->geometry == 2;
+          const double alpha = sph ? 2. : 1.;
+          Sgeo[HY_XMOM] = (Vs[ORC_DENS] * Vs[ORC_VELZ] * Vs[ORC_VELZ] + alpha * Vs[ORC_PRES]) / rc;
           Sgeo[HY_ZMOM] = Vs[ORC_DENS] * Vs[ORC_VELZ] * Vs[ORC_VELX] / rc;
           Sgeo[HY_ZMOM] = -Sgeo[HY_ZMOM];
+          if (sph) {
+            /* the polar momentum is the y one; dx_sph (:126) */
+            const double rl = b->geo[3 * NX + i], rr = b->geo[4 * NX + i];
+            const double dx_sph = (rr * rr * rr - rl * rl * rl) / (3. * (rc * rc));
+            Sgeo[HY_XMOM] = Sgeo[HY_XMOM] + Us[HY_YMOM] * Us[HY_YMOM] / Vs[ORC_DENS] / rc;
+            for (int n = 0; n < nf; n++) Sgeo[n] = Sgeo[n] * dx / dx_sph;
+          }
         }
         for (int n = 0; n < nf; n++) St[n] = Sgeo[n] + Sg[n];
         for (int n = 0; n < nf; n++) {

@@ -312,6 +312,33 @@ def test_batched_forms_apply_the_driver_dt_limits(form):
 
 
 @pytest.mark.parametrize("guard_mode", ["telescope", "refill"])
+def test_spherical_geometry_matches_oracle(guard_mode):
+    """1D spherical blast with polar and azimuthal velocity: facM/facP,
+    (rho w^2 + 2 p)/r, -rho u w / r and (rho v)^2 / rho / r, scaled by
+    dx / dx_sph (geoFacs); the plain form equals the oracle bit for bit."""
+    kw = dict(nguard=6, guard_mode="refill") if guard_mode == "refill" else {}
+    prob = problems.spherical_blast1d(nb=4, n=32, **kw)
+    nsteps = 10
+    b = capi.Batch(prob.cfg, prob.blocks)
+    try:
+        b.pack(prob.arrays)
+        b.set_uniform_grid(*prob.ug)
+        b.set_geometry("spherical", prob.info["geo"])
+        d = b.advance(nsteps)
+        out = clone(prob.arrays)
+        b.unpack(out)
+        cnt = b.counters()
+    finally:
+        b.close()
+    ref = clone(prob.arrays)
+    dref, nerr = pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, ref, nsteps,
+                                     refill=guard_mode == "refill", geo=prob.info["geo"])
+    assert cnt[0] == 0 and nerr == 0
+    assert np.array_equal(d, dref), (d, dref)
+    compare(interior_views(prob, out), interior_views(prob, ref), f"spherical {guard_mode}")
+
+
+@pytest.mark.parametrize("guard_mode", ["telescope", "refill"])
 def test_cylindrical_geometry_matches_oracle(guard_mode):
     """2D cylindrical (r, z) blast with azimuthal swirl: geometric factors
     facM/facP on the radial faces and the sources (rho w^2 + p)/r and
@@ -326,7 +353,7 @@ def test_cylindrical_geometry_matches_oracle(guard_mode):
     try:
         b.pack(prob.arrays)
         b.set_uniform_grid(*prob.ug)
-        b.set_geometry("cylindrical", [g[0] for g in geo], [g[1] for g in geo], [g[2] for g in geo])
+        b.set_geometry("cylindrical", geo)
         d = b.advance(nsteps)
         out = clone(prob.arrays)
         b.unpack(out)
@@ -343,18 +370,19 @@ def test_cylindrical_geometry_matches_oracle(guard_mode):
 
 def test_geometry_is_refused_where_it_is_not_implemented():
     prob = problems.cylindrical_blast(nbx=1, nby=1, n=16)
-    geo = prob.info["geo"]
-    rows = ([g[0] for g in geo], [g[1] for g in geo], [g[2] for g in geo])
+    rows = prob.info["geo"]
     s3 = problems.sedov3d(nb=1, n=16, nguard=10)
     b = capi.Batch(dict(s3.cfg, form="batched_fp64"), s3.blocks)
     try:
         with pytest.raises(capi.SparkError, match="Cartesian only"):
-            b.set_geometry("cylindrical", *rows)
+            b.set_geometry("cylindrical", rows)
     finally:
         b.close()
     b = capi.Batch(s3.cfg, s3.blocks)
     try:
         with pytest.raises(capi.SparkError, match="2D"):
-            b.set_geometry("cylindrical", *rows)
+            b.set_geometry("cylindrical", rows)
+        with pytest.raises(capi.SparkError, match="1D"):
+            b.set_geometry("spherical", rows)
     finally:
         b.close()

@@ -338,12 +338,26 @@ def test_cylindrical_rows_are_the_grid_geometry():
     prob = _cyl()
     dx = prob.blocks[0]["dx"]
     for g in prob.info["geo"]:
-        rc, area, vol = g
+        rc, area, vol, rl, rr = g
         facm, facp = area[:-1] * dx / vol[:-1], area[1:] * dx / vol[:-1]
-        rl = rc - 0.5 * dx
         assert np.allclose(facm, rl[:-1] / rc[:-1], rtol=1e-14, atol=0)
-        assert np.allclose(facp, (rl[:-1] + dx) / rc[:-1], rtol=1e-14, atol=0)
+        assert np.allclose(facp, rr[:-1] / rc[:-1], rtol=1e-14, atol=0)
         assert np.allclose(np.diff(rc), dx, rtol=1e-12, atol=0)
+        assert np.allclose(0.5 * (rl + rr), rc, rtol=1e-15, atol=0)
+
+
+def test_spherical_rows_are_the_grid_geometry():
+    """Spherical: facM = 3 r_l^2 dr / (r_r^3 - r_l^3), facP with r_r, and
+    dx_sph = (r_r^3 - r_l^3) / (3 r_c^2) = dr (1 + dr^2 / (12 r_c^2))."""
+    prob = problems.spherical_blast1d(nb=2, n=16)
+    dx = prob.blocks[0]["dx"]
+    for g in prob.info["geo"]:
+        rc, area, vol, rl, rr = g
+        facm, facp = area[:-1] * dx / vol[:-1], area[1:] * dx / vol[:-1]
+        dv = rr ** 3 - rl ** 3
+        assert np.allclose(facm, 3.0 * rl[:-1] ** 2 * dx / dv[:-1], rtol=1e-12, atol=0)
+        assert np.allclose(facp, 3.0 * rr[:-1] ** 2 * dx / dv[:-1], rtol=1e-12, atol=0)
+        assert np.allclose(dv / (3.0 * rc * rc), dx * (1.0 + dx * dx / (12.0 * rc * rc)), rtol=1e-12, atol=0)
 
 
 def test_cylindrical_rest_state_is_a_fixed_point():
@@ -388,3 +402,42 @@ def test_cylindrical_uniform_flow_follows_the_axisymmetric_equations():
         assert np.allclose((rho - 1.0) / dt, -u / rc, rtol=1e-3)
         assert np.allclose((mu - u) / dt, (w * w - u * u) / rc, rtol=1e-3)
         assert np.allclose((mw - w) / dt, -2.0 * u * w / rc, rtol=1e-3)
+
+
+def test_spherical_rest_state_is_a_fixed_point():
+    """At rest with uniform p the spherical pressure flux difference
+    -3 p (r_r^2 - r_l^2) / (r_r^3 - r_l^3) cancels the source 2 p / r_c
+    scaled by dx / dx_sph: the state stays at rest to rounding."""
+    prob = problems.spherical_blast1d(nb=2, n=16, p_in=1.0, vtheta=0.0, swirl=0.0)
+    arrs = [a.copy() for a in prob.arrays]
+    d, nerr = pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, arrs, 3, geo=prob.info["geo"])
+    assert nerr == 0 and d[0] > 1e-3
+    g = prob.cfg["nguard"]
+    vmax = max(float(np.abs(a[0, 0, g:-g, VI["velx"]]).max()) for a in arrs)
+    dmax = max(float(np.abs(a[0, 0, g:-g, VI["dens"]] - 1.0).max()) for a in arrs)
+    assert vmax < 1e-13 and dmax < 1e-13, (vmax, dmax)
+
+
+def test_spherical_uniform_flow_follows_the_reference_rates():
+    """Uniform rho, p, radial u, polar v and azimuthal w over a short step:
+    d(rho)/dt = -2 rho u / r, d(rho u)/dt = rho (v^2 + w^2 - 2 u^2) / r,
+    d(rho w)/dt = -3 rho u w / r and, as the reference has no polar source,
+    d(rho v)/dt = -2 rho u v / r."""
+    u, v, w = 0.1, 0.05, 0.2
+    prob = problems.spherical_blast1d(nb=2, n=16, p_in=1.0, vtheta=v, swirl=0.0, vr=u)
+    for a in prob.arrays:
+        a[..., VI["velz"]] = w
+        a[..., VI["ener"]] = a[..., VI["eint"]] + 0.5 * (u * u + v * v + w * w)
+    arrs = [a.copy() for a in prob.arrays]
+    dt = 1e-5
+    with pyoracle.dt_control(dt_init=dt):
+        pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, arrs, 1, geo=prob.info["geo"])
+    g = prob.cfg["nguard"]
+    for a, geo in zip(arrs, prob.info["geo"]):
+        rc = geo[0][g + 2:-g - 2]
+        c = a[0, 0, g + 2:-g - 2]
+        rho = c[..., VI["dens"]]
+        assert np.allclose((rho - 1.0) / dt, -2.0 * u / rc, rtol=1e-3)
+        assert np.allclose((rho * c[..., VI["velx"]] - u) / dt, (v * v + w * w - 2.0 * u * u) / rc, rtol=1e-3)
+        assert np.allclose((rho * c[..., VI["velz"]] - w) / dt, -3.0 * u * w / rc, rtol=1e-3)
+        assert np.allclose((rho * c[..., VI["vely"]] - v) / dt, -2.0 * u * v / rc, rtol=1e-3)
