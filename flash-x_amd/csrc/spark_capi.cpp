@@ -1272,6 +1272,8 @@ int spark_set_geometry(spark_batch* b, int geometry, const double* const* r_cent
   if (geometry != SPARK_GEOM_CYLINDRICAL && geometry != SPARK_GEOM_SPHERICAL)
     return fail(SPARK_ERR_ARG, "spark_set_geometry: geometry %d", geometry);
   if (b->fused) return fail(SPARK_ERR_ARG, "spark_set_geometry: the batched forms are Cartesian only");
+  // the hierarchy's prolongation and flux restriction are the Cartesian ones
+  if (b->amr) return fail(SPARK_ERR_STATE, "spark_set_geometry: the block hierarchy is Cartesian only");
   // the Grid's cell volumes exist for cylindrical geometry in 2D and
   // spherical in 1D and 2D (Grid_getCellVolumes.F90:49-55); spherical 2D
   // areas and volumes depend on theta, which these radial rows do not carry
@@ -1443,6 +1445,7 @@ int spark_set_amr_grid(spark_batch* b, int nbx0, int nby0, int nbz0, const int* 
     return fail(SPARK_ERR_ARG, "spark_set_amr_grid: the block hierarchy is filled once per step (telescope mode)");
   if (nbx0 < 1 || nby0 < 1 || nbz0 < 1) return fail(SPARK_ERR_ARG, "spark_set_amr_grid: bad root grid");
   if (b->has_ug) return fail(SPARK_ERR_STATE, "spark_set_amr_grid: the batch has a uniform grid");
+  if (b->dp.geom != 0) return fail(SPARK_ERR_STATE, "spark_set_amr_grid: the block hierarchy is Cartesian only");
   if (b->classes.size() != 1) return fail(SPARK_ERR_ARG, "spark_set_amr_grid: blocks of one shape only");
   const int n = b->meta_h[0].nx;
   if (b->meta_h[0].ny != n || b->meta_h[0].nz != n || (n & 1))

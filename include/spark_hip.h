@@ -212,7 +212,8 @@ int spark_set_dt_control(spark_batch* b, double dt_init, double change_factor, d
  * geometry (rho vtheta)^2 / rho / r, scaled by dx / dx_sph).
  * Replaces: Spark/Hydro.ini:40-56 (geom_1d_alloc), geoFacs in
  * Spark/hy_rk_updateSoln.F90-mc:98-175 and hy_rk_correctFluxes.F90:341-358.
- * SPARK_GEOM_CARTESIAN (the default) takes no arrays. */
+ * SPARK_GEOM_CARTESIAN (the default) takes no arrays.  A block hierarchy
+ * (spark_set_amr_grid) is Cartesian only. */
 enum spark_geometry { SPARK_GEOM_CARTESIAN = 0, SPARK_GEOM_CYLINDRICAL = 1, SPARK_GEOM_SPHERICAL = 2 };
 int spark_set_geometry(spark_batch* b, int geometry, const double* const* r_center, const double* const* x_face_area,
                        const double* const* cell_volume, const double* const* r_left, const double* const* r_right);
