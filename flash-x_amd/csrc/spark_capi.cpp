@@ -900,7 +900,7 @@ int spark_batch_create(const spark_params* p, int nblocks, const spark_block_des
   d.ndim = p->ndim; d.ng = p->nguard; d.ns = p->nscalars;
   d.use_grav = p->use_gravity; d.use_flat = p->use_flattening; d.hybrid = p->use_hybrid_riemann;
   d.recon = p->recon; d.riemann = p->riemann;
-  d.geom = 0; d.geo_nx = 0; d.geo = nullptr;  // Cartesian until spark_set_geometry
+  d.geom = 0; d.geo_nx = 0; d.geo_ny = 0; d.geo = nullptr;  // Cartesian until spark_set_geometry
 
   const int g = p->nguard;
   const int nf = 5 + p->nscalars;
@@ -1275,31 +1275,39 @@ int spark_set_geometry(spark_batch* b, int geometry, const double* const* r_cent
   // the hierarchy's prolongation and flux restriction are the Cartesian ones
   if (b->amr) return fail(SPARK_ERR_STATE, "spark_set_geometry: the block hierarchy is Cartesian only");
   // the Grid's cell volumes exist for cylindrical geometry in 2D and
-  // spherical in 1D and 2D (Grid_getCellVolumes.F90:49-55); spherical 2D
-  // areas and volumes depend on theta, which these radial rows do not carry
+  // spherical in 1D and 2D (Grid_getCellVolumes.F90:49-55)
   if (geometry == SPARK_GEOM_CYLINDRICAL && b->p.ndim != 2)
     return fail(SPARK_ERR_ARG, "spark_set_geometry: cylindrical geometry is 2D (r, z) only");
-  if (geometry == SPARK_GEOM_SPHERICAL && b->p.ndim != 1)
-    return fail(SPARK_ERR_ARG, "spark_set_geometry: spherical geometry is 1D (r) only");
+  if (geometry == SPARK_GEOM_SPHERICAL && b->p.ndim > 2)
+    return fail(SPARK_ERR_ARG, "spark_set_geometry: spherical geometry is 1D (r) or 2D (r, theta) only");
   if (b->classes.size() != 1) return fail(SPARK_ERR_ARG, "spark_set_geometry: blocks of one shape only");
   if (!r_center || !x_face_area || !cell_volume || !r_left || !r_right)
     return fail(SPARK_ERR_ARG, "spark_set_geometry: null argument");
-  const int NX = b->meta_h[0].NX;
-  const int R = spark::GEO_ROWS;
-  std::vector<double> h((size_t)b->nblk * R * NX);
+  const int NX = b->meta_h[0].NX, NY = b->meta_h[0].NY;
+  const int64_t S = spark::geo_stride(NX, NY), nxy = (int64_t)NX * NY;
+  std::vector<double> h((size_t)(b->nblk * S));
   for (int i = 0; i < b->nblk; i++) {
-    const double* rows[5] = {r_center[i], x_face_area[i], cell_volume[i], r_left[i], r_right[i]};
-    for (int r = 0; r < R; r++)
-      if (!rows[r]) return fail(SPARK_ERR_ARG, "spark_set_geometry: block %d array %d is null", i, r);
+    if (!r_center[i] || !x_face_area[i] || !cell_volume[i] || !r_left[i] || !r_right[i])
+      return fail(SPARK_ERR_ARG, "spark_set_geometry: block %d: null array", i);
+    double* o = h.data() + (size_t)(i * S);
     for (int q = 0; q < NX; q++) {
-      if (!(cell_volume[i][q] != 0.0)) return fail(SPARK_ERR_ARG, "block %d: cell volume %d is zero", i, q);
-      for (int r = 0; r < R; r++) h[((size_t)i * R + r) * NX + q] = rows[r][q];
+      o[q] = r_center[i][q];
+      o[NX + q] = r_left[i][q];
+      o[2 * NX + q] = r_right[i][q];
+    }
+    for (int64_t q = 0; q < nxy; q++) {
+      if (!(cell_volume[i][q] != 0.0)) return fail(SPARK_ERR_ARG, "block %d: cell volume %lld is zero", i, (long long)q);
+      o[3 * NX + q] = x_face_area[i][q];
+      o[3 * NX + nxy + q] = cell_volume[i][q];
     }
   }
-  if (!b->geo_d) HIPCHK(hipMalloc((void**)&b->geo_d, sizeof(double) * h.size()));
+  if (b->geo_d) (void)hipFree(b->geo_d);
+  b->geo_d = nullptr;
+  HIPCHK(hipMalloc((void**)&b->geo_d, sizeof(double) * h.size()));
   HIPCHK(hipMemcpy(b->geo_d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
   b->dp.geom = geometry == SPARK_GEOM_CYLINDRICAL ? 1 : 2;
   b->dp.geo_nx = NX;
+  b->dp.geo_ny = NY;
   b->dp.geo = b->geo_d;
   drop_graph(b);  // the kernels take the parameters by value
   return SPARK_OK;

@@ -329,15 +329,16 @@ struct DevParams {
   double gasConst;
   double mg_gm1[8];     // 1/(gamma_s - 1) of each species
   double mg_ainv[8];    // 1/A_s
-  // geometry: 0 Cartesian, 1 cylindrical (r, z), 2 spherical (r).  geo holds
-  // per block (BlockMeta::id) GEO_ROWS rows of geo_nx values over the
-  // guard-celled x index: cell-centre radius, area of the cell's low x face,
-  // cell volume, left and right edge radius (the Grid's hy_xCenter,
-  // hy_farea, hy_cvol, hy_xLeft, hy_xRight; Hydro.ini:40-56)
-  int geom, geo_nx;
+  // geometry: 0 Cartesian, 1 cylindrical (r, z), 2 spherical (r[, theta]).
+  // geo holds per block (BlockMeta::id), over the guard-celled box of
+  // geo_nx x geo_ny cells: the cell-centre, left-edge and right-edge radius
+  // (geo_nx each), then the area of each cell's low x face and each cell's
+  // volume (geo_nx * geo_ny each, x fastest) -- the Grid's hy_xCenter,
+  // hy_xLeft, hy_xRight, hy_farea and hy_cvol (Hydro.ini:40-56)
+  int geom, geo_nx, geo_ny;
   const double* geo;
 };
-constexpr int GEO_ROWS = 5;
+__host__ __device__ __forceinline__ int64_t geo_stride(int nx, int ny) { return 3 * (int64_t)nx + 2 * (int64_t)nx * ny; }
 
 // geometric sources of the update (geoFacs, hy_rk_updateSoln.F90-mc:131-
 // 170) from the star state V (vd, vx, vz, p) and the star conserved theta
@@ -346,12 +347,12 @@ constexpr int GEO_ROWS = 5;
 // dx_sph = (r_R^3 - r_L^3) / (3 r_c^2); the azimuthal momentum is the z one
 __device__ __forceinline__ void geo_sources(const DevParams& P, int id, int i, double dx, double rc, double vd,
                                             double vx, double vz, double pres, double uy, double* Sgeo) {
+  const double* g = P.geo + (int64_t)id * geo_stride(P.geo_nx, P.geo_ny);
   const double alpha = P.geom == 2 ? 2. : 1.;
   Sgeo[F_XMOM] = (vd * vz * vz + alpha * pres) / rc;
   Sgeo[F_ZMOM] = -(vd * vz * vx / rc);
   if (P.geom == 2) {
-    const double* g = P.geo + (int64_t)id * GEO_ROWS * P.geo_nx;
-    const double rl = g[3 * P.geo_nx + i], rr = g[4 * P.geo_nx + i];
+    const double rl = g[P.geo_nx + i], rr = g[2 * P.geo_nx + i];
     const double dx_sph = (rr * rr * rr - rl * rl * rl) / (3. * (rc * rc));
     Sgeo[F_XMOM] = Sgeo[F_XMOM] + uy * uy / vd / rc;
 #pragma unroll
@@ -359,21 +360,24 @@ __device__ __forceinline__ void geo_sources(const DevParams& P, int id, int i, d
   }
 }
 
-// geoFacs for the x faces of cell i (hy_rk_updateSoln.F90-mc:98-175,
+// geoFacs for the x faces of cell (i, j) (hy_rk_updateSoln.F90-mc:98-175,
 // hy_rk_correctFluxes.F90:341-358): facM, facP = farea * dx / cvol of the
 // low and high face, and the cell-centre radius; all factors 1 (and rc 0)
 // in Cartesian geometry.  Returns false where the reference zeroes them
 // (a cell centre at negative radius).
-__device__ __forceinline__ bool geo_facs(const DevParams& P, int id, int i, double dx, double& facM, double& facP,
-                                         double& rc) {
+__device__ __forceinline__ bool geo_facs(const DevParams& P, int id, int i, int j, double dx, double& facM,
+                                         double& facP, double& rc) {
   facM = 1.0;
   facP = 1.0;
   rc = 0.0;
   if (P.geom == 0) return true;
-  const double* g = P.geo + (int64_t)id * GEO_ROWS * P.geo_nx;
+  const double* g = P.geo + (int64_t)id * geo_stride(P.geo_nx, P.geo_ny);
+  const double* area = g + 3 * P.geo_nx;
+  const double* vol = area + (int64_t)P.geo_nx * P.geo_ny;
+  const int64_t q = (int64_t)j * P.geo_nx + i;
   rc = g[i];
-  facM = g[P.geo_nx + i] * dx / g[2 * P.geo_nx + i];
-  facP = g[P.geo_nx + i + 1] * dx / g[2 * P.geo_nx + i];
+  facM = area[q] * dx / vol[q];
+  facP = area[q + 1] * dx / vol[q];
   if (rc < 0.0) {
     facM = 0.;
     facP = 0.;

@@ -335,7 +335,7 @@ __global__ void __launch_bounds__(256) k_update(DevParams P, const BlockMeta* __
     const int64_t fpz = m.fbase + fix(m, i, j, k + 1);
     // geometric factors and sources (geoFacs, hy_rk_updateSoln.F90-mc:98-175)
     double facM, facP, rc, Sgeo[F_NUM] = {0., 0., 0., 0., 0.};
-    if (geo_facs(P, m.id, i, dx, facM, facP, rc) && P.geom != 0)
+    if (geo_facs(P, m.id, i, j, dx, facM, facP, rc) && P.geom != 0)
       geo_sources(P, m.id, i, dx, rc, ui[V_DENS * cs + c], ui[V_VELX * cs + c], ui[V_VELZ * cs + c],
                   ui[V_PRES * cs + c], B[F_YMOM], Sgeo);
 #pragma unroll
@@ -469,7 +469,10 @@ __global__ void __launch_bounds__(256) k_dt_part(DevParams P, const BlockMeta* _
     const double snd2 = up[V_GAMC * cs + c] * up[V_PRES * cs + c] / up[V_DENS * cs + c];
     const double cf = Num<double>::sqr(snd2);
     double lt = (__builtin_fabs(up[V_VELX * cs + c]) + cf) * dxi;
-    if (P.ndim > 1) lt = fmax2(lt, (__builtin_fabs(up[V_VELY * cs + c]) + cf) * dyi);
+    // spherical: y is theta, delyinv = 1/(r dtheta) (Hydro_computeDt.F90:104-111)
+    const double dyc = (P.geom == 2 && P.ndim > 1) ? 1.0 / (P.geo[(int64_t)m.id * geo_stride(P.geo_nx, P.geo_ny) + i] * m.dy)
+                                                    : dyi;
+    if (P.ndim > 1) lt = fmax2(lt, (__builtin_fabs(up[V_VELY * cs + c]) + cf) * dyc);
     if (P.ndim > 2) lt = fmax2(lt, (__builtin_fabs(up[V_VELZ * cs + c]) + cf) * dzi);
     const double key = lt * P.cfl;
     if (key > bk || (key == bk && (long long)c < bi)) { bk = key; bl = lt; bi = c; }
@@ -652,7 +655,7 @@ __global__ void __launch_bounds__(256) k_correct(DevParams P, const BlockMeta* _
         double fac = 1.0;
         if (d == 0 && P.geom != 0) {
           double facM, facP, rc;
-          geo_facs(P, m.id, L.lo[0] + c[0], m.dx, facM, facP, rc);
+          geo_facs(P, m.id, L.lo[0] + c[0], L.lo[1] + c[1], m.dx, facM, facP, rc);
           fac = side ? facP : facM;
         }
         for (int q = 0; q < nf; q++) {

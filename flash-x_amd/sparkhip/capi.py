@@ -262,16 +262,14 @@ class Batch:
 
     def set_geometry(self, geometry, rows=None):
         """geometry GEOM["cartesian"] (0), ["cylindrical"] (1) or
-        ["spherical"] (2); rows: per block a (5, NX) array over the
-        guard-celled x index -- cell-centre radius, low x-face area, cell
-        volume, left and right edge radius."""
+        ["spherical"] (2); rows: per block a dict of the Grid arrays over the
+        guard-celled box -- "rc", "rl", "rr" (NX), "area", "vol" (NY x NX)."""
         g = GEOM[geometry] if isinstance(geometry, str) else int(geometry)
         if g == 0:
             check(lib().spark_set_geometry(self.h, 0, None, None, None, None, None), "spark_set_geometry")
             return
-        keep = [[np.ascontiguousarray(r[q], dtype=np.float64) for r in rows] for q in range(5)]
-        ptrs = [_ptrs(k) for k in keep]
-        check(lib().spark_set_geometry(self.h, g, *ptrs), "spark_set_geometry")
+        keep = [[np.ascontiguousarray(r[k], dtype=np.float64) for r in rows] for k in ("rc", "area", "vol", "rl", "rr")]
+        check(lib().spark_set_geometry(self.h, g, *[_ptrs(k) for k in keep]), "spark_set_geometry")
 
     def set_dt_control(self, dt_init=0.0, change_factor=0.0, dt_min=0.0, dt_max=0.0):
         """The driver's dt limits for advance() (all 0: the CFL dt)."""

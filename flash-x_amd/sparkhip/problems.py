@@ -213,62 +213,89 @@ def species_tube3d(nb=2, n=16, nguard=10, species_a=(1.0, 4.0), species_gamma=(1
     return Problem("species_tube3d", cfg, blocks, arrays, ug=(nb, nb, nb))
 
 
-def _radii(n, nguard, bi, dx, r0):
-    """Cell-centre, left and right edge radius over the guard-celled x index
-    of block bi, as Grid_getCellCoords.F90:173-200 computes them:
-    r0 + (K - shift) dx with K the 1-based global cell index plus one and
-    shift 1.5 (centre), 2 (left edge, faces) or 1 (right edge)."""
+def _coords(n, nguard, bi, dx, x0):
+    """Cell-centre, left and right edge coordinates over the guard-celled
+    index of block bi along one axis, as Grid_getCellCoords.F90:173-200
+    computes them: x0 + (K - shift) dx with K the 1-based global cell index
+    plus one and shift 1.5 (centre), 2 (left edge, faces) or 1 (right edge)."""
     NX = n + 2 * nguard
     K = (bi * n + 1 + (np.arange(NX) - nguard) + 1).astype(np.float64)
-    return r0 + (K - 1.5) * dx, r0 + (K - 2.0) * dx, r0 + (K - 1.0) * dx
+    return x0 + (K - 1.5) * dx, x0 + (K - 2.0) * dx, x0 + (K - 1.0) * dx
 
 
-def cylindrical_rows(n, nguard, bi, nb, dx, dz, r0):
-    """Geometry rows of block bi along r (2D cylindrical, r from r0): cell-
-    centre radius, low x-face area, cell volume, left and right edge radius
-    over the guard-celled x index, as the Grid computes them: area
-    2 pi |r_left| dz (Grid_getCellFaceAreas.F90, CYLINDRICAL, 2D), volume
-    2 pi |r_centre| dr dz (Grid_getCellVolumes.F90, CYLINDRICAL, 2D)."""
-    rc, rl, rr = _radii(n, nguard, bi, dx, r0)
+def geo_flat(rows):
+    """The block's geometry arrays in the layout the oracle reads: rc, rl,
+    rr, then area and volume (NY x NX, x fastest)."""
+    return np.ascontiguousarray(np.concatenate([rows["rc"], rows["rl"], rows["rr"], rows["area"].ravel(),
+                                                rows["vol"].ravel()]))
+
+
+def cylindrical_rows(n, nguard, bi, nb, dx, dz, r0, ny=1):
+    """Grid arrays of block bi along r (2D cylindrical, r from r0) over the
+    guard-celled box (ny rows in z): radii (_coords), area 2 pi |r_left| dz
+    (Grid_getCellFaceAreas.F90, CYLINDRICAL, 2D) and volume
+    2 pi |r_centre| dr dz (Grid_getCellVolumes.F90, CYLINDRICAL, 2D), the
+    same for every z row."""
+    rc, rl, rr = _coords(n, nguard, bi, dx, r0)
     area = 2.0 * np.pi * np.abs(rl) * dz
     vol = 2.0 * np.pi * np.abs(rc) * dx * dz
-    return np.ascontiguousarray(np.stack([rc, area, vol, rl, rr]))
+    return dict(rc=rc, rl=rl, rr=rr, area=np.tile(area, (ny, 1)), vol=np.tile(vol, (ny, 1)))
 
 
-def spherical_rows(n, nguard, bi, dx, r0):
-    """Geometry rows of block bi along r (1D spherical): as cylindrical_rows
-    with area r_left^2 4 pi (Grid_getCellFaceAreas.F90, SPHERICAL, 1D) and
-    volume dr (r_l^2 + r_l r_r + r_r^2) 4 pi / 3 (Grid_getCellVolumes.F90,
-    SPHERICAL, 1D)."""
-    rc, rl, rr = _radii(n, nguard, bi, dx, r0)
-    area = rl * rl * 4.0 * np.pi
-    vol = dx * (rl * rl + rl * rr + rr * rr) * 4.0 * np.pi / 3.0
-    return np.ascontiguousarray(np.stack([rc, area, vol, rl, rr]))
+def spherical_rows(n, nguard, bi, dx, r0, thf=None):
+    """Grid arrays of block bi along r (spherical, r from r0): radii, area
+    r_left^2 4 pi and volume dr (r_l^2 + r_l r_r + r_r^2) 4 pi / 3 in 1D;
+    with the theta faces thf (NY + 1) of the block's rows, area
+    r_left^2 (cos th_j - cos th_j+1) 2 pi and volume
+    dr (...) (cos th_j - cos th_j+1) 2 pi / 3 (Grid_getCellFaceAreas.F90,
+    Grid_getCellVolumes.F90, SPHERICAL)."""
+    rc, rl, rr = _coords(n, nguard, bi, dx, r0)
+    cv = dx * (rl * rl + rl * rr + rr * rr)
+    if thf is None:
+        area = (rl * rl * 4.0 * np.pi)[None, :]
+        vol = (cv * 4.0 * np.pi / 3.0)[None, :]
+    else:
+        dc = (np.cos(thf[:-1]) - np.cos(thf[1:]))[:, None]
+        area = (rl * rl)[None, :] * dc * 2.0 * np.pi
+        vol = cv[None, :] * dc * 2.0 * np.pi / 3.0
+    return dict(rc=rc, rl=rl, rr=rr, area=np.ascontiguousarray(area), vol=np.ascontiguousarray(vol))
 
 
-def spherical_blast1d(nb=4, n=32, nguard=10, r0=0.5, p_in=10.0, r_in=0.3, vtheta=0.05, swirl=0.1, vr=0.0,
-                      gamma=1.4, **kw) -> Problem:
-    """1D spherical blast: r in [r0, r0 + 1], rho = 1, p = p_in for
-    r < r0 + r_in and 1 beyond, radial velocity vr, polar velocity vtheta
-    (the y component) and azimuthal velocity swirl * r (the z component).
-    info["geo"]: per block the geometry rows (spherical_rows)."""
-    cfg = base_cfg(ndim=1, nguard=nguard, gamma=gamma, geometry="spherical", **kw)
-    dx = 1.0 / (nb * n)
+def spherical_blast(nb=(4, 1), n=32, nguard=10, ndim=1, r0=0.5, th0=np.pi / 4, p_in=10.0, r_in=0.3,
+                    vtheta=0.05, swirl=0.1, vr=0.0, gamma=1.4, **kw) -> Problem:
+    """Spherical blast: r in [r0, r0 + 1] (and, in 2D, theta from th0 over
+    nb[1] / nb[0] rad), rho = 1, p = p_in for r < r0 + r_in (2D: also theta
+    below the middle) and 1 beyond, radial velocity vr, polar velocity
+    vtheta (y component) and azimuthal velocity swirl * r (z component).
+    info["geo"]: per block the Grid arrays (spherical_rows); each block dict
+    carries them flat for the oracle."""
+    cfg = base_cfg(ndim=ndim, nguard=nguard, gamma=gamma, geometry="spherical", **kw)
+    nbr, nbt = nb[0], (nb[1] if ndim > 1 else 1)
+    dx = 1.0 / (nbr * n)
     blocks, arrays, geo = [], [], []
     nunk = SPARK_NPROP + cfg["nscalars"]
-    for bi in range(nb):
-        b = dict(nx=n, ny=1, nz=1, level=1, dx=dx, dy=1.0, dz=1.0)
-        NZ, NY, NX = gc_shape(cfg, b)
-        Z, Y, X = _centers(cfg, b, (r0 + bi * n * dx, 0.0, 0.0), NZ, NY, NX)
-        pres = np.where(X < r0 + r_in, p_in, 1.0)
-        dens = np.ones_like(pres)
-        U = np.zeros((NZ, NY, NX, nunk))
-        fill_thermo(U, dens, pres, np.full_like(pres, vr), np.full_like(pres, vtheta), swirl * X, gamma)
-        g = spherical_rows(n, nguard, bi, dx, r0)
-        blocks.append(dict(b, geo=g, geometry=2))
-        arrays.append(U)
-        geo.append(g)
-    return Problem("spherical_blast1d", cfg, blocks, arrays, ug=(nb, 1, 1), info=dict(geo=geo, r0=r0))
+    thmid = th0 + 0.5 * nbt * n * dx
+    for bj in range(nbt):
+        for bi in range(nbr):
+            b = dict(nx=n, ny=n if ndim > 1 else 1, nz=1, level=1, dx=dx, dy=dx if ndim > 1 else 1.0, dz=1.0)
+            NZ, NY, NX = gc_shape(cfg, b)
+            Z, Y, X = _centers(cfg, b, (r0 + bi * n * dx, th0 + bj * n * dx, 0.0), NZ, NY, NX)
+            hot = X < r0 + r_in
+            if ndim > 1:
+                hot = hot & (Y < thmid)
+            pres = np.where(hot, p_in, 1.0)
+            dens = np.ones_like(pres)
+            U = np.zeros((NZ, NY, NX, nunk))
+            fill_thermo(U, dens, pres, np.full_like(pres, vr), np.full_like(pres, vtheta), swirl * X, gamma)
+            thf = None
+            if ndim > 1:
+                _, tl, tr = _coords(n, nguard, bj, dx, th0)
+                thf = np.append(tl, tr[-1])
+            g = spherical_rows(n, nguard, bi, dx, r0, thf)
+            blocks.append(dict(b, geo=geo_flat(g), geometry=2))
+            arrays.append(U)
+            geo.append(g)
+    return Problem("spherical_blast", cfg, blocks, arrays, ug=(nbr, nbt, 1), info=dict(geo=geo, r0=r0))
 
 
 def cylindrical_blast(nbx=4, nby=4, n=16, nguard=10, r0=1.0, p_in=10.0, radius=0.15, swirl=0.1, vr=0.0,
@@ -294,8 +321,8 @@ def cylindrical_blast(nbx=4, nby=4, n=16, nguard=10, r0=1.0, p_in=10.0, radius=0
             dens = np.ones_like(pres)
             U = np.zeros((NZ, NY, NX, nunk))
             fill_thermo(U, dens, pres, np.full_like(pres, vr), np.zeros_like(pres), swirl * X, gamma)
-            g = cylindrical_rows(n, nguard, bi, nbx, dx, dx, r0)
-            b = dict(b, geo=g, geometry=1)
+            g = cylindrical_rows(n, nguard, bi, nbx, dx, dx, r0, ny=NY)
+            b = dict(b, geo=geo_flat(g), geometry=1)
             blocks.append(b)
             arrays.append(U)
             geo.append(g)

@@ -202,16 +202,19 @@ int spark_set_dt_control(spark_batch* b, double dt_init, double change_factor, d
 
 /* Geometry of the plain form (the batched forms are Cartesian only).
  * SPARK_GEOM_CYLINDRICAL: 2D (r, z), x the radius, the z velocity the
- * azimuthal one.  SPARK_GEOM_SPHERICAL: 1D (r), the y and z velocities the
- * polar and azimuthal ones.  Per block, over the guard-celled x index (NX
- * values): the cell-centre radius, the area of the cell's low x face, the
- * cell volume, the left and right edge radius -- the Grid's hy_xCenter,
- * hy_farea, hy_cvol, hy_xLeft and hy_xRight, which the update and the flux
- * correction turn into facM/facP = farea dx / cvol and the geometric
- * sources ((rho vphi^2 + alpha p) / r, -rho vphi vr / r, and in spherical
- * geometry (rho vtheta)^2 / rho / r, scaled by dx / dx_sph).
+ * azimuthal one.  SPARK_GEOM_SPHERICAL: 1D (r) or 2D (r, theta), the y and
+ * z velocities the polar and azimuthal ones.  Per block, over its
+ * guard-celled box of NX x NY cells (NY = 1 in 1D): the cell-centre radius,
+ * the area of each cell's low x face (NX * NY, x fastest), each cell's
+ * volume (NX * NY), the left and right edge radius (NX each) -- the Grid's
+ * hy_xCenter, hy_farea, hy_cvol, hy_xLeft and hy_xRight, which the update
+ * and the flux correction turn into facM/facP = farea dx / cvol and the
+ * geometric sources ((rho vphi^2 + alpha p) / r, -rho vphi vr / r, and in
+ * spherical geometry (rho vtheta)^2 / rho / r, scaled by dx / dx_sph); the
+ * CFL step takes r dtheta as the theta width.
  * Replaces: Spark/Hydro.ini:40-56 (geom_1d_alloc), geoFacs in
- * Spark/hy_rk_updateSoln.F90-mc:98-175 and hy_rk_correctFluxes.F90:341-358.
+ * Spark/hy_rk_updateSoln.F90-mc:98-175 and hy_rk_correctFluxes.F90:341-358,
+ * Spark/Hydro_computeDt.F90:103-111.
  * SPARK_GEOM_CARTESIAN (the default) takes no arrays.  A block hierarchy
  * (spark_set_amr_grid) is Cartesian only. */
 enum spark_geometry { SPARK_GEOM_CARTESIAN = 0, SPARK_GEOM_CYLINDRICAL = 1, SPARK_GEOM_SPHERICAL = 2 };

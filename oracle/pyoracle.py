@@ -116,12 +116,12 @@ def params(cfg: dict) -> OrcParams:
 
 
 def block(b: dict) -> OrcBlock:
-    """b["geo"] (optional): (5, NX) float64 rows -- cell-centre radius, low
-    x-face area, cell volume, left and right edge radius -- with
+    """b["geo"] (optional): the flat float64 geometry arrays of the block
+    (rc, rl, rr over NX, then area and volume over NY x NX) with
     b["geometry"] 1 (cylindrical) or 2 (spherical)."""
     geo = b.get("geo")
     if geo is not None:
-        assert geo.dtype == np.float64 and geo.flags["C_CONTIGUOUS"] and geo.shape[0] == 5
+        assert geo.dtype == np.float64 and geo.flags["C_CONTIGUOUS"] and geo.ndim == 1
     return OrcBlock(int(b["nx"]), int(b["ny"]), int(b["nz"]), float(b["dx"]), float(b["dy"]), float(b["dz"]),
                     geo.ctypes.data if geo is not None else None, int(b.get("geometry", 0)))
 
@@ -155,7 +155,7 @@ def compute_dt(cfg, b, U):
 
 
 def ug_advance(cfg, b, ug, arrays, nsteps, refill=False, nthreads=0, geo=None):
-    """geo (optional): per block (3, NX) geometry rows (see block())."""
+    """geo (optional): per block the flat geometry arrays (see block())."""
     P, B = params(cfg), block(b)
     dts = np.zeros(max(1, nsteps))
     gp = ptr_array(geo) if geo is not None else None  # kept alive: the oracle holds the pointers

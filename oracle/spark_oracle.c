@@ -560,14 +560,18 @@ void orc_eos_wrapped(const orc_params* P, const orc_block* b, double* S, const i
  * for cylindrical geometry: facM, facP = farea * dx / cvol of the x faces
  * of cell i; 0 and no sources where the cell centre is at negative radius.
  * Returns 1 where the sources apply. */
-static int geo_facs(const orc_block* b, int NX, int i, double dx, double* facM, double* facP, double* rc) {
+static int geo_facs(const orc_block* b, int NX, int NY, int i, int j, double dx, double* facM, double* facP,
+                    double* rc) {
   *facM = 1.0;
   *facP = 1.0;
   *rc = 0.0;
   if (!b->geo) return 0;
+  const double* area = b->geo + 3 * NX;
+  const double* vol = area + (size_t)NX * NY;
+  const size_t q = (size_t)j * NX + i;
   *rc = b->geo[i];
-  *facM = b->geo[NX + i] * dx / b->geo[2 * NX + i];
-  *facP = b->geo[NX + i + 1] * dx / b->geo[2 * NX + i];
+  *facM = area[q] * dx / vol[q];
+  *facP = area[q + 1] * dx / vol[q];
   if (*rc < 0.0) {
     *facM = 0.;
     *facP = 0.;
@@ -623,7 +627,7 @@ void orc_update_soln(const orc_params* P, const orc_block* b, const double* U0v,
          * momentum is the z one, V the star state (geoFacs :131-157) */
         double Sgeo[MAXFLX], facM, facP, rc;
         for (int n = 0; n < nf; n++) Sgeo[n] = 0.;
-        if (geo_facs(b, NX, i, dx, &facM, &facP, &rc)) {
+        if (geo_facs(b, NX, NY, i, j, dx, &facM, &facP, &rc)) {
           const int sph = b->geometry == 2;
           const double alpha = sph ? 2. : 1.;
           Sgeo[HY_XMOM] = (Vs[ORC_DENS] * Vs[ORC_VELZ] * Vs[ORC_VELZ] + alpha * Vs[ORC_PRES]) / rc;
@@ -631,7 +635,7 @@ void orc_update_soln(const orc_params* P, const orc_block* b, const double* U0v,
           Sgeo[HY_ZMOM] = -Sgeo[HY_ZMOM];
           if (sph) {
             /* the polar momentum is the y one; dx_sph (:126) */
-            const double rl = b->geo[3 * NX + i], rr = b->geo[4 * NX + i];
+            const double rl = b->geo[NX + i], rr = b->geo[2 * NX + i];
             const double dx_sph = (rr * rr * rr - rl * rl * rl) / (3. * (rc * rc));
             Sgeo[HY_XMOM] = Sgeo[HY_XMOM] + Us[HY_YMOM] * Us[HY_YMOM] / Vs[ORC_DENS] / rc;
             for (int n = 0; n < nf; n++) Sgeo[n] = Sgeo[n] * dx / dx_sph;
@@ -723,6 +727,8 @@ double orc_compute_dt(const orc_params* P, const orc_block* b, const double* U, 
         const double* v = U + (size_t)nunk * cidx(NX, NY, i, j, k);
         double snd2 = v[ORC_GAMC] * v[ORC_PRES] / v[ORC_DENS];
         double lt = (fabs(v[ORC_VELX]) + sqrt(snd2)) * delxinv;
+        /* spherical: y is theta, delyinv = 1/(x(i) dy) (Hydro_computeDt.F90:108-110) */
+        if (b->geo && b->geometry == 2 && P->ndim > 1) delyinv = 1.0 / (b->geo[i] * b->dy);
         if (P->ndim > 1) lt = fmax2(lt, (fabs(v[ORC_VELY]) + sqrt(snd2)) * delyinv);
         if (P->ndim > 2) lt = fmax2(lt, (fabs(v[ORC_VELZ]) + sqrt(snd2)) * delzinv);
         if (lt * P->cfl > dt_temp * P->cfl) {
@@ -902,7 +908,7 @@ void orc_correct_fluxes(const orc_params* P, const orc_block* b, double* U, doub
           double fac = 1.0;
           if (dir == 0 && b->geo) {
             double facM, facP, rc;
-            geo_facs(b, NX, lim[0][0] + c[0], b->dx, &facM, &facP, &rc);
+            geo_facs(b, NX, NY, lim[0][0] + c[0], lim[1][0] + c[1], b->dx, &facM, &facP, &rc);
             fac = side ? facP : facM;
           }
           correct_zone(P, V, dF, dt, del[dir], fac);
@@ -954,10 +960,13 @@ void orc_ug_fill_guards(const orc_params* P, const orc_block* b, int nbx, int nb
   }
 }
 
+static orc_block with_geo(const orc_block* b, int i);
+
 double orc_ug_compute_dt(const orc_params* P, const orc_block* b, int nblocks, double* const* blocks) {
   double dt = DBL_MAX;
   for (int i = 0; i < nblocks; i++) {
-    double d = orc_compute_dt(P, b, blocks[i], NULL);
+    const orc_block bi = with_geo(b, i);
+    double d = orc_compute_dt(P, &bi, blocks[i], NULL);
     if (d < dt) dt = d;
   }
   return dt;

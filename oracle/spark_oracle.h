@@ -52,11 +52,12 @@ typedef struct orc_params {
 } orc_params;
 
 /* One block: interior sizes (1 for unused dims) and its deltas.  geo:
- * NULL in Cartesian geometry; otherwise (x = radius) five rows of NX values
- * over the guard-celled x index -- cell-centre radius, area of the cell's
- * low x face, cell volume, left and right edge radius (the Grid arrays
- * hy_xCenter, hy_farea, hy_cvol, hy_xLeft, hy_xRight of Spark/Hydro.ini:
- * 40-56) -- and geometry 1 cylindrical, 2 spherical. */
+ * NULL in Cartesian geometry; otherwise (x = radius), over the guard-celled
+ * box of NX x NY cells: cell-centre, left-edge and right-edge radius (NX
+ * each), then the area of each cell's low x face and each cell's volume
+ * (NX * NY each, x fastest) -- the Grid arrays hy_xCenter, hy_xLeft,
+ * hy_xRight, hy_farea, hy_cvol of Spark/Hydro.ini:40-56 -- with geometry 1
+ * cylindrical, 2 spherical. */
 typedef struct orc_block {
   int nx, ny, nz;
   double dx, dy, dz;

@@ -311,13 +311,19 @@ def test_batched_forms_apply_the_driver_dt_limits(form):
         assert nd == 0
 
 
-@pytest.mark.parametrize("guard_mode", ["telescope", "refill"])
-def test_spherical_geometry_matches_oracle(guard_mode):
-    """1D spherical blast with polar and azimuthal velocity: facM/facP,
-    (rho w^2 + 2 p)/r, -rho u w / r and (rho v)^2 / rho / r, scaled by
-    dx / dx_sph (geoFacs); the plain form equals the oracle bit for bit."""
+@pytest.mark.parametrize("case", ["1d_telescope", "1d_refill", "2d_telescope"])
+def test_spherical_geometry_matches_oracle(case):
+    """Spherical blast with polar and azimuthal velocity, 1D and 2D (r,
+    theta): facM/facP from theta-dependent areas and volumes,
+    (rho w^2 + 2 p)/r, -rho u w / r and (rho v)^2 / rho / r scaled by
+    dx / dx_sph (geoFacs), the r dtheta width in the CFL step; the plain
+    form equals the oracle bit for bit."""
+    guard_mode = "refill" if case.endswith("refill") else "telescope"
     kw = dict(nguard=6, guard_mode="refill") if guard_mode == "refill" else {}
-    prob = problems.spherical_blast1d(nb=4, n=32, **kw)
+    if case.startswith("2d"):
+        prob = problems.spherical_blast(nb=(4, 2), n=16, ndim=2, **kw)
+    else:
+        prob = problems.spherical_blast(nb=(4, 1), n=32, **kw)
     nsteps = 10
     b = capi.Batch(prob.cfg, prob.blocks)
     try:
@@ -332,10 +338,10 @@ def test_spherical_geometry_matches_oracle(guard_mode):
         b.close()
     ref = clone(prob.arrays)
     dref, nerr = pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, ref, nsteps,
-                                     refill=guard_mode == "refill", geo=prob.info["geo"])
+                                     refill=guard_mode == "refill", geo=[b["geo"] for b in prob.blocks])
     assert cnt[0] == 0 and nerr == 0
     assert np.array_equal(d, dref), (d, dref)
-    compare(interior_views(prob, out), interior_views(prob, ref), f"spherical {guard_mode}")
+    compare(interior_views(prob, out), interior_views(prob, ref), f"spherical {case}")
 
 
 @pytest.mark.parametrize("guard_mode", ["telescope", "refill"])
@@ -362,7 +368,7 @@ def test_cylindrical_geometry_matches_oracle(guard_mode):
         b.close()
     ref = clone(prob.arrays)
     dref, nerr = pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, ref, nsteps,
-                                     refill=guard_mode == "refill", geo=geo)
+                                     refill=guard_mode == "refill", geo=[b["geo"] for b in prob.blocks])
     assert cnt[0] == 0 and nerr == 0
     assert np.array_equal(d, dref), (d, dref)
     compare(interior_views(prob, out), interior_views(prob, ref), f"cylindrical {guard_mode}")
@@ -382,7 +388,7 @@ def test_geometry_is_refused_where_it_is_not_implemented():
     try:
         with pytest.raises(capi.SparkError, match="2D"):
             b.set_geometry("cylindrical", rows)
-        with pytest.raises(capi.SparkError, match="1D"):
+        with pytest.raises(capi.SparkError, match="1D \\(r\\) or 2D"):
             b.set_geometry("spherical", rows)
     finally:
         b.close()

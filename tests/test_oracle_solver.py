@@ -338,7 +338,9 @@ def test_cylindrical_rows_are_the_grid_geometry():
     prob = _cyl()
     dx = prob.blocks[0]["dx"]
     for g in prob.info["geo"]:
-        rc, area, vol, rl, rr = g
+        rc, rl, rr = g["rc"], g["rl"], g["rr"]
+        area, vol = g["area"][0], g["vol"][0]
+        assert np.array_equal(g["area"], np.tile(area, (g["area"].shape[0], 1)))
         facm, facp = area[:-1] * dx / vol[:-1], area[1:] * dx / vol[:-1]
         assert np.allclose(facm, rl[:-1] / rc[:-1], rtol=1e-14, atol=0)
         assert np.allclose(facp, rr[:-1] / rc[:-1], rtol=1e-14, atol=0)
@@ -349,10 +351,11 @@ def test_cylindrical_rows_are_the_grid_geometry():
 def test_spherical_rows_are_the_grid_geometry():
     """Spherical: facM = 3 r_l^2 dr / (r_r^3 - r_l^3), facP with r_r, and
     dx_sph = (r_r^3 - r_l^3) / (3 r_c^2) = dr (1 + dr^2 / (12 r_c^2))."""
-    prob = problems.spherical_blast1d(nb=2, n=16)
+    prob = problems.spherical_blast(nb=(2, 1), n=16)
     dx = prob.blocks[0]["dx"]
     for g in prob.info["geo"]:
-        rc, area, vol, rl, rr = g
+        rc, rl, rr = g["rc"], g["rl"], g["rr"]
+        area, vol = g["area"][0], g["vol"][0]
         facm, facp = area[:-1] * dx / vol[:-1], area[1:] * dx / vol[:-1]
         dv = rr ** 3 - rl ** 3
         assert np.allclose(facm, 3.0 * rl[:-1] ** 2 * dx / dv[:-1], rtol=1e-12, atol=0)
@@ -368,7 +371,7 @@ def test_cylindrical_rest_state_is_a_fixed_point():
     ~dt p / (rho r) ~ 1e-3 in a step."""
     prob = _cyl(p_in=1.0, swirl=0.0)
     arrs = [a.copy() for a in prob.arrays]
-    d, nerr = pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, arrs, 3, geo=prob.info["geo"])
+    d, nerr = pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, arrs, 3, geo=[b["geo"] for b in prob.blocks])
     assert nerr == 0 and d[0] > 1e-3
     g = prob.cfg["nguard"]
     vmax = max(float(np.abs(a[0, g:-g, g:-g, VI["velx"]]).max()) for a in arrs)
@@ -391,10 +394,10 @@ def test_cylindrical_uniform_flow_follows_the_axisymmetric_equations():
     arrs = [a.copy() for a in prob.arrays]
     dt = 1e-5
     with pyoracle.dt_control(dt_init=dt):
-        pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, arrs, 1, geo=prob.info["geo"])
+        pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, arrs, 1, geo=[b["geo"] for b in prob.blocks])
     g = prob.cfg["nguard"]
     for a, geo in zip(arrs, prob.info["geo"]):
-        rc = geo[0][g + 2:-g - 2][None, :]
+        rc = geo["rc"][g + 2:-g - 2][None, :]
         c = a[0, g + 2:-g - 2, g + 2:-g - 2]
         rho = c[..., VI["dens"]]
         mu = rho * c[..., VI["velx"]]
@@ -408,9 +411,9 @@ def test_spherical_rest_state_is_a_fixed_point():
     """At rest with uniform p the spherical pressure flux difference
     -3 p (r_r^2 - r_l^2) / (r_r^3 - r_l^3) cancels the source 2 p / r_c
     scaled by dx / dx_sph: the state stays at rest to rounding."""
-    prob = problems.spherical_blast1d(nb=2, n=16, p_in=1.0, vtheta=0.0, swirl=0.0)
+    prob = problems.spherical_blast(nb=(2, 1), n=16, p_in=1.0, vtheta=0.0, swirl=0.0)
     arrs = [a.copy() for a in prob.arrays]
-    d, nerr = pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, arrs, 3, geo=prob.info["geo"])
+    d, nerr = pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, arrs, 3, geo=[b["geo"] for b in prob.blocks])
     assert nerr == 0 and d[0] > 1e-3
     g = prob.cfg["nguard"]
     vmax = max(float(np.abs(a[0, 0, g:-g, VI["velx"]]).max()) for a in arrs)
@@ -424,20 +427,38 @@ def test_spherical_uniform_flow_follows_the_reference_rates():
     d(rho w)/dt = -3 rho u w / r and, as the reference has no polar source,
     d(rho v)/dt = -2 rho u v / r."""
     u, v, w = 0.1, 0.05, 0.2
-    prob = problems.spherical_blast1d(nb=2, n=16, p_in=1.0, vtheta=v, swirl=0.0, vr=u)
+    prob = problems.spherical_blast(nb=(2, 1), n=16, p_in=1.0, vtheta=v, swirl=0.0, vr=u)
     for a in prob.arrays:
         a[..., VI["velz"]] = w
         a[..., VI["ener"]] = a[..., VI["eint"]] + 0.5 * (u * u + v * v + w * w)
     arrs = [a.copy() for a in prob.arrays]
     dt = 1e-5
     with pyoracle.dt_control(dt_init=dt):
-        pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, arrs, 1, geo=prob.info["geo"])
+        pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, arrs, 1, geo=[b["geo"] for b in prob.blocks])
     g = prob.cfg["nguard"]
     for a, geo in zip(arrs, prob.info["geo"]):
-        rc = geo[0][g + 2:-g - 2]
+        rc = geo["rc"][g + 2:-g - 2]
         c = a[0, 0, g + 2:-g - 2]
         rho = c[..., VI["dens"]]
         assert np.allclose((rho - 1.0) / dt, -2.0 * u / rc, rtol=1e-3)
         assert np.allclose((rho * c[..., VI["velx"]] - u) / dt, (v * v + w * w - 2.0 * u * u) / rc, rtol=1e-3)
         assert np.allclose((rho * c[..., VI["velz"]] - w) / dt, -3.0 * u * w / rc, rtol=1e-3)
         assert np.allclose((rho * c[..., VI["vely"]] - v) / dt, -2.0 * u * v / rc, rtol=1e-3)
+
+
+def test_spherical_2d_rest_state_and_cfl_width():
+    """2D spherical (r, theta): at rest the state stays at rest to rounding
+    (theta-dependent areas and volumes cancel in facM/facP), and the CFL
+    step takes r dtheta as the theta width (Hydro_computeDt.F90:108-110):
+    dt = cfl min(dr, r_min dtheta) / c."""
+    prob = problems.spherical_blast(nb=(2, 2), n=16, ndim=2, p_in=1.0, vtheta=0.0, swirl=0.0)
+    arrs = [a.copy() for a in prob.arrays]
+    d, nerr = pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, arrs, 2, geo=[b["geo"] for b in prob.blocks])
+    assert nerr == 0
+    g = prob.cfg["nguard"]
+    dx = prob.blocks[0]["dx"]
+    rmin = min(float(geo["rc"][g]) for geo in prob.info["geo"])
+    c = np.sqrt(1.4)
+    assert np.isclose(d[0], prob.cfg["cfl"] / (c / min(dx, rmin * dx)), rtol=1e-14, atol=0)
+    vmax = max(float(np.abs(a[0, g:-g, g:-g, [VI["velx"], VI["vely"]]]).max()) for a in arrs)
+    assert vmax < 1e-13, vmax
