@@ -441,6 +441,86 @@ __global__ void __launch_bounds__(256) k_shock(DevParams P, const BlockMeta* __r
   }
 }
 
+// shockDetect of 3D blocks in one pass: TW x TH column tiles of the
+// (band + 1) box marched over z, one thread per column, with the pressure
+// and sound speed of three z planes (tile + 1-cell ring) in LDS, so the
+// sound speed is computed once per cell and never stored.  Cells of the
+// band box get the flag, the outer layer 0, as the two-pass kernels above
+// give.  Tiles are as wide as the box up to SHK_T per side.
+constexpr int SHK_T = 32;
+__global__ void __launch_bounds__(1024) k_shock3(DevParams P, const BlockMeta* __restrict__ meta,
+                                                 const int* __restrict__ blist, double* __restrict__ U, int band,
+                                                 int TW, int TH) {
+  extern __shared__ double shk_lds[];
+  const int PW = TW + 2, PS = PW * (TH + 2);
+  double* sp = shk_lds;           // [3][PS]
+  double* sc = shk_lds + 3 * PS;  // [3][PS]
+  const BlockMeta m = meta[blist[blockIdx.y]];
+  const Lims L1 = stage_lims(P, m, band + 1), L0 = stage_lims(P, m, band);
+  const int ex = L1.hi[0] - L1.lo[0] + 1;
+  const int ntx = (ex + TW - 1) / TW;
+  const int x0 = L1.lo[0] + (int)(blockIdx.x % ntx) * TW, y0 = L1.lo[1] + (int)(blockIdx.x / ntx) * TH;
+  const int tid = threadIdx.x;
+  const int i = x0 + tid % TW, j = y0 + tid / TW;
+  const bool own = tid < TW * TH && i <= L1.hi[0] && j <= L1.hi[1];
+  double* up = U + m.base;
+  const int64_t cs = m.cstride;
+  const int64_t sx = 1, sy = m.NX, sz = (int64_t)m.NX * m.NY;
+  const double beta = 0.5, delta = 0.1;
+  // plane k into its slot: cells of the (band + 1) box only (the others
+  // are never read by a flagged cell)
+  auto load = [&](int k) {
+    const int s = (k - L1.lo[2]) % 3;
+    for (int u = tid; u < PS; u += blockDim.x) {
+      const int ii = x0 - 1 + u % PW, jj = y0 - 1 + u / PW;
+      double pv = 0., cv = 0.;
+      if (ii >= L1.lo[0] && ii <= L1.hi[0] && jj >= L1.lo[1] && jj <= L1.hi[1] && k <= L1.hi[2]) {
+        const int64_t c = cix(m, ii, jj, k);
+        pv = up[V_PRES * cs + c];
+        cv = Num<double>::sqr(up[V_GAMC * cs + c] * pv / fmax2(up[V_DENS * cs + c], P.tiny));
+      }
+      sp[s * PS + u] = pv;
+      sc[s * PS + u] = cv;
+    }
+  };
+  load(L1.lo[2]);
+  for (int k = L1.lo[2]; k <= L1.hi[2]; k++) {
+    load(k + 1);
+    __syncthreads();
+    if (own) {
+      const int64_t c = cix(m, i, j, k);
+      double shok = 0.;
+      if (i >= L0.lo[0] && i <= L0.hi[0] && j >= L0.lo[1] && j <= L0.hi[1] && k >= L0.lo[2] && k <= L0.hi[2]) {
+        const int q = (j - y0 + 1) * PW + (i - x0 + 1);
+        double minP = 0., minC = 0.;
+        bool first = true;
+        for (int kk = -1; kk <= 1; kk++) {
+          const int s = (k + kk - L1.lo[2]) % 3;
+          for (int jj = -1; jj <= 1; jj++)
+            for (int ii = -1; ii <= 1; ii++) {
+              const double pv = sp[s * PS + q + jj * PW + ii], cv = sc[s * PS + q + jj * PW + ii];
+              if (first) { minP = pv; minC = cv; first = false; }
+              else { minP = fmin2(minP, pv); minC = fmin2(minC, cv); }
+            }
+        }
+        const double* p = up + V_PRES * cs;
+        double divv = up[V_VELX * cs + c + sx] - up[V_VELX * cs + c - sx];
+        divv = divv + up[V_VELY * cs + c + sy] - up[V_VELY * cs + c - sy];
+        divv = divv + up[V_VELZ * cs + c + sz] - up[V_VELZ * cs + c - sz];
+        divv = 0.5 * divv;
+        const double gradPx = 0.5 * (p[c + sx] - p[c - sx]);
+        const double gradPy = 0.5 * (p[c + sy] - p[c - sy]);
+        const double gradPz = 0.5 * (p[c + sz] - p[c - sz]);
+        const bool SW1 = (__builtin_fabs(gradPx) + __builtin_fabs(gradPy) + __builtin_fabs(gradPz) >= beta * minP);
+        const bool SW2 = (-delta * minC >= divv);
+        if (SW1 && SW2) shok = 1.;
+      }
+      up[V_SHOK * cs + c] = shok;
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------------------
 // CFL dt (Hydro_computeDt.F90:77-148).  Pass 1: per (block, chunk) the first
 // cell in (k,j,i) order with the largest dt_ltemp*cfl.  Pass 2 (one
@@ -852,6 +932,18 @@ hipError_t plain_shock_detect(const PlainLaunch& L, double* U, double* scratch, 
   int b1 = 1, b0 = 1;
   for (int d = 0; d < L.P.ndim; d++) { b1 *= n[d] + 2 * band + 2; b0 *= n[d] + 2 * band; }
   const double nc1 = (double)b1 * L.nblk, nc0 = (double)b0 * L.nblk;
+  if (L.P.ndim == 3) {
+    const int e1 = n[0] + 2 * band + 2, e2 = n[1] + 2 * band + 2;
+    const int tw = e1 < SHK_T ? e1 : SHK_T, th = e2 < SHK_T ? e2 : SHK_T;
+    const int tiles = ((e1 + tw - 1) / tw) * ((e2 + th - 1) / th);
+    const int threads = (tw * th + 63) / 64 * 64;
+    const size_t lds = sizeof(double) * 6 * (tw + 2) * (th + 2);
+    if (L.prof) L.prof->begin(L.stream, "k_shock", 8.0 * nc1 * 7, nc1 * 6 + nc0 * 70);
+    hipLaunchKernelGGL(k_shock3, dim3(tiles, L.nblk), dim3(threads), lds, L.stream, L.P, L.meta, L.blist, U, band,
+                       tw, th);
+    if (L.prof) L.prof->end(L.stream);
+    return hipGetLastError();
+  }
   if (L.prof) L.prof->begin(L.stream, "k_shock_vc", 8.0 * nc1 * 5, nc1 * 6);
   hipLaunchKernelGGL(k_shock_vc, dim3(grid_x_for(b1, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta,
                      L.blist, U, scratch, band);
