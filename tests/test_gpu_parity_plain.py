@@ -125,6 +125,16 @@ def test_hybrid_riemann_and_no_flattening():
     run_case(prob, 5, "sod2d hybrid, no flattening")
 
 
+def test_hybrid_riemann_3d_mixed_sizes():
+    """3D hybrid HLLC/HLL on blocks of 8^3, 16^3 and 32^3: the shock flags of
+    the guard band pick the Riemann solver of faces there, and the 32^3 boxes
+    (50 cells wide with the band) span several shockDetect column tiles."""
+    prob = problems.batched(nblocks=12, nguard=10, sizes=(8, 16, 32), probs=(0.25, 0.5, 0.25), shock_every=3,
+                            use_hybrid_riemann=1)
+    assert 32 in [b["nx"] for b in prob.blocks]
+    run_case(prob, 3, "batched hybrid 3d mixed sizes")
+
+
 def test_scalars_and_gravity():
     prob = problems.batched(nblocks=8, nguard=10, sizes=(8, 16), probs=(0.5, 0.5), shock_every=2, nscalars=8,
                             use_gravity=1, grav=(0.0, 0.0, -1.0))
