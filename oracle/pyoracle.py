@@ -42,49 +42,67 @@ _lib = None
 _D = ctypes.POINTER(ctypes.c_double)
 
 
+def load(path):
+    """Open a build of the restatement at `path` with every signature set."""
+    path = Path(path)
+    if not path.exists():
+        raise RuntimeError(f"{path} missing: run `make -C oracle`")
+    L = ctypes.CDLL(str(path))
+    P = ctypes.POINTER(OrcParams)
+    B = ctypes.POINTER(OrcBlock)
+    pp = ctypes.POINTER(ctypes.c_void_p)
+    L.orc_prim2con.argtypes = [_D, _D]
+    L.orc_prim2flx.argtypes = [ctypes.c_int, _D, _D]
+    L.orc_riemann_hllc.argtypes = [P, ctypes.c_int, _D, _D, ctypes.c_int, _D]
+    L.orc_riemann_hllc.restype = ctypes.c_int
+    L.orc_riemann_hll.argtypes = [P, ctypes.c_int, _D, _D, _D]
+    L.orc_riemann_hll.restype = ctypes.c_int
+    L.orc_recon.argtypes = [P, ctypes.c_int, _D, ctypes.c_int, ctypes.c_int, ctypes.c_double, _D, _D]
+    L.orc_eos_dens_ei_cell.argtypes = [P, _D, ctypes.c_int]
+    L.orc_flattening.argtypes = [P, B, _D, ctypes.POINTER(ctypes.c_int), _D]
+    L.orc_get_face_flux.argtypes = [P, B, _D, _D, ctypes.POINTER(ctypes.c_int), _D, _D, _D,
+                                    ctypes.POINTER(ctypes.c_int)]
+    L.orc_update_soln.argtypes = [P, B, _D, _D, ctypes.POINTER(ctypes.c_int), ctypes.c_double, _D, _D, _D, _D]
+    L.orc_eos_wrapped.argtypes = [P, B, _D, ctypes.POINTER(ctypes.c_int)]
+    L.orc_shock_detect.argtypes = [P, B, _D]
+    L.orc_compute_dt.argtypes = [P, B, _D, ctypes.POINTER(ctypes.c_int)]
+    L.orc_compute_dt.restype = ctypes.c_double
+    L.orc_stage.argtypes = [P, B, _D, _D, ctypes.c_int, ctypes.c_int, ctypes.c_double, _D, _D, _D,
+                            ctypes.POINTER(ctypes.c_int)]
+    L.orc_hydro_block.argtypes = [P, B, _D, ctypes.c_double, _D, _D, _D]
+    L.orc_hydro_block.restype = ctypes.c_int
+    L.orc_correct_fluxes.argtypes = [P, B, _D, ctypes.c_double, _D, _D, _D]
+    L.orc_ug_fill_guards.argtypes = [P, B, ctypes.c_int, ctypes.c_int, ctypes.c_int, pp, ctypes.c_int]
+    L.orc_ug_advance.argtypes = [P, B, ctypes.c_int, ctypes.c_int, ctypes.c_int, pp, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_int, _D]
+    L.orc_ug_advance.restype = ctypes.c_int
+    L.orc_batch_advance.argtypes = [P, B, ctypes.c_int, pp, ctypes.c_int, ctypes.c_int, _D]
+    L.orc_batch_advance.restype = ctypes.c_int
+    L.orc_set_dt_control.argtypes = [ctypes.c_double] * 4
+    L.orc_set_dt_control.restype = None
+    L.orc_set_geometry.argtypes = [pp, ctypes.c_int]
+    L.orc_set_geometry.restype = None
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
-        if not LIB_PATH.exists():
-            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
-        L = ctypes.CDLL(str(LIB_PATH))
-        vp = ctypes.c_void_p
-        P = ctypes.POINTER(OrcParams)
-        B = ctypes.POINTER(OrcBlock)
-        pp = ctypes.POINTER(ctypes.c_void_p)
-        L.orc_prim2con.argtypes = [_D, _D]
-        L.orc_prim2flx.argtypes = [ctypes.c_int, _D, _D]
-        L.orc_riemann_hllc.argtypes = [P, ctypes.c_int, _D, _D, ctypes.c_int, _D]
-        L.orc_riemann_hllc.restype = ctypes.c_int
-        L.orc_riemann_hll.argtypes = [P, ctypes.c_int, _D, _D, _D]
-        L.orc_riemann_hll.restype = ctypes.c_int
-        L.orc_recon.argtypes = [P, ctypes.c_int, _D, ctypes.c_int, ctypes.c_int, ctypes.c_double, _D, _D]
-        L.orc_eos_dens_ei_cell.argtypes = [P, _D, ctypes.c_int]
-        L.orc_flattening.argtypes = [P, B, _D, ctypes.POINTER(ctypes.c_int), _D]
-        L.orc_get_face_flux.argtypes = [P, B, _D, _D, ctypes.POINTER(ctypes.c_int), _D, _D, _D,
-                                        ctypes.POINTER(ctypes.c_int)]
-        L.orc_update_soln.argtypes = [P, B, _D, _D, ctypes.POINTER(ctypes.c_int), ctypes.c_double, _D, _D, _D, _D]
-        L.orc_eos_wrapped.argtypes = [P, B, _D, ctypes.POINTER(ctypes.c_int)]
-        L.orc_shock_detect.argtypes = [P, B, _D]
-        L.orc_compute_dt.argtypes = [P, B, _D, ctypes.POINTER(ctypes.c_int)]
-        L.orc_compute_dt.restype = ctypes.c_double
-        L.orc_stage.argtypes = [P, B, _D, _D, ctypes.c_int, ctypes.c_int, ctypes.c_double, _D, _D, _D,
-                                ctypes.POINTER(ctypes.c_int)]
-        L.orc_hydro_block.argtypes = [P, B, _D, ctypes.c_double, _D, _D, _D]
-        L.orc_hydro_block.restype = ctypes.c_int
-        L.orc_correct_fluxes.argtypes = [P, B, _D, ctypes.c_double, _D, _D, _D]
-        L.orc_ug_fill_guards.argtypes = [P, B, ctypes.c_int, ctypes.c_int, ctypes.c_int, pp, ctypes.c_int]
-        L.orc_ug_advance.argtypes = [P, B, ctypes.c_int, ctypes.c_int, ctypes.c_int, pp, ctypes.c_int, ctypes.c_int,
-                                     ctypes.c_int, _D]
-        L.orc_ug_advance.restype = ctypes.c_int
-        L.orc_batch_advance.argtypes = [P, B, ctypes.c_int, pp, ctypes.c_int, ctypes.c_int, _D]
-        L.orc_batch_advance.restype = ctypes.c_int
-        L.orc_set_dt_control.argtypes = [ctypes.c_double] * 4
-        L.orc_set_dt_control.restype = None
-        L.orc_set_geometry.argtypes = [pp, ctypes.c_int]
-        L.orc_set_geometry.restype = None
-        _lib = L
+        _lib = load(LIB_PATH)
     return _lib
+
+
+@contextlib.contextmanager
+def using(path):
+    """Route every call of this module to another build of the restatement
+    (the planted-mistake builds of the pin tests), then restore."""
+    global _lib
+    saved = _lib
+    _lib = load(path)
+    try:
+        yield _lib
+    finally:
+        _lib = saved
 
 
 def params(cfg: dict) -> OrcParams:

@@ -21,8 +21,14 @@
  * Pinning status of each part (see tests/test_oracle_*.py):
  *   prim2con/prim2flx, HLLC, HLL, WENO5-Z, MP5, PLM, FOG, Gamma EOS, update,
  *   dt, flux correction: pinned by closed forms, invariants and hand cases.
- *   flattening, shockDetect, artificial viscosity: pinned by closed forms on
- *   special inputs (smooth/constant data) and by brute-force restatements.
+ *   flattening, shockDetect: hand-computed cases on special inputs.
+ *   artificial viscosity: the face flux of a supersonic first-order pencil
+ *   equals F(V_{i-1}) + cvisc max(-du, 0) (U_{i-1} - U_i) (nothing added
+ *   on expansion).  mass-scalar fluxes: rho X follows the upwind RK Taylor
+ *   polynomial for u > 0 and u < 0.  gravity source: on a uniform state one
+ *   RK3 step gives d(rho u) = rho g dt and dE = m0.g dt + rho |g|^2 dt^2 / 2.
+ *   Planted mistakes in these three parts (tests/test_oracle_mutants.py)
+ *   each fail one of those pins.
  *   Whole-solver trajectories: pinned by the Sod exact solution (Toro ch.4)
  *   and the Sedov analytic profile held by the reference
  *   (Simulation/SimulationMain/Sedov/sedovSolution.dat).
