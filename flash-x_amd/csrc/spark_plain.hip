@@ -531,8 +531,8 @@ struct DtPart { double key, lt; long long idx; };
 __global__ void __launch_bounds__(256) k_dt_part(DevParams P, const BlockMeta* __restrict__ meta,
                                                  const int* __restrict__ blist, const double* __restrict__ U,
                                                  DtPart* __restrict__ out, int chunks) {
-  __shared__ double skey[256], slt[256];
-  __shared__ long long sidx[256];
+  __shared__ double skey[4], slt[4];
+  __shared__ long long sidx[4];
   const int b = blockIdx.y;
   const BlockMeta m = meta[blist[b]];
   Lims L = stage_lims(P, m, 0);
@@ -557,19 +557,22 @@ __global__ void __launch_bounds__(256) k_dt_part(DevParams P, const BlockMeta* _
     const double key = lt * P.cfl;
     if (key > bk || (key == bk && (long long)c < bi)) { bk = key; bl = lt; bi = c; }
   }
-  skey[threadIdx.x] = bk; slt[threadIdx.x] = bl; sidx[threadIdx.x] = bi;
-  __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      const int o = threadIdx.x + w;
-      if (skey[o] > skey[threadIdx.x] || (skey[o] == skey[threadIdx.x] && sidx[o] < sidx[threadIdx.x])) {
-        skey[threadIdx.x] = skey[o]; slt[threadIdx.x] = slt[o]; sidx[threadIdx.x] = sidx[o];
-      }
-    }
-    __syncthreads();
+  // the combine is a total order (larger key, then smaller cell index), so
+  // any reduction tree gives the result of the reference's first-cell scan:
+  // a butterfly over the 64 lanes of each wavefront, then the 4 wavefront
+  // results through LDS
+  for (int off = 32; off > 0; off >>= 1) {
+    const double k2 = __shfl_xor(bk, off, 64), l2 = __shfl_xor(bl, off, 64);
+    const long long i2 = __shfl_xor(bi, off, 64);
+    if (k2 > bk || (k2 == bk && i2 < bi)) { bk = k2; bl = l2; bi = i2; }
   }
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { skey[wv] = bk; slt[wv] = bl; sidx[wv] = bi; }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    DtPart r; r.key = skey[0]; r.lt = slt[0]; r.idx = sidx[0];
+    for (int q = 1; q < (int)(blockDim.x >> 6); q++)
+      if (skey[q] > bk || (skey[q] == bk && sidx[q] < bi)) { bk = skey[q]; bl = slt[q]; bi = sidx[q]; }
+    DtPart r; r.key = bk; r.lt = bl; r.idx = bi;
     out[(size_t)blist[b] * chunks + blockIdx.x] = r;
   }
 }
@@ -578,9 +581,9 @@ __global__ void __launch_bounds__(256) k_dt_final(DevParams P, const DtPart* __r
                                                   int chunks, double* __restrict__ dt_out,
                                                   long long* __restrict__ loc_out, double* __restrict__ dt_hist,
                                                   int* __restrict__ hist_slot, DtCtl* __restrict__ ctl) {
-  __shared__ double sdt[256];
-  __shared__ int sb[256];
-  __shared__ long long sc[256];
+  __shared__ double sdt[4];
+  __shared__ int sb[4];
+  __shared__ long long sc[4];
   double best = 1.7976931348623157e308;
   int bb = 0x7fffffff;
   long long bc = 0;
@@ -594,19 +597,22 @@ __global__ void __launch_bounds__(256) k_dt_final(DevParams P, const DtPart* __r
     const double dtb = (bl != 0.0) ? P.cfl / bl : 1.7976931348623157e308;
     if (dtb < best || (dtb == best && b < bb)) { best = dtb; bb = b; bc = bi; }
   }
-  sdt[threadIdx.x] = best; sb[threadIdx.x] = bb; sc[threadIdx.x] = bc;
-  __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      const int o = threadIdx.x + w;
-      if (sdt[o] < sdt[threadIdx.x] || (sdt[o] == sdt[threadIdx.x] && sb[o] < sb[threadIdx.x])) {
-        sdt[threadIdx.x] = sdt[o]; sb[threadIdx.x] = sb[o]; sc[threadIdx.x] = sc[o];
-      }
-    }
-    __syncthreads();
+  // smaller dt, then the lower block index (Driver_computeDt's strict '<'
+  // over blocks in order): a total order, reduced per wavefront by a
+  // butterfly and then over the 4 wavefronts
+  for (int off = 32; off > 0; off >>= 1) {
+    const double d2 = __shfl_xor(best, off, 64);
+    const int b2 = __shfl_xor(bb, off, 64);
+    const long long c2 = __shfl_xor(bc, off, 64);
+    if (d2 < best || (d2 == best && b2 < bb)) { best = d2; bb = b2; bc = c2; }
   }
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { sdt[wv] = best; sb[wv] = bb; sc[wv] = bc; }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    double dt = sdt[0];
+    for (int q = 1; q < (int)(blockDim.x >> 6); q++)
+      if (sdt[q] < best || (sdt[q] == best && sb[q] < bb)) { best = sdt[q]; bb = sb[q]; bc = sc[q]; }
+    double dt = best;
     if (ctl) {
       // Driver_evolveAll's first dt, then Driver_computeDt.F90:450-460:
       // dtNew = min(dtNew, dtOld * factor); dtNew = min(max(dtNew, dtMin), dtMax)
@@ -628,9 +634,9 @@ __global__ void __launch_bounds__(256) k_dt_final(DevParams P, const DtPart* __r
       dt_hist[h] = dt;
       *hist_slot = h + 1;
     }
-    if (sb[0] < nblocks) {
-      loc_out[0] = sb[0];
-      loc_out[1] = sc[0];
+    if (bb < nblocks) {
+      loc_out[0] = bb;
+      loc_out[1] = bc;
     }
   }
 }

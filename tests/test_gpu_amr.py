@@ -167,6 +167,65 @@ def same_level_blocks(prob):
     return full, some
 
 
+def rt_three_level_16():
+    """The shape of the 3D Rayleigh-Taylor workload at an eighth of its size:
+    16^3 blocks on three levels of leaves (8 / 32 / 256 on a 2 x 2 x 4
+    level-1 grid, z in [0, 2], level 2 where z meets (0.5, 1.5), level 3
+    where it meets (0.75, 1.25)), so the hierarchy has two level jumps;
+    8 mass scalars, gravity g = -1 along z, periodic x/y, reflecting z,
+    fine/coarse flux correction every step."""
+    return problems.rt_amr(nb0=(2, 2, 4), n=16, lmax=3, slabs=((0.5, 1.5), (0.75, 1.25)), nscalars=8)
+
+
+@pytest.fixture(scope="module")
+def rt16_oracle():
+    prob = rt_three_level_16()
+    nsteps = 3
+    ref = [a.copy() for a in prob.arrays]
+    dref, nerr = pyoracle.amr_advance(prob.cfg, prob.blocks, prob.info["nb0"], prob.info["origins"], ref, nsteps,
+                                      flux_correct=True, bc=prob.info["bc"])
+    return prob, nsteps, dref, nerr, ref
+
+
+@pytest.mark.parametrize("form", ["plain_fp64", "batched_fp64"])
+def test_three_level_16cubed_steps_match_oracle(form, rt16_oracle):
+    """3 steps over the three-level 16^3 hierarchy against the oracle's
+    hierarchy driver (guard fill over levels, dt, Hydro per leaf, register
+    exchange, correction): dt sequence and every interior value equal."""
+    prob, nsteps, dref, nerr, ref = rt16_oracle
+    assert prob.cfg["flux_correct"] == 1 and prob.cfg["use_gravity"] == 1
+    assert list(prob.info["leaves_per_level"]) == [8, 32, 256]
+    assert prob.info["bc"] == (problems.BC_PERIODIC, problems.BC_PERIODIC, problems.BC_REFLECT)
+    print(f"RT 3 levels {prob.info['leaves_per_level']}, 16^3 blocks, {form}, {nsteps} steps", flush=True)
+    d, out, cnt = run(prob, nsteps, form)
+    assert nerr == 0 and cnt[0] == 0
+    assert np.array_equal(d, dref), (d, dref)
+    nd = sum(int(np.count_nonzero(x != y)) for x, y in zip(interiors(prob, out), interiors(prob, ref)))
+    print(f"differing interior values {nd}", flush=True)
+    assert nd == 0
+
+
+def test_three_level_16cubed_fp32_flux_vs_oracle(rt16_oracle):
+    """The FP32-flux form over the same 3 steps: L1(rho) and L1(E) within
+    1e-5 of the oracle (L1(rho u) is printed: the flow starts at rest, so
+    the relative momentum error has a near-zero denominator)."""
+    prob, nsteps, dref, _, ref = rt16_oracle
+    d32, o32, cnt = run(prob, nsteps, "fp32_flux")
+    assert cnt[0] == 0
+    vols = [b["dx"] ** 3 for b in prob.blocks]
+    gi, ri = interiors(prob, o32), interiors(prob, ref)
+    V = capi.VI
+    fields = {"rho": lambda a: a[..., V["dens"]], "rhou": lambda a: a[..., V["dens"]] * a[..., V["velx"]],
+              "E": lambda a: a[..., V["dens"]] * a[..., V["ener"]]}
+    for name, f in fields.items():
+        num = sum(np.sum(np.abs(f(x) - f(y))) * v for x, y, v in zip(gi, ri, vols))
+        den = sum(np.sum(np.abs(f(y))) * v for y, v in zip(ri, vols))
+        print(f"L1({name}) {num / den:.2e}", flush=True)
+        if name != "rhou":
+            assert num / den <= 1e-5, name
+    assert np.max(np.abs(d32 - dref) / dref) <= 1e-5
+
+
 @pytest.mark.parametrize("case", ["gamma", "hybrid", "species"])
 def test_batched_fp64_amr_same_level_halos_bit_exact_with_plain(case):
     prob = three_level(nscalars=8, **{"gamma": {}, "hybrid": SHOCKED, "species": SPECIES}[case])

@@ -211,19 +211,36 @@ def test_fp32_flux_scalars_and_gravity_vs_oracle():
         assert np.mean(np.abs(x32 - xr)) / np.mean(np.abs(xr)) < 1e-5
 
 
-def test_fp32_flux_refill_uniform_grid_vs_plain():
+@pytest.mark.parametrize("flat", [1, 0])
+def test_fp32_flux_refill_uniform_grid_vs_plain(flat):
     """Sedov blast on a uniform block grid with device guard refills.
-    Flattening is off here: its compression test u(i+1) < u(i) is a
-    discontinuous switch that, in the quiescent gas ahead of the blast,
-    FP64 trips on rounding-level (~1e-20) velocities where FP32 sees exact
-    zeros, so with it on the two precisions differ by ~1% in a few front
-    cells (a property of the reference algorithm)."""
-    prob = problems.sedov3d(nb=4, n=16, nguard=6, guard_mode="refill", use_flattening=0)
+    Without flattening the FP32-flux form is held to L1(rho), L1(E) <= 1e-5
+    of the plain FP64 form.  With flattening on (the reference default) the
+    compression test u(i+1) < u(i) ahead of the blast compares velocities
+    that are rounding noise, so the FP64 form itself moves by L1(rho) ~ 3e-5
+    when its velocities are perturbed by 1e-22 (the oracle shows the same,
+    tests/test_oracle_solver.py); there the FP32-flux form is held to that
+    FP64 self-sensitivity: no further from FP64 than twice what a 1e-22
+    perturbation does to FP64."""
+    prob = problems.sedov3d(nb=4, n=16, nguard=6, guard_mode="refill", use_flattening=flat)
     d64, out64, _, _ = run_gpu(prob, 8, "plain_fp64")
     d32, out32, tiles, _ = run_gpu(prob, 8, "fp32_flux")
     nm = norms(prob, out32, out64)
-    print(f"tiles {tiles}; " + ", ".join(f"{k}: L1 {v[0]:.2e} Linf {v[1]:.2e}" for k, v in nm.items()), flush=True)
-    assert nm["rho"][0] <= 1e-5 and nm["E"][0] <= 1e-5
+    print(f"flattening {flat}, tiles {tiles}; " + ", ".join(f"{k}: L1 {v[0]:.2e} Linf {v[1]:.2e}"
+                                                            for k, v in nm.items()), flush=True)
+    bar = 1e-5
+    if flat:
+        rng = np.random.default_rng(1)
+        pert = problems.sedov3d(nb=4, n=16, nguard=6, guard_mode="refill", use_flattening=flat)
+        for a in pert.arrays:
+            a[..., capi.VI["velx"]] += 1e-22 * rng.standard_normal(a.shape[:-1])
+        _, outp, _, _ = run_gpu(pert, 8, "plain_fp64")
+        nself = norms(prob, outp, out64)
+        print("FP64 moved by 1e-22 velocity noise: " + ", ".join(f"{k}: L1 {v[0]:.2e}" for k, v in nself.items()),
+              flush=True)
+        assert nself["rho"][0] > 1e-6  # the switch does amplify the noise
+        bar = max(bar, 2.0 * nself["rho"][0], 2.0 * nself["E"][0])
+    assert nm["rho"][0] <= bar and nm["E"][0] <= bar
 
 
 def test_fp32_flux_conserves_mass():

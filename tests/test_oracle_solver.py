@@ -123,6 +123,33 @@ def test_flattening_special_cases():
     assert np.all(f[g - 1:-g + 1, g - 1:-g + 1, g - 1:-g + 1] == 1.0)
 
 
+def test_flattening_switch_amplifies_rounding_noise_at_a_blast_front():
+    """Ahead of a Sedov blast the compression test u(i+1) < u(i)
+    (hy_rk_getFaceFlux.F90-mc:234) compares velocities that are rounding
+    noise, in cells where beta ~ 1 and Z > 0.75, so flat jumps between 0 and
+    1 on the noise.  Two FP64 runs whose velocities differ by 1e-22 (far
+    below FP32 rounding) then differ by L1(rho) > 1e-5 after 8 steps with
+    flattening on, and by rounding only with it off.  An FP32-flux form can
+    therefore not be held to 1e-5 of FP64 on this problem with flattening
+    on; the GPU test measures it against this FP64 self-sensitivity."""
+    rng = np.random.default_rng(1)
+    l1 = {}
+    for flat in (1, 0):
+        res = []
+        for noise in (0.0, 1e-22):
+            prob = problems.sedov3d(nb=2, n=16, nguard=6, guard_mode="refill", use_flattening=flat)
+            arr = [a.copy() for a in prob.arrays]
+            for a in arr:
+                a[..., VI["velx"]] += noise * rng.standard_normal(a.shape[:-1])
+            pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, arr, 8, refill=True)
+            g = prob.cfg["nguard"]
+            res.append(np.concatenate([a[g:-g, g:-g, g:-g, VI["dens"]].ravel() for a in arr]))
+        l1[flat] = float(np.sum(np.abs(res[1] - res[0])) / np.sum(np.abs(res[0])))
+    print(f"L1(rho) after 1e-22 velocity noise: {l1[1]:.2e} with flattening, {l1[0]:.2e} without")
+    assert l1[1] > 1e-5
+    assert l1[0] < 1e-15
+
+
 def test_shock_detector_special_cases():
     cfg, b, shape = block3()
     k, j, i = np.meshgrid(*(np.arange(s) for s in shape), indexing="ij")
