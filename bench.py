@@ -120,6 +120,89 @@ def calibrate_copy(mib: int):
     print(json.dumps({"calibrate_copy_bytes": n * 8, "launches": 3}), flush=True)
 
 
+def make_problem(workload):
+    from sparkhip import problems
+    if workload == "batched4096":
+        return problems.batched(nblocks=4096, nguard=10)
+    if workload == "rt_amr":
+        return problems.rt_amr()
+    if workload == "species2d":
+        return problems.sod2d_species(nbx=16, nby=16, n=16)
+    return problems.sedov3d(nb=8, n=16, nguard=6, guard_mode="refill", cfl=SEDOV_CFL)
+
+
+def cpu_baseline(workload, target_s: float):
+    """The oracle (CPU restatement, OpenMP over blocks) on a bounded sample of
+    the same workload, about target_s seconds of wall time."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle
+    from sparkhip import problems
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    if workload == "species2d":
+        prob = problems.sod2d_species(nbx=16, nby=16, n=16)
+        run = lambda n: pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, prob.arrays, n, nthreads=threads)
+        what = "the same two-gas shock tube (256^2 zones)"
+    elif workload == "batched4096":
+        prob = problems.batched(nblocks=256, nguard=10)
+        run = lambda n: pyoracle.batch_advance(prob.cfg, prob.blocks, prob.arrays, n, nthreads=threads)
+        what = f"the first 256 of the 4096 mixed blocks ({prob.zones} zones)"
+    elif workload == "rt_amr":
+        prob = problems.rt_amr(nb0=(1, 1, 2))
+        run = lambda n: pyoracle.amr_advance(prob.cfg, prob.blocks, prob.info["nb0"], prob.info["origins"],
+                                             prob.arrays, n, nthreads=threads, bc=prob.info["bc"])
+        what = (f"the same Rayleigh-Taylor setup on a 1 x 1 x 2 level-1 grid ({len(prob.blocks)} leaves, "
+                f"{prob.zones} zones, guard fill and flux correction included)")
+    else:
+        prob = problems.sedov3d(nb=8, n=16, nguard=6, guard_mode="refill", cfl=SEDOV_CFL)
+        run = lambda n: pyoracle.ug_advance(prob.cfg, prob.blocks[0], prob.ug, prob.arrays, n, refill=True,
+                                            nthreads=threads)
+        what = "3D Sedov 128^3 (8^3 blocks of 16^3, 6 guards, refill)"
+    t0 = time.perf_counter()
+    run(1)
+    one = time.perf_counter() - t0
+    nsteps = int(max(1, min(100, (target_s - one) / max(one, 1e-3))))
+    t0 = time.perf_counter()
+    run(nsteps)
+    el = time.perf_counter() - t0
+    return {
+        "value": prob.zones * nsteps / el,
+        "unit": "zone-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{what}, {nsteps} RK3 steps after 1 untimed, {el:.1f} s wall, OpenMP over blocks",
+    }
+
+
+
+def norms(prob, got, ref):
+    """Volume-weighted relative L1 and relative L-infinity of rho, rho*u, E."""
+    import numpy as np
+    from sparkhip import capi
+    g = prob.cfg["nguard"]
+    V = capi.VI
+    fields = {
+        "rho": lambda a: a[..., V["dens"]],
+        "rhou": lambda a: a[..., V["dens"]] * a[..., V["velx"]],
+        "E": lambda a: a[..., V["dens"]] * a[..., V["ener"]],
+    }
+    out = {}
+    for name, f in fields.items():
+        num = den = linf = scale = 0.0
+        for b, x, r in zip(prob.blocks, got, ref):
+            vol = b["dx"] * b["dy"] * b["dz"]
+            fx, fr = f(x[g:-g, g:-g, g:-g]), f(r[g:-g, g:-g, g:-g])
+            num += float(np.sum(np.abs(fx - fr))) * vol
+            den += float(np.sum(np.abs(fr))) * vol
+            linf = max(linf, float(np.max(np.abs(fx - fr))))
+            scale = max(scale, float(np.max(np.abs(fr))))
+        # relative norms; absolute ones where the reference field is zero
+        out[name] = {"L1": num / den if den > 0 else num, "Linf": linf / scale if scale > 0 else linf,
+                     "relative": bool(den > 0 and scale > 0)}
+    return out
+
+
+
 # name fragment of the marker dispatch (torch's fill kernel) that separates
 # the untimed warmup / tuning dispatches from the profiled steps
 MARKER = "FillFunctor"
