@@ -308,7 +308,10 @@ def amr_fill(cfg, blocks, nb0, origins, arrays, depth, bc=(0, 0, 0)):
     P = params(cfg)
     A, keep = _amr(blocks[0]["nx"], nb0, [b["level"] for b in blocks], origins, bc)
     nunk = 11 + cfg["nscalars"]
-    lib().orc_amr_fill_guards(ctypes.byref(P), ctypes.byref(A), len(blocks), ptr_array(arrays), nunk, int(depth))
+    nbad = lib().orc_amr_fill_guards(ctypes.byref(P), ctypes.byref(A), len(blocks), ptr_array(arrays), nunk,
+                                     int(depth))
+    if nbad:
+        raise ValueError(f"hierarchy refused: {nbad} blocks have guard cells over a leaf two or more levels finer")
 
 
 def flux_exchange(cfg, blocks, nb0, origins, fb, fc, bc=(0, 0, 0)):
@@ -325,4 +328,6 @@ def amr_advance(cfg, blocks, nb0, origins, arrays, nsteps, flux_correct=True, nt
     dts = np.zeros(max(1, nsteps))
     nerr = lib().orc_amr_advance(ctypes.byref(P), ctypes.byref(A), bl, len(blocks), ptr_array(arrays), nsteps,
                                  int(bool(flux_correct)), nthreads, dptr(dts))
+    if nerr < 0:
+        raise ValueError("hierarchy refused: a guard layer lies over a leaf two or more levels finer")
     return dts[:nsteps], nerr

@@ -1175,13 +1175,19 @@ static inline double flipped(double x, int flip, int v) {
 
 /* variable v at level-L cell gi (inside the domain) from the leaf covering
  * it: the containing cell of a same-level or coarser leaf, the mean of the
- * 8 cells of a finer one */
+ * 8 cells of a leaf one level finer.  A leaf two or more levels finer has
+ * no such cell set (the hierarchy is not nested deeply enough for the fill
+ * depth): *bad is set and nothing is read. */
 static double amr_value(const orc_amr* A, const amr_map* M, double* const* blocks, int nvar, int g, int NX, int L,
-                        const int gi[3], int v) {
+                        const int gi[3], int v, int* bad) {
   const int s = amr_leaf_at(A, M, L, gi);
   const int Ls = A->level[s];
   const int* os = A->origin + 3 * s;
   const double* src = blocks[s];
+  if (Ls > L + 1) {
+    *bad = 1;
+    return 0.0;
+  }
   if (Ls <= L) {
     int sl[3];
     for (int d = 0; d < 3; d++) sl[d] = (gi[d] >> (L - Ls)) - os[d] + g;
@@ -1204,13 +1210,15 @@ static double mc_slope(double a, double b) {
   return a > 0.0 ? m : -m;
 }
 
-void orc_amr_fill_guards(const orc_params* P, const orc_amr* A, int nblocks, double* const* blocks, int nvar,
-                         int depth) {
+int orc_amr_fill_guards(const orc_params* P, const orc_amr* A, int nblocks, double* const* blocks, int nvar,
+                        int depth) {
   amr_map M;
   amr_build_map(A, nblocks, &M);
   const int n = A->n, g = P->nguard, NX = n + 2 * g;
-#pragma omp parallel for schedule(dynamic)
+  int nbad = 0;
+#pragma omp parallel for schedule(dynamic) reduction(+ : nbad)
   for (int b = 0; b < nblocks; b++) {
+    int bad = 0;
     const int L = A->level[b];
     const int* o = A->origin + 3 * b;
     double* dst = blocks[b];
@@ -1228,7 +1236,8 @@ void orc_amr_fill_guards(const orc_params* P, const orc_amr* A, int nblocks, dou
           const int s = amr_leaf_at(A, &M, L, gi);
           double* out = dst + (size_t)nvar * cidx(NX, NX, i, j, k);
           if (A->level[s] != L - 1) {
-            for (int v = 0; v < nvar; v++) out[v] = flipped(amr_value(A, &M, blocks, nvar, g, NX, L, gi, v), flip, v);
+            for (int v = 0; v < nvar; v++)
+              out[v] = flipped(amr_value(A, &M, blocks, nvar, g, NX, L, gi, v, &bad), flip, v);
             continue;
           }
           /* a coarser leaf: limited linear prolongation from the parent
@@ -1236,23 +1245,25 @@ void orc_amr_fill_guards(const orc_params* P, const orc_amr* A, int nblocks, dou
           int gc[3];
           for (int d = 0; d < 3; d++) gc[d] = gi[d] >> 1;
           for (int v = 0; v < nvar; v++) {
-            const double c0 = amr_value(A, &M, blocks, nvar, g, NX, L - 1, gc, v);
+            const double c0 = amr_value(A, &M, blocks, nvar, g, NX, L - 1, gc, v, &bad);
             double val = c0;
             for (int d = 0; d < 3; d++) {
               int gm[3] = {gc[0], gc[1], gc[2]}, gp[3] = {gc[0], gc[1], gc[2]};
               gm[d] -= 1;
               gp[d] += 1;
               const int fm = amr_domain(A, L - 1, gm), fp = amr_domain(A, L - 1, gp);
-              const double cm = flipped(amr_value(A, &M, blocks, nvar, g, NX, L - 1, gm, v), fm, v);
-              const double cp = flipped(amr_value(A, &M, blocks, nvar, g, NX, L - 1, gp, v), fp, v);
+              const double cm = flipped(amr_value(A, &M, blocks, nvar, g, NX, L - 1, gm, v, &bad), fm, v);
+              const double cp = flipped(amr_value(A, &M, blocks, nvar, g, NX, L - 1, gp, v, &bad), fp, v);
               const double sl = mc_slope(c0 - cm, cp - c0);
               val = val + sl * ((gi[d] & 1) ? 0.25 : -0.25);
             }
             out[v] = flipped(val, flip, v);
           }
         }
+    nbad += bad;
   }
   free(M.leaf);
+  return nbad;
 }
 
 void orc_flux_exchange(const orc_params* P, const orc_amr* A, int nblocks, double* const* fb, double* const* fc) {
@@ -1335,8 +1346,11 @@ int orc_amr_advance(const orc_params* P, const orc_amr* A, const orc_block* bl, 
     }
   }
   int nerr = 0;
-  for (int step = 0; step < nsteps; step++) {
-    orc_amr_fill_guards(P, A, nblocks, blocks, nunk, depth);
+  for (int step = 0; step < nsteps && nerr >= 0; step++) {
+    if (orc_amr_fill_guards(P, A, nblocks, blocks, nunk, depth)) {
+      nerr = -1;  /* a guard layer lies over a leaf two levels finer */
+      break;
+    }
     double dt = DBL_MAX;
     for (int i = 0; i < nblocks; i++) {
       double d = orc_compute_dt(P, &bl[i], blocks[i], NULL);

@@ -156,9 +156,12 @@ enum { ORC_BC_OUTFLOW = 0, ORC_BC_PERIODIC = 1, ORC_BC_REFLECT = 2 };
  * leaf, the mean of the 8 fine cells from a finer leaf, and from a coarser
  * leaf a limited linear prolongation: the parent coarse cell plus, per
  * dimension, a monotonized-central slope of its face neighbours times
- * +-1/4.  depth guard layers of all nvar variables. */
-void orc_amr_fill_guards(const orc_params* P, const orc_amr* A, int nblocks, double* const* blocks, int nvar,
-                         int depth);
+ * +-1/4.  depth guard layers of all nvar variables.  Returns the number of
+ * blocks with a guard cell (or a prolongation neighbour) over a leaf two or
+ * more levels finer, a hierarchy not nested deeply enough for the depth;
+ * those values are not filled (0: all filled). */
+int orc_amr_fill_guards(const orc_params* P, const orc_amr* A, int nblocks, double* const* blocks, int nvar,
+                        int depth);
 
 /* Flux-register exchange (Grid_putFluxData_block, Grid_communicateFluxes,
  * Grid_getFluxCorrData_block with consv_flux_densities, red_f = 0.25,
@@ -169,7 +172,9 @@ void orc_amr_fill_guards(const orc_params* P, const orc_amr* A, int nblocks, dou
 void orc_flux_exchange(const orc_params* P, const orc_amr* A, int nblocks, double* const* fb, double* const* fc);
 
 /* nsteps of the hierarchy: guard fill, shared CFL dt, Hydro() on every
- * leaf, then (flux_correct) the exchange and hy_rk_correctFluxes. */
+ * leaf, then (flux_correct) the exchange and hy_rk_correctFluxes.  Returns
+ * the count of Riemann errors, or -1 when the guard fill refuses the
+ * hierarchy (see orc_amr_fill_guards). */
 int orc_amr_advance(const orc_params* P, const orc_amr* A, const orc_block* bl, int nblocks, double* const* blocks,
                     int nsteps, int flux_correct, int nthreads, double* dts_out);
 

@@ -8,6 +8,10 @@ problem with 8 mass scalars, gravity and flux correction; the batched
 FP64 form equals the plain form bit for bit; the FP32-flux form stays
 within L1(rho) <= 1e-5 of the oracle.
 """
+import importlib.util
+import json
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -15,6 +19,8 @@ from sparkhip import capi, problems
 import pyoracle
 
 pytestmark = pytest.mark.gpu
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -168,13 +174,14 @@ def same_level_blocks(prob):
 
 
 def rt_three_level_16():
-    """The shape of the 3D Rayleigh-Taylor workload at an eighth of its size:
-    16^3 blocks on three levels of leaves (8 / 32 / 256 on a 2 x 2 x 4
-    level-1 grid, z in [0, 2], level 2 where z meets (0.5, 1.5), level 3
-    where it meets (0.75, 1.25)), so the hierarchy has two level jumps;
-    8 mass scalars, gravity g = -1 along z, periodic x/y, reflecting z,
-    fine/coarse flux correction every step."""
-    return problems.rt_amr(nb0=(2, 2, 4), n=16, lmax=3, slabs=((0.5, 1.5), (0.75, 1.25)), nscalars=8)
+    """The shape of the 3D Rayleigh-Taylor workload at a twelfth of its size:
+    16^3 blocks on three levels of leaves (8 / 96 / 256 on a 2 x 2 x 6
+    level-1 grid, z in [0, 3], level 2 where z meets (0.5, 2.5), level 3
+    where it meets (1.25, 1.75)), so the hierarchy has two level jumps and
+    no guard layer (9 deep at level 1: 18 level-2 cells) reaches past a
+    level-2 leaf into level 3; 8 mass scalars, gravity g = -1 along z,
+    periodic x/y, reflecting z, fine/coarse flux correction every step."""
+    return problems.rt_amr(nb0=(2, 2, 6), n=16, lmax=3, slabs=((0.5, 2.5), (1.25, 1.75)), nscalars=8)
 
 
 @pytest.fixture(scope="module")
@@ -194,7 +201,7 @@ def test_three_level_16cubed_steps_match_oracle(form, rt16_oracle):
     exchange, correction): dt sequence and every interior value equal."""
     prob, nsteps, dref, nerr, ref = rt16_oracle
     assert prob.cfg["flux_correct"] == 1 and prob.cfg["use_gravity"] == 1
-    assert list(prob.info["leaves_per_level"]) == [8, 32, 256]
+    assert list(prob.info["leaves_per_level"]) == [8, 96, 256]
     assert prob.info["bc"] == (problems.BC_PERIODIC, problems.BC_PERIODIC, problems.BC_REFLECT)
     print(f"RT 3 levels {prob.info['leaves_per_level']}, 16^3 blocks, {form}, {nsteps} steps", flush=True)
     d, out, cnt = run(prob, nsteps, form)
@@ -224,6 +231,66 @@ def test_three_level_16cubed_fp32_flux_vs_oracle(rt16_oracle):
         if name != "rhou":
             assert num / den <= 1e-5, name
     assert np.max(np.abs(d32 - dref) / dref) <= 1e-5
+
+
+def _golden_rt16():
+    """tests/golden/make_rt16_oracle.py as a module, and its record."""
+    spec = importlib.util.spec_from_file_location("make_rt16_oracle", GOLDEN / "make_rt16_oracle.py")
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod, json.loads((GOLDEN / "rt16_oracle_3steps.json").read_text())
+
+
+@pytest.fixture(scope="module")
+def rt16_golden():
+    mod, rec = _golden_rt16()
+    prob = mod.problem()
+    assert list(prob.info["leaves_per_level"]) == rec["leaves_per_level"] == [8, 96, 256]
+    # the oracle's hashes hold for exactly these input bytes
+    assert mod.input_hash(prob) == rec["input_sha256"], "generated input differs from the golden record's"
+    return mod, rec, prob
+
+
+@pytest.mark.parametrize("form", ["plain_fp64", "batched_fp64"])
+def test_three_level_16cubed_steps_match_oracle_hashes(form, rt16_golden):
+    """3 steps over the three-level 16^3 hierarchy (two level jumps, 8 mass
+    scalars, gravity, periodic x/y, reflecting z, flux correction) against
+    the oracle's hierarchy driver: the dt sequence equals the oracle's as
+    hex floats and every leaf interior hashes to the oracle's SHA-256."""
+    mod, rec, prob = rt16_golden
+    assert prob.cfg["flux_correct"] == 1 and prob.cfg["use_gravity"] == 1
+    assert prob.info["bc"] == (problems.BC_PERIODIC, problems.BC_PERIODIC, problems.BC_REFLECT)
+    print(f"RT 3 levels {rec['leaves_per_level']}, 16^3 blocks, {form}, {rec['steps']} steps", flush=True)
+    d, out, cnt = run(prob, rec["steps"], form)
+    assert cnt[0] == 0
+    assert [float(x).hex() for x in d] == rec["dts"], (d, rec["dts"])
+    got = mod.interior_hashes(prob, out)
+    bad = [i for i, (x, y) in enumerate(zip(got, rec["interior_sha256"])) if x != y]
+    print(f"leaves differing from the oracle: {len(bad)}", flush=True)
+    assert len(got) == len(rec["interior_sha256"]) and not bad, bad[:10]
+
+
+def test_three_level_16cubed_fp32_flux_vs_plain(rt16_golden):
+    """The FP32-flux form over the same 3 steps against the plain FP64 form
+    (which the hash test pins to the oracle): L1(rho) and L1(E) within 1e-5
+    (L1(rho u) is printed: the flow starts at rest, so the relative momentum
+    error has a near-zero denominator)."""
+    mod, rec, prob = rt16_golden
+    d64, o64, _ = run(prob, rec["steps"], "plain_fp64")
+    d32, o32, cnt = run(prob, rec["steps"], "fp32_flux")
+    assert cnt[0] == 0
+    vols = [b["dx"] ** 3 for b in prob.blocks]
+    gi, ri = interiors(prob, o32), interiors(prob, o64)
+    V = capi.VI
+    fields = {"rho": lambda a: a[..., V["dens"]], "rhou": lambda a: a[..., V["dens"]] * a[..., V["velx"]],
+              "E": lambda a: a[..., V["dens"]] * a[..., V["ener"]]}
+    for name, f in fields.items():
+        num = sum(np.sum(np.abs(f(x) - f(y))) * v for x, y, v in zip(gi, ri, vols))
+        den = sum(np.sum(np.abs(f(y))) * v for y, v in zip(ri, vols))
+        print(f"L1({name}) {num / den:.2e}", flush=True)
+        if name != "rhou":
+            assert num / den <= 1e-5, name
+    assert np.max(np.abs(d32 - d64) / d64) <= 1e-5
 
 
 @pytest.mark.parametrize("case", ["gamma", "hybrid", "species"])

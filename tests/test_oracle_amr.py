@@ -1,6 +1,9 @@
 """The oracle's block-hierarchy services (guard fill over levels,
 flux-register exchange) against closed forms and the single-level fill."""
+from pathlib import Path
+
 import numpy as np
+import pytest
 
 from sparkhip import problems
 import pyoracle
@@ -159,3 +162,44 @@ def test_flux_correction_conserves_mass():
         drift[fc] = abs(mass(prob, arr) - m0) / m0
     assert drift[1] < 1e-14
     assert drift[0] > 1e-7
+
+
+def test_golden_rt16_record_is_the_oracle_result():
+    """tests/golden/rt16_oracle_3steps.json (the three-level 16^3 hierarchy,
+    3 steps) is what the oracle gives for what the generator builds: same
+    input bytes, same dt sequence, same interior hash for every leaf."""
+    import importlib.util
+    import json
+    gold = Path(__file__).resolve().parent / "golden"
+    spec = importlib.util.spec_from_file_location("make_rt16_oracle", gold / "make_rt16_oracle.py")
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    rec = json.loads((gold / "rt16_oracle_3steps.json").read_text())
+    prob = mod.problem()
+    assert mod.input_hash(prob) == rec["input_sha256"]
+    ref = [a.copy() for a in prob.arrays]
+    dts, nerr = pyoracle.amr_advance(prob.cfg, prob.blocks, prob.info["nb0"], prob.info["origins"], ref,
+                                     rec["steps"], flux_correct=True, bc=prob.info["bc"])
+    assert nerr == 0
+    assert [float(d).hex() for d in dts] == rec["dts"]
+    assert mod.interior_hashes(prob, ref) == rec["interior_sha256"]
+
+
+def test_fill_refuses_a_guard_layer_over_a_leaf_two_levels_finer():
+    """A level-1 block whose 9 guard layers (1.125 in z) reach past the
+    level-2 leaves above it into level 3: the oracle's fill has no cell set
+    for a leaf two levels finer, so it refuses the hierarchy (as the device
+    library does) instead of reading outside the finer block."""
+    prob = problems.rt_amr(nb0=(1, 1, 4), n=8, lmax=3, slabs=((1.5, 2.5), (1.9, 2.1)), nguard=10, nscalars=0)
+    assert list(prob.info["leaves_per_level"])[0] > 0 and list(prob.info["leaves_per_level"])[2] > 0
+    arr = [a.copy() for a in prob.arrays]
+    with pytest.raises(ValueError, match="two or more levels finer"):
+        pyoracle.amr_fill(prob.cfg, prob.blocks, prob.info["nb0"], prob.info["origins"], arr, depth=9,
+                          bc=prob.info["bc"])
+    with pytest.raises(ValueError, match="two or more levels finer"):
+        pyoracle.amr_advance(prob.cfg, prob.blocks, prob.info["nb0"], prob.info["origins"], arr, 1,
+                             bc=prob.info["bc"])
+    # one layer shallower than a level-2 leaf is tall (8 level-2 cells = 4
+    # level-1 cells) the same hierarchy fills
+    pyoracle.amr_fill(prob.cfg, prob.blocks, prob.info["nb0"], prob.info["origins"], arr, depth=4,
+                      bc=prob.info["bc"])
