@@ -37,6 +37,10 @@ namespace {
 thread_local std::string g_err;
 int g_device = -1;
 hipStream_t g_stream = nullptr;
+// device address of each device's fixed staging area (spark_pack_noalloc),
+// looked up once per device at spark_init
+constexpr int kMaxDevices = 64;
+void* g_noalloc[kMaxDevices] = {};
 
 int fail(int code, const char* fmt, ...) {
   char buf[512];
@@ -814,6 +818,12 @@ int spark_init(int device) {
     HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
   }
   g_device = device;
+  if (device < kMaxDevices && g_noalloc[device] == nullptr) {
+    void* ptr = nullptr;
+    HIPCHK(spark::noalloc_scratch_address(&ptr));
+    if (ptr == nullptr) return fail(SPARK_ERR_HIP, "spark_init: no device address for the staging area");
+    g_noalloc[device] = ptr;
+  }
   return SPARK_OK;
 }
 
@@ -1034,38 +1044,35 @@ int spark_batch_destroy(spark_batch* b) {
   return SPARK_OK;
 }
 
-static int transfer(spark_batch* b, double* const* host, int to_device, int interior_only, double* pool) {
+// AoS <-> SoA transfer in rounds of blocks through a device staging area of
+// `cap` doubles (`stage`), with device room for `maxb` block offsets
+// (`off_d`) and the variable map (`vidx_d`); off_h is host room for maxb
+// offsets.  Every round ends with the stream synchronised, so the staging
+// area is free again when the next round writes it.
+static int transfer_rounds(spark_batch* b, double* const* host, int to_device, int interior_only, double* pool,
+                           double* stage, int64_t cap, int64_t* off_d, int* vidx_d, int maxb, int64_t* off_h) {
   const int nunk = b->p.nunk_vars;
-  // stage at most ~256 MiB of AoS data per round trip
-  const int64_t cap = std::max<int64_t>((int64_t)nunk * b->max_cells, (int64_t)32 << 20);
-  double* stage = nullptr;
-  int64_t* off_d = nullptr;
-  int* vidx_d = nullptr;
-  HIPCHK(hipMalloc((void**)&stage, sizeof(double) * cap));
-  HIPCHK(hipMalloc((void**)&off_d, sizeof(int64_t) * b->nblk));
-  HIPCHK(hipMalloc((void**)&vidx_d, sizeof(int) * b->nvar));
   HIPCHK(hipMemcpy(vidx_d, b->p.var_index, sizeof(int) * b->nvar, hipMemcpyHostToDevice));
   int rc = SPARK_OK;
   int i0 = 0;
   while (i0 < b->nblk && rc == SPARK_OK) {
-    std::vector<int64_t> off;
     int64_t used = 0;
     int i1 = i0;
-    while (i1 < b->nblk) {
+    while (i1 < b->nblk && i1 - i0 < maxb) {
       const BlockMeta& m = b->meta_h[i1];
       const int64_t sz = (int64_t)nunk * m.NX * m.NY * m.NZ;
       if (used + sz > cap && i1 > i0) break;
-      off.push_back(used);
+      off_h[i1 - i0] = used;
       used += sz;
       i1++;
     }
-    hipError_t e = hipMemcpy(off_d, off.data(), sizeof(int64_t) * off.size(), hipMemcpyHostToDevice);
+    hipError_t e = hipMemcpy(off_d, off_h, sizeof(int64_t) * (i1 - i0), hipMemcpyHostToDevice);
     for (int i = i0; i < i1 && e == hipSuccess; i++) {
       const BlockMeta& m = b->meta_h[i];
       const int64_t sz = (int64_t)nunk * m.NX * m.NY * m.NZ;
       // unpack also stages the host block first, so unk slots outside the
       // variable map (and guards when interior_only) keep their host values
-      e = hipMemcpy(stage + off[i - i0], host[i], sizeof(double) * sz, hipMemcpyHostToDevice);
+      e = hipMemcpy(stage + off_h[i - i0], host[i], sizeof(double) * sz, hipMemcpyHostToDevice);
     }
     if (e == hipSuccess) {
       if (to_device)
@@ -1079,11 +1086,27 @@ static int transfer(spark_batch* b, double* const* host, int to_device, int inte
     for (int i = i0; i < i1 && e == hipSuccess && !to_device; i++) {
       const BlockMeta& m = b->meta_h[i];
       const int64_t sz = (int64_t)nunk * m.NX * m.NY * m.NZ;
-      e = hipMemcpy(host[i], stage + off[i - i0], sizeof(double) * sz, hipMemcpyDeviceToHost);
+      e = hipMemcpy(host[i], stage + off_h[i - i0], sizeof(double) * sz, hipMemcpyDeviceToHost);
     }
     if (e != hipSuccess) rc = fail(SPARK_ERR_HIP, "pack/unpack: %s", hipGetErrorString(e));
     i0 = i1;
   }
+  return rc;
+}
+
+static int transfer(spark_batch* b, double* const* host, int to_device, int interior_only, double* pool) {
+  const int nunk = b->p.nunk_vars;
+  // stage at most ~256 MiB of AoS data per round trip
+  const int64_t cap = std::max<int64_t>((int64_t)nunk * b->max_cells, (int64_t)32 << 20);
+  double* stage = nullptr;
+  int64_t* off_d = nullptr;
+  int* vidx_d = nullptr;
+  HIPCHK(hipMalloc((void**)&stage, sizeof(double) * cap));
+  HIPCHK(hipMalloc((void**)&off_d, sizeof(int64_t) * b->nblk));
+  HIPCHK(hipMalloc((void**)&vidx_d, sizeof(int) * b->nvar));
+  std::vector<int64_t> off_h(b->nblk);
+  const int rc = transfer_rounds(b, host, to_device, interior_only, pool, stage, cap, off_d, vidx_d, b->nblk,
+                                 off_h.data());
   (void)hipFree(stage);
   (void)hipFree(off_d);
   (void)hipFree(vidx_d);
@@ -1095,6 +1118,29 @@ int spark_pack(spark_batch* b, const double* const* unk_blocks) {
   for (int i = 0; i < b->nblk; i++)
     if (!unk_blocks[i]) return fail(SPARK_ERR_ARG, "spark_pack: block %d pointer is null", i);
   return transfer(b, const_cast<double* const*>(unk_blocks), 1, 0, b->U);
+}
+
+int spark_pack_noalloc(spark_batch* b, const double* const* unk_blocks) {
+  if (!b || !unk_blocks) return fail(SPARK_ERR_ARG, "spark_pack_noalloc: null argument");
+  for (int i = 0; i < b->nblk; i++)
+    if (!unk_blocks[i]) return fail(SPARK_ERR_ARG, "spark_pack_noalloc: block %d pointer is null", i);
+  if (g_device < 0 || g_device >= kMaxDevices || g_noalloc[g_device] == nullptr)
+    return fail(SPARK_ERR_STATE, "spark_pack_noalloc: no staging area on device %d (call spark_init)", g_device);
+  const int64_t cap = (int64_t)((spark::NOALLOC_BYTES - spark::NOALLOC_STAGE) / sizeof(double));
+  for (int i = 0; i < b->nblk; i++) {
+    const BlockMeta& m = b->meta_h[i];
+    const int64_t sz = (int64_t)b->p.nunk_vars * m.NX * m.NY * m.NZ;
+    if (sz > cap)
+      return fail(SPARK_ERR_ARG,
+                  "spark_pack_noalloc: block %d holds %lld unk values, more than the %lld of the fixed %zu MiB "
+                  "staging area (use spark_pack)", i, (long long)sz, (long long)cap, spark::NOALLOC_BYTES >> 20);
+  }
+  unsigned char* area = static_cast<unsigned char*>(g_noalloc[g_device]);
+  int64_t off_h[spark::NOALLOC_MAXB];
+  return transfer_rounds(b, const_cast<double* const*>(unk_blocks), 1, 0, b->U,
+                         reinterpret_cast<double*>(area + spark::NOALLOC_STAGE), cap,
+                         reinterpret_cast<int64_t*>(area + 256), reinterpret_cast<int*>(area), spark::NOALLOC_MAXB,
+                         off_h);
 }
 
 int spark_unpack(spark_batch* b, double* const* unk_blocks, int interior_only, int which) {

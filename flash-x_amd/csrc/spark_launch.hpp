@@ -108,4 +108,13 @@ hipError_t plain_soa_to_aos(hipStream_t st, const DevParams& P, const BlockMeta*
                             const double* U, double* aos, const int64_t* aos_off, int nunk, const int* var_index,
                             int nvar, int interior);
 
+// fixed staging area of spark_pack_noalloc (spark_noalloc.hip): 64 MiB per
+// device, of which the AoS staging of one round of blocks starts at
+// NOALLOC_STAGE; a round holds at most NOALLOC_MAXB blocks
+constexpr size_t NOALLOC_BYTES = size_t(64) << 20;
+constexpr int NOALLOC_MAXB = 1024;
+constexpr size_t NOALLOC_STAGE = 256 + 8 * (size_t)NOALLOC_MAXB;
+// device address of this device's area (call with the device current)
+hipError_t noalloc_scratch_address(void** ptr);
+
 }  // namespace spark

@@ -59,7 +59,7 @@ module spark_hip
             spark_get_flux_registers, spark_set_flux_corrections, spark_flux_correct, &
             spark_set_option, spark_batched_tiles, spark_synchronize, spark_get_counters, &
             spark_last_error, spark_profile, spark_profile_read, spark_set_amr_grid, spark_flux_exchange, &
-            spark_set_dt_control, spark_set_geometry
+            spark_set_dt_control, spark_set_geometry, spark_pack_noalloc
   public :: spark_check, spark_set_abort_hook, spark_error_message
 
   interface
@@ -96,6 +96,13 @@ module spark_hip
        type(c_ptr), value :: batch
        type(c_ptr), intent(in) :: unk_blocks(*)
      end function spark_pack
+
+     !! spark_pack through a fixed device staging area (no device allocation)
+     integer(c_int) function spark_pack_noalloc(batch, unk_blocks) bind(C, name="spark_pack_noalloc")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: batch
+       type(c_ptr), intent(in) :: unk_blocks(*)
+     end function spark_pack_noalloc
 
      integer(c_int) function spark_unpack(batch, unk_blocks, interior_only, which) bind(C, name="spark_unpack")
        import :: c_int, c_ptr

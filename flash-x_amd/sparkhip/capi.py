@@ -86,6 +86,7 @@ def lib():
                                    ctypes.POINTER(vp)],
             "spark_batch_destroy": [vp],
             "spark_pack": [vp, dpp],
+            "spark_pack_noalloc": [vp, dpp],
             "spark_unpack": [vp, dpp, ctypes.c_int, ctypes.c_int],
             "spark_set_uniform_grid": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int],
             "spark_fill_guardcells": [vp],
@@ -128,7 +129,7 @@ def exported_symbols():
         "spark_get_flux_registers", "spark_set_flux_corrections", "spark_flux_correct", "spark_synchronize",
         "spark_get_counters", "spark_last_error", "spark_profile", "spark_profile_read", "spark_set_option",
         "spark_batched_tiles", "spark_set_amr_grid", "spark_flux_exchange", "spark_set_dt_control",
-        "spark_set_geometry",
+        "spark_set_geometry", "spark_pack_noalloc",
     ]
 
 
@@ -226,6 +227,9 @@ class Batch:
 
     def pack(self, arrays):
         check(lib().spark_pack(self.h, _ptrs(arrays)), "spark_pack")
+
+    def pack_noalloc(self, arrays):
+        check(lib().spark_pack_noalloc(self.h, _ptrs(arrays)), "spark_pack_noalloc")
 
     def unpack(self, arrays, interior_only=False, which=0):
         check(lib().spark_unpack(self.h, _ptrs(arrays), int(interior_only), int(which)), "spark_unpack")

@@ -123,6 +123,16 @@ int spark_batch_destroy(spark_batch* b);
  * Replaces: tileDesc%getDataPtr + saveState's read of Uin,
  * Grid_tile.F90:157-267 and Spark/Hydro_funcs.F90-mc:512-541. */
 int spark_pack(spark_batch* b, const double* const* unk_blocks);
+/* spark_pack without device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (tight stepping
+ * loops, harnesses that forbid hipMalloc).  The host blocks are staged in a
+ * fixed 64 MiB device area per GPU, held from spark_init until the process
+ * ends, in rounds of at most 1024 blocks; a block larger than the area
+ * (nunk_vars * NX * NY * NZ doubles beyond 64 MiB less 8.25 KiB) is refused
+ * with SPARK_ERR_ARG.  The result equals spark_pack's.  All calls on one
+ * GPU share the area: calls that overlap in time on one GPU (from several
+ * threads) race. */
+int spark_pack_noalloc(spark_batch* b, const double* const* unk_blocks);
 /* Device -> host unk blocks; interior_only mirrors updateState
  * (Spark/Hydro_funcs.F90-mc:573-574).  which = 0: solution, 1: the stage
  * state of the last stage run (hy_starState). */
