@@ -90,22 +90,13 @@ __global__ void __launch_bounds__(256) k_flatten(DevParams P, const BlockMeta* _
 // (DIR 0: rows of x-pencils; DIR 1/2: x-columns of y/z pencils, so adjacent
 // lanes always touch adjacent x cells and every global access coalesces).
 // Each thread reconstructs one cell; the i-1/2 face of that cell takes the
-// i+1/2 state of the cell below through LDS.
-//
-// The stencil inputs are staged in LDS one variable at a time: the
-// workgroup loads each cell of its pencils (plus R cells at either end)
-// from global memory once, into one of two staging buffers, and every
-// thread reads its 2R+1 stencil values from there.  The loads of variable
-// v+1 are issued before the reconstruction of variable v and stored after
-// it, so one barrier per variable separates the two buffers' uses.
-template <int DIR, int RECON>
-struct FluxStage {
-  static constexpr int R = ReconRadius<RECON>::R;
-  // staged elements per thread: lanes * (Lr + 2R) over lanes * Lr threads,
-  // at most 2 for Lr >= 2R (the launcher checks it)
-  static constexpr int E = 2;
-};
-
+// i+1/2 state of the cell below through LDS.  Workgroups are ordered so
+// that each XCD runs a contiguous range of (pencil chunk, plane, block)
+// triples (xcd_chunk), so neighbouring chunks, which share cache lines at
+// their edges, share an L2.  The 2R+1 stencil values come straight from
+// global memory (L1/L2 serve the repeats): staging them in LDS, one
+// variable per barrier, was measured slower (the kernel is bound by FP64
+// VALU issue, not by its loads).
 template <int DIR, int RECON, int RIEMANN, int NSM>
 __global__ void __launch_bounds__(1024) k_flux(DevParams P, const BlockMeta* __restrict__ meta,
                                                const int* __restrict__ blist, const double* __restrict__ S,
@@ -115,7 +106,6 @@ __global__ void __launch_bounds__(1024) k_flux(DevParams P, const BlockMeta* __r
   constexpr int R = ReconRadius<RECON>::R;
   constexpr int NRM = R_NUM + NSM;
   constexpr int NFM = F_NUM + NSM;
-  constexpr int NE = FluxStage<DIR, RECON>::E;
   extern __shared__ double sm[];
   const int ns = P.ns;
   int bx, by;
@@ -125,7 +115,6 @@ __global__ void __launch_bounds__(1024) k_flux(DevParams P, const BlockMeta* __r
   constexpr int T1 = (DIR == 0) ? 1 : 0;
   constexpr int T2 = (DIR == 2) ? 1 : 2;
   const int Lr = L.hi[DIR] - L.lo[DIR] + 3;
-  const int Ls = Lr + 2 * R;  // staged cells per pencil
   const int nt1 = L.hi[T1] - L.lo[T1] + 1;
   const int chunks = (nt1 + lanes - 1) / lanes;
   const int c1 = bx % chunks, c2 = bx / chunks;
@@ -141,92 +130,49 @@ __global__ void __launch_bounds__(1024) k_flux(DevParams P, const BlockMeta* __r
   ijk[T1] = t1c;
   ijk[T2] = L.lo[T2] + c2;
   const int64_t c = cix(m, ijk[0], ijk[1], ijk[2]);
+  const int64_t st = (DIR == 0) ? 1 : (DIR == 1) ? (int64_t)m.NX : (int64_t)m.NX * m.NY;
   const double* sp = S + m.base;
   const int64_t cs = m.cstride;
-  // two staging buffers after the NRM + 6 exchange slots
-  double* stg = sm + (NRM + 6) * NT;
-  const int slot = lanes * Ls;
-  // this thread's stencil base in a staging buffer: position s + q + R of lane a
-  const int sb = (DIR == 0) ? a * Ls + s : s * lanes + a;
-  const int sq = (DIR == 0) ? 1 : lanes;  // staging stride between pencil positions
 
-  // staged elements this thread loads: element e = tid + j NT of the
-  // workgroup's lanes x Ls box (DIR 0: pencil-major, x contiguous; DIR 1/2:
-  // position-major, lanes contiguous), lanes past the region skipped
-  int64_t goff[NE];
-  int lix[NE];
-  const int ne = lanes * Ls;
-#pragma unroll
-  for (int jx = 0; jx < NE; jx++) {
-    const int e = tid + jx * NT;
-    lix[jx] = -1;
-    goff[jx] = 0;
-    if (e < ne) {
-      int ea, ep;
-      if (DIR == 0) { ea = e / Ls; ep = e - ea * Ls; }
-      else { ep = e / lanes; ea = e - ep * lanes; }
-      const int t1 = L.lo[T1] + c1 * lanes + ea;
-      if (t1 <= L.hi[T1]) {
-        int q3[3];
-        q3[DIR] = L.lo[DIR] - 1 - R + ep;
-        q3[T1] = t1;
-        q3[T2] = L.lo[T2] + c2;
-        goff[jx] = cix(m, q3[0], q3[1], q3[2]);
-        lix[jx] = e;
-      }
-    }
-  }
-  // staged variable v: 0 dens, 1-5 velx vely velz pres gamc, 6 rhoe =
-  // dens * eint (the product per cell, as the reference forms it before
-  // reconstructing), 7.. mass scalars
-  const int svar[6] = {V_DENS, V_VELX, V_VELY, V_VELZ, V_PRES, V_GAMC};
-  auto load = [&](int v, double* r) {
-#pragma unroll
-    for (int jx = 0; jx < NE; jx++) {
-      if (lix[jx] >= 0) {
-        const int64_t o = goff[jx];
-        if (v < 6) r[jx] = sp[svar[v] * cs + o];
-        else if (v == 6) r[jx] = sp[V_DENS * cs + o] * sp[V_EINT * cs + o];
-        else r[jx] = sp[(V_NPROP + v - 7) * cs + o];
-      }
-    }
-  };
-  auto store = [&](int v, const double* r) {
-    double* b = stg + (v & 1) * slot;
-#pragma unroll
-    for (int jx = 0; jx < NE; jx++)
-      if (lix[jx] >= 0) b[lix[jx]] = r[jx];
-  };
-
-  const double fl_c = (valid && RECON == RECON_WENO5 && P.use_flat) ? flat[m.obase + c] : 1.0;
-  if (valid) sm[(NRM + 5) * NT + tid] = Ushok[m.base + V_SHOK * cs + c];
-  const int nv = R_NUM + ns;
   double umr[NRM];
-  double rg[NE];
-  load(0, rg);
-  store(0, rg);
-  __syncthreads();
+  if (valid) {
+    const double fl_c = (RECON == RECON_WENO5 && P.use_flat) ? flat[m.obase + c] : 1.0;
+    double ud[5], u[5], up, um;
 #pragma unroll
-  for (int v = 0; v < NRM; v++) {
-    if (v < nv) {
-      if (v + 1 < nv) load(v + 1, rg);
-      if (valid) {
-        const double* b = stg + (v & 1) * slot + sb;
-        double u[5], up, um;
+    for (int q = -R; q <= R; q++) ud[q + 2] = sp[V_DENS * cs + c + q * st];
+    recon1<RECON>(ud, fl_c, up, um);
+    sm[R_DENS * NT + tid] = up;
+    umr[R_DENS] = um;
+    const int svar[5] = {V_VELX, V_VELY, V_VELZ, V_PRES, V_GAMC};
 #pragma unroll
-        for (int q = 0; q < 5; q++) u[q] = (q >= 2 - R && q <= 2 + R) ? b[(q - 2 + R) * sq] : 0.0;
+    for (int v = 0; v < 5; v++) {
+#pragma unroll
+      for (int q = -R; q <= R; q++) u[q + 2] = sp[svar[v] * cs + c + q * st];
+      recon1<RECON>(u, fl_c, up, um);
+      sm[(1 + v) * NT + tid] = up;
+      umr[1 + v] = um;
+      if (v < 3) sm[(NRM + 1 + v) * NT + tid] = u[2];
+    }
+#pragma unroll
+    for (int q = -R; q <= R; q++) u[q + 2] = ud[q + 2] * sp[V_EINT * cs + c + q * st];
+    recon1<RECON>(u, fl_c, up, um);
+    sm[R_RHOE * NT + tid] = up;
+    umr[R_RHOE] = um;
+    sm[(NRM + 4) * NT + tid] = u[2];
+    sm[(NRM + 0) * NT + tid] = ud[2];
+    sm[(NRM + 5) * NT + tid] = Ushok[m.base + V_SHOK * cs + c];
+#pragma unroll
+    for (int v = 0; v < NSM; v++) {
+      if (v < ns) {
+#pragma unroll
+        for (int q = -R; q <= R; q++) u[q + 2] = sp[(V_NPROP + v) * cs + c + q * st];
         recon1<RECON>(u, fl_c, up, um);
-        sm[v * NT + tid] = up;
-        umr[v] = um;
-        // centred dens, velx, vely, velz and rhoe for the viscosity term
-        if (v == R_DENS) sm[(NRM + 0) * NT + tid] = u[2];
-        else if (v <= R_VELZ) sm[(NRM + v) * NT + tid] = u[2];
-        else if (v == R_RHOE) sm[(NRM + 4) * NT + tid] = u[2];
+        sm[(R_NUM + v) * NT + tid] = up;
+        umr[R_NUM + v] = um;
       }
-      if (v + 1 < nv) store(v + 1, rg);
-      __syncthreads();
     }
   }
+  __syncthreads();
   if (!valid || s < 1) return;
   const int tn = (DIR == 0) ? tid - 1 : tid - lanes;
   double left[NRM], right[NRM];
@@ -894,11 +840,7 @@ static hipError_t launch_flux_t(const PlainLaunch& L, int ngoff) {
   // uneven split leaves a chunk of mostly idle lanes), the largest on
   // ties.  y/z sweeps keep >= 16 x-columns per row where the plane has them,
   // so each row of a wave's loads fills a 128-byte line.
-  // exchange slots (NRM + 6 per thread) and two staging buffers of
-  // Lr + 2R cells per pencil
-  constexpr int RR = ReconRadius<RECON>::R;
-  if (Lr < 2 * RR) return hipErrorInvalidValue;  // FluxStage::E covers Lr >= 2R
-  const int cap = 81920 / (int)(sizeof(double) * ((NRM + 6) * Lr + 2 * (Lr + 2 * RR)));
+  const int cap = 81920 / (int)(sizeof(double) * (NRM + 6) * Lr);
   const int lmax = std::max(1, std::min(std::min(1024 / Lr, cap), nt1));
   const int lmin = DIR == 0 ? 1 : std::min(std::min(16, nt1), lmax);
   int lanes = lmax, best = 1 << 30;
@@ -909,7 +851,7 @@ static hipError_t launch_flux_t(const PlainLaunch& L, int ngoff) {
   const int chunks = (nt1 + lanes - 1) / lanes;
   dim3 grid(chunks * nt2, L.nblk);
   dim3 block(lanes * Lr);
-  const size_t shm = sizeof(double) * ((size_t)(NRM + 6) * block.x + 2 * (size_t)lanes * (Lr + 2 * RR));
+  const size_t shm = sizeof(double) * (size_t)(NRM + 6) * block.x;
   if (L.prof) {
     // algorithmic traffic: every state value of the stencil box read once
     // (dens velx vely velz pres gamc eint shok, scalars, flat), every face
