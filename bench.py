@@ -108,12 +108,16 @@ def parse():
 
 
 def calibrate_copy(mib: int):
-    """A plain device copy of known size: its FETCH_SIZE / WRITE_SIZE counter
-    readings give the scale of those counters for the traffic figure."""
+    """Device passes of known size for the counter scale: a 1 GiB read and
+    write by an ordinary elementwise kernel (torch.neg, what the scale is
+    taken from) and by the runtime's blit copy (reported beside it: its
+    reads are counted at half their bytes)."""
     import torch
     n = mib * (1 << 20) // 8
     src = torch.ones(n, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.neg(src, out=dst)
     for _ in range(3):
         dst.copy_(src)
     torch.cuda.synchronize()
@@ -207,6 +211,7 @@ def norms(prob, got, ref):
 # the untimed warmup / tuning dispatches from the profiled steps
 MARKER = "FillFunctor"
 NUM_CU = 256
+CLOCK_HZ = 2.4e9  # MI355X peak engine clock
 
 
 def _csv_rows(path):
@@ -230,21 +235,23 @@ def _segment(rows, i):
 def summarize(args):
     """Per-kernel table of the profiled steps: dispatches per step, mean
     duration (kernel trace) and per-dispatch means of the counters (pmc
-    passes), HBM bytes scaled by the copy calibration."""
+    passes), HBM bytes scaled by the calibration pass."""
     from collections import defaultdict
     root = Path(args.summarize)
     nsteps = args.summary_steps
-    # calibration: the largest copy dispatch moves calibrate_copy_bytes in and out
-    calib = {}
-    for f in Path(args.calib).rglob("*counter_collection.csv"):
-        best = {}
-        for r in _csv_rows(f):
-            if "copy" in r["kernel_name"].lower():
-                c = r["counter_name"]
-                best[c] = max(best.get(c, 0.0), float(r["counter_value"]))
-        for c, v in best.items():
-            calib[c] = v * 1024.0  # KiB -> bytes
+    # calibration: the largest dispatch of a --calibrate-copy kernel moves
+    # 1 GiB in and 1 GiB out
+    def calib_of(match):
+        out = {}
+        for f in Path(args.calib).rglob("*counter_collection.csv"):
+            for r in _csv_rows(f):
+                if match(r["kernel_name"].lower()):
+                    c = r["counter_name"]
+                    out[c] = max(out.get(c, 0.0), float(r["counter_value"]) * 1024.0)  # KiB -> bytes
+        return out
     true_bytes = 1024 << 20
+    calib = calib_of(lambda k: "neg" in k and "elementwise" in k)
+    blit = calib_of(lambda k: "copybuffer" in k)
     scale = {c: true_bytes / v for c, v in calib.items() if v > 0}
     dur = defaultdict(list)
     for f in root.rglob("*kernel_trace.csv"):
@@ -257,6 +264,8 @@ def summarize(args):
 
     def mean(k, c):
         d = ctr[k].get(c)
+        if not d:
+            d = ctr[k].get(c + "_sum")
         return sum(d.values()) / len(d) if d else None
 
     def short(name):
@@ -265,7 +274,7 @@ def summarize(args):
 
     lines = []
     cols = ["kernel", "per_step", "avg_us", "fetch_MB", "write_MB", "hbm_MB", "hbm_frac", "valu_insts", "valu_busy",
-            "fp64_gflop", "fp64_frac", "fp32_gflop", "lds_insts"]
+            "valu_issue", "fp64_gflop", "fp64_frac", "fp32_gflop", "lds_insts", "salu_insts", "waves"]
     lines.append(" | ".join(cols))
     for k in sorted(dur, key=lambda k: -sum(dur[k])):
         n = len(dur[k])
@@ -276,6 +285,10 @@ def summarize(args):
         hbm = fe + wr if fe is not None and wr is not None else None
         act, gui = mean(k, "SQ_ACTIVE_INST_VALU"), mean(k, "GRBM_GUI_ACTIVE")
         busy = act / NUM_CU / gui if act is not None and gui else None
+        # VALU issue fraction: wave64 VALU instructions x 4 cycles (16-lane
+        # SIMD) over the 1024 SIMDs' cycles of the launch at 2.4 GHz
+        vi = mean(k, "SQ_INSTS_VALU")
+        issue = vi * 4.0 / (us * 1e-6 * CLOCK_HZ * 4 * NUM_CU) if vi is not None else None
         f64 = [mean(k, "SQ_INSTS_VALU_" + x + "_F64") for x in ("ADD", "MUL", "FMA", "TRANS")]
         f32 = [mean(k, "SQ_INSTS_VALU_" + x + "_F32") for x in ("ADD", "MUL", "FMA", "TRANS")]
         fl64 = 64.0 * (f64[0] + f64[1] + 2.0 * f64[2] + f64[3]) if None not in f64 else None
@@ -285,12 +298,17 @@ def summarize(args):
             return "-" if x is None else format(x, spec)
         row = [short(k), fmt(n / nsteps, ".2f"), fmt(us, ".1f"), fmt(fe and fe / 1e6, ".1f"), fmt(wr and wr / 1e6, ".1f"),
                fmt(hbm and hbm / 1e6, ".1f"), fmt(hbm and hbm / (us * 1e-6) / (HBM_PEAK_GBS * 1e9), ".3f"),
-               fmt(mean(k, "SQ_INSTS_VALU"), ".4g"), fmt(busy, ".3f"), fmt(fl64 and fl64 / 1e9, ".2f"),
+               fmt(mean(k, "SQ_INSTS_VALU"), ".4g"), fmt(busy, ".3f"), fmt(issue, ".3f"),
+               fmt(fl64 and fl64 / 1e9, ".2f"),
                fmt(fl64 and fl64 / (us * 1e-6) / (FP64_PEAK_TFS * 1e12), ".3f"), fmt(fl32 and fl32 / 1e9, ".2f"),
-               fmt(mean(k, "SQ_INSTS_LDS"), ".4g")]
+               fmt(mean(k, "SQ_INSTS_LDS"), ".4g"), fmt(mean(k, "SQ_INSTS_SALU"), ".4g"),
+               fmt(mean(k, "SQ_WAVES"), ".4g")]
         lines.append(" | ".join(row))
-    lines.append(f"# calibration factors (true bytes / counter bytes of a 1 GiB copy): " +
-                 ", ".join(f"{c} x{v:.3f}" for c, v in sorted(scale.items())))
+    if scale:
+        lines.append("# FETCH_SIZE / WRITE_SIZE scaled by true bytes / counter bytes of a 1 GiB elementwise "
+                     "read + write (torch.neg): " + ", ".join(f"{c} x{v:.3f}" for c, v in sorted(scale.items())) +
+                     "; the runtime's blit copy of 1 GiB reads as " +
+                     ", ".join(f"{c} {v / true_bytes:.3f}" for c, v in sorted(blit.items())) + " of its bytes")
     print("\n".join(lines))
 
 
@@ -464,8 +482,11 @@ def main():
     frows = [v for k, v in cs["rows"].items() if k.startswith("k_flux<")] if cs else []
     traffic = (sum(r["hbm_MB"] for r in frows) / len(frows) * 1e6
                if frows and all(r.get("hbm_MB") is not None for r in frows) else None)
-    busy = (sum(r["valu_busy"] for r in frows) / len(frows)
-            if frows and all(r.get("valu_busy") is not None for r in frows) else None)
+    # VALU issue fraction (wave64 VALU instructions x 4 cycles over the SIMD
+    # cycles of the launch), else the SQ_ACTIVE_INST_VALU busy fraction
+    col = "valu_issue" if frows and all(r.get("valu_issue") is not None for r in frows) else "valu_busy"
+    busy = (sum(r[col] for r in frows) / len(frows)
+            if frows and all(r.get(col) is not None for r in frows) else None)
     hbm_counter_frac = traffic / avg_s / (HBM_PEAK_GBS * 1e9) if traffic else None
     # the binding roof: the counters' VALU busy against their HBM fraction;
     # without counters, the algorithmic intensity against the FP64 ridge
@@ -489,7 +510,8 @@ def main():
                 "counter_gbs": round(traffic / avg_s / 1e9, 1) if traffic else None,
                 "counter_frac": round(hbm_counter_frac, 4) if hbm_counter_frac else None},
         "valu_fp64": {"alg_tflops": round(alg_tfs, 2), "alg_frac": round(alg_tfs / FP64_PEAK_TFS, 4),
-                      "counter_valu_busy": round(busy, 4) if busy is not None else None},
+                      "counter_valu_frac": round(busy, 4) if busy is not None else None,
+                      "counter_valu_column": col if busy is not None else None},
     }
     # algorithmic FP64 state bytes per zone-update: 8 (48 + 8 N_s)
     b_alg = 8.0 * (48 + 8 * prob.cfg.get("nscalars", 0))
