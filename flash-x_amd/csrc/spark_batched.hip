@@ -423,9 +423,18 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
           plane[v * psz + pcc] = win[v][2];
         }
         // scalars: the plane only (their z stencils are read again when the
-        // scalar chunks are reconstructed)
+        // scalar chunks are reconstructed); all loads issue before the first
+        // store waits on them
         const int64_t ck = cell_at(m, nbt, g, ci, cj, k);
-        for (int s = 0; s < ns; s++) plane[(NH + s) * psz + pcc] = (Real)S[(V_NPROP + s) * cs + ck];
+#pragma unroll
+        for (int s0 = 0; s0 < NSM; s0 += 4) {
+          Real sv[4];
+#pragma unroll
+          for (int q = 0; q < 4; q++) sv[q] = (Real)S[(V_NPROP + (s0 + q < ns ? s0 + q : ns - 1)) * cs + ck];
+#pragma unroll
+          for (int q = 0; q < 4; q++)
+            if (s0 + q < ns) plane[(NH + s0 + q) * psz + pcc] = sv[q];
+        }
       }
       {
         // the 3-cell halo ring: 3 full rows below and above, 3 + 3 cells in
@@ -443,7 +452,21 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
           }
           const int64_t c = cell_at(m, nbt, g, I.x0 - 3 + qx, I.y0 - 3 + qy, k);
           const int pc = qy * PW + qx;
-          for (int v = 0; v < nvt; v++) plane[v * psz + pc] = load_slot(S, cs, c, v, P.gamma);
+          // the hydro slots, then the scalars, each group loaded before
+          // the first of its stores waits
+          Real hv[NH > NSA ? NH : NSA];
+#pragma unroll
+          for (int v = 0; v < NH; v++) hv[v] = load_slot(S, cs, c, v, P.gamma);
+#pragma unroll
+          for (int v = 0; v < NH; v++) plane[v * psz + pc] = hv[v];
+          // (slots past ns load scalar ns-1 again, so no load is conditional)
+          if (NSM > 0) {
+#pragma unroll
+            for (int v = 0; v < NSA; v++) hv[v] = (Real)S[(V_NPROP + (v < ns ? v : ns - 1)) * cs + c];
+#pragma unroll
+            for (int v = 0; v < NSA; v++)
+              if (v < ns) plane[(NH + v) * psz + pc] = hv[v];
+          }
         }
       }
       __syncthreads();
@@ -669,14 +692,22 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
                   int64_t zc[5];
 #pragma unroll
                   for (int w = 0; w < 5; w++) zc[w] = cell_at(m, nbt, g, ci, cj, k - 2 + w);
+                  // the z stencils of two scalars load before the first of
+                  // their reconstructions waits (a slot past nc loads scalar
+                  // nc-1 again)
 #pragma unroll
-                  for (int j = 0; j < CH; j++) {
-                    if (j < nc) {
-                      const double* sp = S + (int64_t)(V_NPROP + c0 + j) * cs;
-                      const Real u5[5] = {(Real)sp[zc[0]], (Real)sp[zc[1]], (Real)sp[zc[2]], (Real)sp[zc[3]],
-                                          (Real)sp[zc[4]]};
-                      recon1<RECON, Real>(u5, fl, upsk[j], ums[j]);
+                  for (int j0 = 0; j0 < CH; j0 += 2) {
+                    Real u5s[2][5];
+#pragma unroll
+                    for (int jj = 0; jj < 2; jj++) {
+                      const int j = j0 + jj;
+                      const double* sp = S + (int64_t)(V_NPROP + c0 + (j < nc ? j : nc - 1)) * cs;
+#pragma unroll
+                      for (int w = 0; w < 5; w++) u5s[jj][w] = (Real)sp[zc[w]];
                     }
+#pragma unroll
+                    for (int jj = 0; jj < 2; jj++)
+                      if (j0 + jj < nc) recon1<RECON, Real>(u5s[jj], fl, upsk[j0 + jj], ums[j0 + jj]);
                   }
                 }
                 continue;
