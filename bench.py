@@ -225,7 +225,7 @@ def _segment(rows, i):
     """Dispatches of the i-th profiled form: between its two marker
     dispatches (one before and one after its timed steps)."""
     rows = sorted(rows, key=lambda r: int(r["dispatch_id"]))
-    marks = [int(r["dispatch_id"]) for r in rows if MARKER in r["kernel_name"]]
+    marks = sorted({int(r["dispatch_id"]) for r in rows if MARKER in r["kernel_name"]})
     if len(marks) < 2 * i + 2:
         raise SystemExit(f"profile holds {len(marks)} marker dispatches, form {i} needs {2 * i + 2}")
     lo, hi = marks[2 * i], marks[2 * i + 1]
