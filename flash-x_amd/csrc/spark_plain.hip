@@ -162,13 +162,23 @@ __global__ void __launch_bounds__(1024) k_flux(DevParams P, const BlockMeta* __r
     sm[(NRM + 0) * NT + tid] = ud[2];
     sm[(NRM + 5) * NT + tid] = Ushok[m.base + V_SHOK * cs + c];
 #pragma unroll
-    for (int v = 0; v < NSM; v++) {
-      if (v < ns) {
+    for (int v0 = 0; v0 < NSM; v0 += 2) {
+      // the stencils of two scalars load before the first reconstruction
+      // waits (a slot past ns re-reads scalar ns-1)
+      double u2[2][5];
 #pragma unroll
-        for (int q = -R; q <= R; q++) u[q + 2] = sp[(V_NPROP + v) * cs + c + q * st];
-        recon1<RECON>(u, fl_c, up, um);
-        sm[(R_NUM + v) * NT + tid] = up;
-        umr[R_NUM + v] = um;
+      for (int h = 0; h < 2; h++) {
+        const int v = v0 + h < ns ? v0 + h : ns - 1;
+#pragma unroll
+        for (int q = 0; q < 5; q++) u2[h][q] = (q >= 2 - R && q <= 2 + R) ? sp[(V_NPROP + v) * cs + c + (q - 2) * st] : 0.0;
+      }
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        if (v0 + h < ns) {
+          recon1<RECON>(u2[h], fl_c, up, um);
+          sm[(R_NUM + v0 + h) * NT + tid] = up;
+          umr[R_NUM + v0 + h] = um;
+        }
       }
     }
   }
@@ -281,8 +291,12 @@ __global__ void __launch_bounds__(1024) k_flux(DevParams P, const BlockMeta* __r
 
 // ---------------------------------------------------------------------------
 // conserved update + floors (hy_rk_updateSoln.F90-mc:211-287) and the
-// Gamma EOS in MODE_DENS_EI (Eos_wrapped over the same limits).
-template <int NSM>
+// Gamma EOS in MODE_DENS_EI (Eos_wrapped over the same limits).  Every
+// load of a cell (the U0 and stage-input state, the low and high face
+// fluxes of each direction) is issued before the first use waits: the
+// dimension count is a template parameter and scalar slots past ns re-read
+// scalar ns-1, so no load sits behind a branch.
+template <int NSM, int NDIM>
 __global__ void __launch_bounds__(256) k_update(DevParams P, const BlockMeta* __restrict__ meta,
                                                 const int* __restrict__ blist, const double* U0,
                                                 const double* Sin, double* Sout, const double* __restrict__ flx,
@@ -308,66 +322,92 @@ __global__ void __launch_bounds__(256) k_update(DevParams P, const BlockMeta* __
   for (int t = cx * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
     const int i = L.lo[0] + t % ex, j = L.lo[1] + (t / ex) % ey, k = L.lo[2] + t / (ex * ey);
     const int64_t c = cix(m, i, j, k);
-    double A[NFM], B[NFM];
-    const double d0 = u0[V_DENS * cs + c];
-    A[F_MASS] = d0;
-    A[F_XMOM] = d0 * u0[V_VELX * cs + c];
-    A[F_YMOM] = d0 * u0[V_VELY * cs + c];
-    A[F_ZMOM] = d0 * u0[V_VELZ * cs + c];
-    A[F_ENER] = d0 * u0[V_ENER * cs + c];
-    const double ds = ui[V_DENS * cs + c];
-    B[F_MASS] = ds;
-    B[F_XMOM] = ds * ui[V_VELX * cs + c];
-    B[F_YMOM] = ds * ui[V_VELY * cs + c];
-    B[F_ZMOM] = ds * ui[V_VELZ * cs + c];
-    B[F_ENER] = ds * ui[V_ENER * cs + c];
+    const int64_t fm = m.fbase + fix(m, i, j, k);
+    const int64_t fpx = m.fbase + fix(m, i + 1, j, k);
+    const int64_t fpy = m.fbase + fix(m, i, j + 1, k);
+    const int64_t fpz = m.fbase + fix(m, i, j, k + 1);
+    // hydro part: its 10 state and 10 NDIM face-flux loads issue together
+    double A[F_NUM], B[F_NUM], fl0[F_NUM], fh0[F_NUM], fl1[F_NUM], fh1[F_NUM], fl2[F_NUM], fh2[F_NUM];
+    {
+      const int sv[5] = {V_DENS, V_VELX, V_VELY, V_VELZ, V_ENER};
 #pragma unroll
-    for (int v = 0; v < NSM; v++) {
-      if (v < ns) {
-        A[F_NUM + v] = u0[(V_NPROP + v) * cs + c] * d0;
-        B[F_NUM + v] = ui[(V_NPROP + v) * cs + c] * ds;
+      for (int q = 0; q < F_NUM; q++) {
+        A[q] = u0[sv[q] * cs + c];
+        B[q] = ui[sv[q] * cs + c];
+        fl0[q] = flx[fm + q * fs];
+        fh0[q] = flx[fpx + q * fs];
+        fl1[q] = fh1[q] = fl2[q] = fh2[q] = 0.;
+        if (NDIM > 1) { fl1[q] = fly[fm + q * fs]; fh1[q] = fly[fpy + q * fs]; }
+        if (NDIM > 2) { fl2[q] = flz[fm + q * fs]; fh2[q] = flz[fpz + q * fs]; }
       }
     }
-    double Sg[NFM];
+    const double d0 = A[F_MASS];
 #pragma unroll
-    for (int n = 0; n < NFM; n++) Sg[n] = 0.;
+    for (int q = 1; q < F_NUM; q++) A[q] = d0 * A[q];
+    const double ds = B[F_MASS];
+#pragma unroll
+    for (int q = 1; q < F_NUM; q++) B[q] = ds * B[q];
+    double Sg[F_NUM];
+#pragma unroll
+    for (int n = 0; n < F_NUM; n++) Sg[n] = 0.;
     if (P.use_grav) {
       Sg[F_XMOM] = B[F_MASS] * P.grav[0];
       Sg[F_YMOM] = B[F_MASS] * P.grav[1];
       Sg[F_ZMOM] = B[F_MASS] * P.grav[2];
       Sg[F_ENER] = B[F_XMOM] * P.grav[0] + B[F_YMOM] * P.grav[1] + B[F_ZMOM] * P.grav[2];
     }
-    const int64_t fm = m.fbase + fix(m, i, j, k);
-    const int64_t fpx = m.fbase + fix(m, i + 1, j, k);
-    const int64_t fpy = m.fbase + fix(m, i, j + 1, k);
-    const int64_t fpz = m.fbase + fix(m, i, j, k + 1);
     // geometric factors and sources (geoFacs, hy_rk_updateSoln.F90-mc:98-175)
     double facM, facP, rc, Sgeo[F_NUM] = {0., 0., 0., 0., 0.};
     if (geo_facs(P, m.id, i, j, dx, facM, facP, rc) && P.geom != 0)
       geo_sources(P, m.id, i, dx, rc, ui[V_DENS * cs + c], ui[V_VELX * cs + c], ui[V_VELZ * cs + c],
                   ui[V_PRES * cs + c], B[F_YMOM], Sgeo);
 #pragma unroll
-    for (int n = 0; n < NFM; n++) {
-      if (n < F_NUM + ns) {
-        double inner = -(dt / dx * (facP * flx[fpx + n * fs] - facM * flx[fm + n * fs]));
-        if (P.ndim > 1) inner = inner - dt / dy * (fly[fpy + n * fs] - fly[fm + n * fs]);
-        if (P.ndim == 3) inner = inner - dt / dz * (flz[fpz + n * fs] - flz[fm + n * fs]);
-        inner = inner + dt * ((n < F_NUM ? Sgeo[n < F_NUM ? n : 0] : 0.0) + Sg[n]);
-        B[n] = c1 * A[n] + c2 * B[n] + c3 * inner;
-      }
+    for (int n = 0; n < F_NUM; n++) {
+      double inner = -(dt / dx * (facP * fh0[n] - facM * fl0[n]));
+      if (NDIM > 1) inner = inner - dt / dy * (fh1[n] - fl1[n]);
+      if (NDIM == 3) inner = inner - dt / dz * (fh2[n] - fl2[n]);
+      inner = inner + dt * (Sgeo[n] + Sg[n]);
+      B[n] = c1 * A[n] + c2 * B[n] + c3 * inner;
     }
     const double dens = fmax2(B[F_MASS], P.smlrho);
     const double vx = B[F_XMOM] / dens, vy = B[F_YMOM] / dens, vz = B[F_ZMOM] / dens;
     const double ener = fmax2(P.smalle, B[F_ENER] / dens);
     const double ekin = .5 * (vx * vx + vy * vy + vz * vz);
     const double eint = fmax2(P.smalle, ener - ekin);
+    // mass scalars, four at a time: the 8 state and 8 NDIM face-flux loads
+    // of a group issue together (slots past ns re-read scalar ns-1)
     double Xs[NSM > 0 ? NSM : 1];  // the mass fractions (species first) the EOS mixes
 #pragma unroll
-    for (int v = 0; v < NSM; v++)
-      if (v < ns) {
-        Xs[v] = B[F_NUM + v] / dens;
-        us[(V_NPROP + v) * cs + c] = Xs[v];
+    for (int v0 = 0; v0 < NSM; v0 += 4) {
+      double a4[4], b4[4], l0[4], h0[4], l1[4], h1[4], l2[4], h2[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int v = v0 + q < ns ? v0 + q : ns - 1;
+        const int64_t o = (int64_t)(V_NPROP + v) * cs + c, of = (int64_t)(F_NUM + v) * fs;
+        a4[q] = u0[o];
+        b4[q] = ui[o];
+        l0[q] = flx[fm + of];
+        h0[q] = flx[fpx + of];
+        l1[q] = h1[q] = l2[q] = h2[q] = 0.;
+        if (NDIM > 1) { l1[q] = fly[fm + of]; h1[q] = fly[fpy + of]; }
+        if (NDIM > 2) { l2[q] = flz[fm + of]; h2[q] = flz[fpz + of]; }
       }
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int v = v0 + q;
+        if (v < ns) {
+          const double As = a4[q] * d0;
+          double Bs = b4[q] * ds;
+          double inner = -(dt / dx * (facP * h0[q] - facM * l0[q]));
+          if (NDIM > 1) inner = inner - dt / dy * (h1[q] - l1[q]);
+          if (NDIM == 3) inner = inner - dt / dz * (h2[q] - l2[q]);
+          inner = inner + dt * (0.0 + 0.0);
+          Bs = c1 * As + c2 * Bs + c3 * inner;
+          Xs[v] = Bs / dens;
+          us[(V_NPROP + v) * cs + c] = Xs[v];
+        }
+      }
+    }
     EosOut e = eos_dens_ei(P, dens, vx, vy, vz, ener, eint, NSM > 0 ? Xs : nullptr);
     us[V_DENS * cs + c] = dens;
     us[V_VELX * cs + c] = vx;
@@ -929,12 +969,23 @@ hipError_t plain_stage(const PlainLaunch& L, int ngoff, double c1, double c2, do
     const double bytes = 8.0 * L.nblk * ((double)box * (2 * (5 + nsv) + 8 + nsv + 2 * final_stage) + faces * (5 + nsv));
     L.prof->begin(L.stream, "k_update", bytes, (double)L.nblk * box * (60.0 + 10.0 * L.P.ndim * (5 + nsv)));
   }
-  if (L.P.ns == 0)
-    hipLaunchKernelGGL(k_update<0>, dim3(grid_x_for(box, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta, L.blist,
-                       L.U0, L.Sin, L.Sout, L.fl[0], L.fl[1], L.fl[2], L.dtp, c1, c2, c3, ngoff, final_stage);
-  else
-    hipLaunchKernelGGL(k_update<8>, dim3(grid_x_for(box, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta, L.blist,
-                       L.U0, L.Sin, L.Sout, L.fl[0], L.fl[1], L.fl[2], L.dtp, c1, c2, c3, ngoff, final_stage);
+  {
+    const dim3 grid(grid_x_for(box, 512), L.nblk), blk(256);
+#define SPARK_UPDATE(NSM_, ND_)                                                                              \
+  hipLaunchKernelGGL((k_update<NSM_, ND_>), grid, blk, 0, L.stream, L.P, L.meta, L.blist, L.U0, L.Sin, L.Sout, \
+                     L.fl[0], L.fl[1], L.fl[2], L.dtp, c1, c2, c3, ngoff, final_stage)
+    const int nd = L.P.ndim;
+    if (L.P.ns == 0) {
+      if (nd == 3) SPARK_UPDATE(0, 3);
+      else if (nd == 2) SPARK_UPDATE(0, 2);
+      else SPARK_UPDATE(0, 1);
+    } else {
+      if (nd == 3) SPARK_UPDATE(8, 3);
+      else if (nd == 2) SPARK_UPDATE(8, 2);
+      else SPARK_UPDATE(8, 1);
+    }
+#undef SPARK_UPDATE
+  }
   if (L.prof) L.prof->end(L.stream);
   return hipGetLastError();
 }
