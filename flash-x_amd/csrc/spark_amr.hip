@@ -161,9 +161,13 @@ __global__ void __launch_bounds__(256) k_amr_fill(DevParams P, const BlockMeta* 
       constexpr int CHV = 8;
       for (int q0 = 0; q0 < vars.n; q0 += CHV) {
         double val[CHV];
+        // a slot past vars.n re-reads the last variable, so no load is
+        // conditional
 #pragma unroll
-        for (int u = 0; u < CHV; u++)
-          if (q0 + u < vars.n) val[u] = flipped(src_val(U, r, S, vars.v[q0 + u]), flip, vars.v[q0 + u]);
+        for (int u = 0; u < CHV; u++) {
+          const int v = vars.v[q0 + u < vars.n ? q0 + u : vars.n - 1];
+          val[u] = flipped(src_val(U, r, S, v), flip, v);
+        }
 #pragma unroll
         for (int u = 0; u < CHV; u++)
           if (q0 + u < vars.n) U[dst + vars.v[q0 + u] * m.cstride] = val[u];
@@ -184,18 +188,34 @@ __global__ void __launch_bounds__(256) k_amr_fill(DevParams P, const BlockMeta* 
       sm[d] = amr_src(meta, G, g, L - 1, gm);
       sp[d] = amr_src(meta, G, g, L - 1, gp);
     }
-    for (int q = 0; q < vars.n; q++) {
-      const int v = vars.v[q];
-      const double c = src_val(U, c0, S, v);
-      double val = c;
+    // two variables at a time: their 14 coarse loads issue before the
+    // first slope waits (a slot past vars.n re-reads the last variable)
+    for (int q0 = 0; q0 < vars.n; q0 += 2) {
+      double cc[2], cmv[2][3], cpv[2][3];
 #pragma unroll
-      for (int d = 0; d < 3; d++) {
-        const double cm = flipped(src_val(U, sm[d], S, v), fm[d], v);
-        const double cp = flipped(src_val(U, sp[d], S, v), fp[d], v);
-        const double sl = mc_slope(c - cm, cp - c);
-        val = val + sl * ((gi[d] & 1) ? 0.25 : -0.25);
+      for (int h = 0; h < 2; h++) {
+        const int v = vars.v[q0 + h < vars.n ? q0 + h : vars.n - 1];
+        cc[h] = src_val(U, c0, S, v);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          cmv[h][d] = flipped(src_val(U, sm[d], S, v), fm[d], v);
+          cpv[h][d] = flipped(src_val(U, sp[d], S, v), fp[d], v);
+        }
       }
-      U[dst + v * m.cstride] = flipped(val, flip, v);
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        if (q0 + h < vars.n) {
+          const int v = vars.v[q0 + h];
+          const double c = cc[h];
+          double val = c;
+#pragma unroll
+          for (int d = 0; d < 3; d++) {
+            const double sl = mc_slope(c - cmv[h][d], cpv[h][d] - c);
+            val = val + sl * ((gi[d] & 1) ? 0.25 : -0.25);
+          }
+          U[dst + v * m.cstride] = flipped(val, flip, v);
+        }
+      }
     }
   }
 }
@@ -287,7 +307,18 @@ __global__ void __launch_bounds__(256) k_halo_copy(const BlockMeta* __restrict__
     if ((ox | oy | oz) == 0) continue;
     const int64_t src = row[13 + ox + 3 * oy + 9 * oz] + (i - ox * m.nx) + sy * (j - oy * m.ny) + sz * (k - oz * m.nz);
     const int64_t dst = m.base + i + sy * j + sz * k;
-    for (int q = 0; q < vars.n; q++) U[dst + vars.v[q] * m.cstride] = U[src + vars.v[q] * m.cstride];
+    // the loads of a chunk of variables issue before its stores (a store to
+    // U could alias a later load, so the compiler keeps them in order); a
+    // slot past vars.n re-reads the last variable
+    constexpr int CHV = 8;
+    for (int q0 = 0; q0 < vars.n; q0 += CHV) {
+      double val[CHV];
+#pragma unroll
+      for (int u = 0; u < CHV; u++) val[u] = U[src + vars.v[q0 + u < vars.n ? q0 + u : vars.n - 1] * m.cstride];
+#pragma unroll
+      for (int u = 0; u < CHV; u++)
+        if (q0 + u < vars.n) U[dst + vars.v[q0 + u] * m.cstride] = val[u];
+    }
   }
 }
 

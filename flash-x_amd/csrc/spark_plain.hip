@@ -48,7 +48,10 @@ __device__ __forceinline__ int64_t fix(const BlockMeta& m, int i, int j, int k) 
 }
 
 // ---------------------------------------------------------------------------
-// flattening over limits +/- 1 (hy_rk_getFaceFlux.F90-mc:200-276)
+// flattening over limits +/- 1 (hy_rk_getFaceFlux.F90-mc:200-276); the
+// dimension count is a template parameter, so the loads of all directions
+// issue before the first of them is used
+template <int NDIM>
 __global__ void __launch_bounds__(256) k_flatten(DevParams P, const BlockMeta* __restrict__ meta,
                                                  const int* __restrict__ blist, const double* __restrict__ S,
                                                  double* __restrict__ flat, int ngoff) {
@@ -56,7 +59,7 @@ __global__ void __launch_bounds__(256) k_flatten(DevParams P, const BlockMeta* _
   xcd_chunk(cx, cy);
   const BlockMeta m = meta[blist[cy]];
   Lims L = stage_lims(P, m, ngoff);
-  const int kx = 1, ky = (P.ndim > 1), kz = (P.ndim == 3);
+  const int kx = 1, ky = (NDIM > 1), kz = (NDIM == 3);
   const int lo0 = L.lo[0] - kx, lo1 = L.lo[1] - ky, lo2 = L.lo[2] - kz;
   const int ex = L.hi[0] + kx - lo0 + 1, ey = L.hi[1] + ky - lo1 + 1, ez = L.hi[2] + kz - lo2 + 1;
   const int total = ex * ey * ez;
@@ -68,16 +71,28 @@ __global__ void __launch_bounds__(256) k_flatten(DevParams P, const BlockMeta* _
   for (int t = cx * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
     int i = lo0 + t % ex, j = lo1 + (t / ex) % ey, k = lo2 + t / (ex * ey);
     int64_t c = cix(m, i, j, k);
-    double mn = 0.;
-    for (int d = 0; d < P.ndim; d++) {
+    const double* p = sp + V_PRES * cs;
+    const double p0 = p[c], g0 = sp[V_GAMC * cs + c];
+    double pm2[3], pm1[3], pp1[3], pp2[3], u0[3], up1[3];
+#pragma unroll
+    for (int d = 0; d < NDIM; d++) {
       const int64_t s = str[d];
-      const double* p = sp + V_PRES * cs;
-      double dp1 = __builtin_fabs(p[c + s] - p[c - s]);
-      double beta = dp1 / fmax2(Num<double>::flat_floor, __builtin_fabs(p[c + 2 * s] - p[c - 2 * s]));
-      double Z = dp1 / (sp[V_GAMC * cs + c] * p[c]);
-      double ft = fmax2(0., fmin2(1., (betaMax - beta) / (betaMax - betaMin)));
       const double* u = sp + velv[d] * cs;
-      if (u[c + s] < u[c]) ft = fmax2(ft, fmin2(1., (Zmax - Z) / (Zmax - Zmin)));
+      pm2[d] = p[c - 2 * s];
+      pm1[d] = p[c - s];
+      pp1[d] = p[c + s];
+      pp2[d] = p[c + 2 * s];
+      u0[d] = u[c];
+      up1[d] = u[c + s];
+    }
+    double mn = 0.;
+#pragma unroll
+    for (int d = 0; d < NDIM; d++) {
+      double dp1 = __builtin_fabs(pp1[d] - pm1[d]);
+      double beta = dp1 / fmax2(Num<double>::flat_floor, __builtin_fabs(pp2[d] - pm2[d]));
+      double Z = dp1 / (g0 * p0);
+      double ft = fmax2(0., fmin2(1., (betaMax - beta) / (betaMax - betaMin)));
+      if (up1[d] < u0[d]) ft = fmax2(ft, fmin2(1., (Zmax - Z) / (Zmax - Zmin)));
       else ft = 1.0;
       mn = (d == 0) ? ft : fmin2(mn, ft);
     }
@@ -946,8 +961,13 @@ hipError_t plain_stage(const PlainLaunch& L, int ngoff, double c1, double c2, do
       L.prof->begin(L.stream, "k_flatten", 8.0 * L.nblk * ((double)bxr * (2 + L.P.ndim) + bx),
                     (double)L.nblk * bx * 25.0 * L.P.ndim);
     }
-    hipLaunchKernelGGL(k_flatten, dim3(grid_x_for(boxf, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta, L.blist,
-                       L.Sin, L.flat, ngoff);
+    const dim3 fg(grid_x_for(boxf, 512), L.nblk), fb(256);
+    if (L.P.ndim == 3)
+      hipLaunchKernelGGL(k_flatten<3>, fg, fb, 0, L.stream, L.P, L.meta, L.blist, L.Sin, L.flat, ngoff);
+    else if (L.P.ndim == 2)
+      hipLaunchKernelGGL(k_flatten<2>, fg, fb, 0, L.stream, L.P, L.meta, L.blist, L.Sin, L.flat, ngoff);
+    else
+      hipLaunchKernelGGL(k_flatten<1>, fg, fb, 0, L.stream, L.P, L.meta, L.blist, L.Sin, L.flat, ngoff);
     if (L.prof) L.prof->end(L.stream);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
