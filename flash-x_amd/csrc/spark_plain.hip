@@ -109,9 +109,10 @@ __global__ void __launch_bounds__(256) k_flatten(DevParams P, const BlockMeta* _
 // that each XCD runs a contiguous range of (pencil chunk, plane, block)
 // triples (xcd_chunk), so neighbouring chunks, which share cache lines at
 // their edges, share an L2.  The 2R+1 stencil values come straight from
-// global memory (L1/L2 serve the repeats): staging them in LDS, one
-// variable per barrier, was measured slower (the kernel is bound by FP64
-// VALU issue, not by its loads).
+// global memory (L1/L2 serve the repeats), the next variable's while the
+// current one is reconstructed: staging them in LDS, one variable per
+// barrier, was measured slower (the kernel is bound by FP64 VALU issue,
+// not by its loads).
 template <int DIR, int RECON, int RIEMANN, int NSM>
 __global__ void __launch_bounds__(1024) k_flux(DevParams P, const BlockMeta* __restrict__ meta,
                                                const int* __restrict__ blist, const double* __restrict__ S,
@@ -152,50 +153,47 @@ __global__ void __launch_bounds__(1024) k_flux(DevParams P, const BlockMeta* __r
   double umr[NRM];
   if (valid) {
     const double fl_c = (RECON == RECON_WENO5 && P.use_flat) ? flat[m.obase + c] : 1.0;
-    double ud[5], u[5], up, um;
+    // variables in order dens, velx, vely, velz, pres, gamc, eint (rhoe =
+    // dens * eint per cell), the scalars; the stencil of variable v+1 is
+    // loaded before variable v is reconstructed (software pipelining: the
+    // scheduling barrier keeps the loads ahead of the reconstruction)
+    constexpr int NV = R_NUM + NSM;
+    const int nv = R_NUM + ns;
+    auto var_off = [&](int v) -> int64_t {
+      const int sv[7] = {V_DENS, V_VELX, V_VELY, V_VELZ, V_PRES, V_GAMC, V_EINT};
+      return (int64_t)(v < R_NUM ? sv[v < R_NUM ? v : 0] : V_NPROP + (v - R_NUM < ns ? v - R_NUM : ns - 1)) * cs;
+    };
+    double ud[5], cur[5], nxt[5], up, um;
 #pragma unroll
-    for (int q = -R; q <= R; q++) ud[q + 2] = sp[V_DENS * cs + c + q * st];
-    recon1<RECON>(ud, fl_c, up, um);
-    sm[R_DENS * NT + tid] = up;
-    umr[R_DENS] = um;
-    const int svar[5] = {V_VELX, V_VELY, V_VELZ, V_PRES, V_GAMC};
+    for (int q = 0; q < 5; q++) nxt[q] = (q >= 2 - R && q <= 2 + R) ? sp[var_off(0) + c + (q - 2) * st] : 0.0;
 #pragma unroll
-    for (int v = 0; v < 5; v++) {
+    for (int v = 0; v < NV; v++) {
+      if (v < nv) {
 #pragma unroll
-      for (int q = -R; q <= R; q++) u[q + 2] = sp[svar[v] * cs + c + q * st];
-      recon1<RECON>(u, fl_c, up, um);
-      sm[(1 + v) * NT + tid] = up;
-      umr[1 + v] = um;
-      if (v < 3) sm[(NRM + 1 + v) * NT + tid] = u[2];
-    }
+        for (int q = 0; q < 5; q++) cur[q] = nxt[q];
+        if (v + 1 < NV) {
 #pragma unroll
-    for (int q = -R; q <= R; q++) u[q + 2] = ud[q + 2] * sp[V_EINT * cs + c + q * st];
-    recon1<RECON>(u, fl_c, up, um);
-    sm[R_RHOE * NT + tid] = up;
-    umr[R_RHOE] = um;
-    sm[(NRM + 4) * NT + tid] = u[2];
-    sm[(NRM + 0) * NT + tid] = ud[2];
-    sm[(NRM + 5) * NT + tid] = Ushok[m.base + V_SHOK * cs + c];
-#pragma unroll
-    for (int v0 = 0; v0 < NSM; v0 += 2) {
-      // the stencils of two scalars load before the first reconstruction
-      // waits (a slot past ns re-reads scalar ns-1)
-      double u2[2][5];
-#pragma unroll
-      for (int h = 0; h < 2; h++) {
-        const int v = v0 + h < ns ? v0 + h : ns - 1;
-#pragma unroll
-        for (int q = 0; q < 5; q++) u2[h][q] = (q >= 2 - R && q <= 2 + R) ? sp[(V_NPROP + v) * cs + c + (q - 2) * st] : 0.0;
-      }
-#pragma unroll
-      for (int h = 0; h < 2; h++) {
-        if (v0 + h < ns) {
-          recon1<RECON>(u2[h], fl_c, up, um);
-          sm[(R_NUM + v0 + h) * NT + tid] = up;
-          umr[R_NUM + v0 + h] = um;
+          for (int q = 0; q < 5; q++)
+            nxt[q] = (q >= 2 - R && q <= 2 + R) ? sp[var_off(v + 1) + c + (q - 2) * st] : 0.0;
         }
+        __builtin_amdgcn_sched_barrier(0);
+        if (v == R_DENS) {
+#pragma unroll
+          for (int q = 0; q < 5; q++) ud[q] = cur[q];
+        }
+        if (v == R_RHOE) {
+#pragma unroll
+          for (int q = -R; q <= R; q++) cur[q + 2] = ud[q + 2] * cur[q + 2];
+        }
+        recon1<RECON>(cur, fl_c, up, um);
+        sm[v * NT + tid] = up;
+        umr[v] = um;
+        if (v == R_DENS) sm[(NRM + 0) * NT + tid] = cur[2];
+        else if (v <= R_VELZ) sm[(NRM + v) * NT + tid] = cur[2];
+        else if (v == R_RHOE) sm[(NRM + 4) * NT + tid] = cur[2];
       }
     }
+    sm[(NRM + 5) * NT + tid] = Ushok[m.base + V_SHOK * cs + c];
   }
   __syncthreads();
   if (!valid || s < 1) return;
