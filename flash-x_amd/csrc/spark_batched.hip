@@ -253,10 +253,27 @@ __device__ __forceinline__ void centred(const Real* __restrict__ plane, int psz,
   c[4] = plane[SL_RHOE * psz + pc];
 }
 
-// flux-register accumulation of one face (addFluxes, Hydro_funcs.F90-mc:175-201)
-__device__ __forceinline__ void reg_add(double* fb, int64_t o, int64_t tot, int n, double weight, int add, Real f) {
-  const double w = weight * (double)f;
-  fb[o + n * tot] = add ? fb[o + n * tot] + w : w;
+// flux-register accumulation of one face (addFluxes, Hydro_funcs.F90-mc:175-201):
+// entries n0 .. n0+cnt-1 (cnt <= N) of the face take weight * f[i]; with
+// add, every old entry is loaded before the first store (the compiler
+// cannot tell the entries apart, so a load after a store would wait for it)
+template <int N>
+__device__ __forceinline__ void reg_add_n(double* fb, int64_t o, int64_t tot, int n0, int cnt, double weight, int add,
+                                          const Real* f, int fstride = 1) {
+  double w[N], prev[N];
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    w[i] = weight * (double)f[i * fstride];
+    prev[i] = 0.0;
+  }
+  if (add) {
+#pragma unroll
+    for (int i = 0; i < N; i++)
+      if (i < cnt) prev[i] = fb[o + (n0 + i) * tot];
+  }
+#pragma unroll
+  for (int i = 0; i < N; i++)
+    if (i < cnt) fb[o + (n0 + i) * tot] = add ? prev[i] + w[i] : w[i];
 }
 
 // x (DIR 0) or y (DIR 1) face w of the tile at level k: states from the
@@ -295,8 +312,7 @@ __device__ __forceinline__ void xy_face(const FusedArgs& A, const BlockMeta& m, 
       const int ex = m.nx + (DIR == 0), ey = m.ny + (DIR == 1);
       const int64_t tot = (int64_t)ex * ey * m.nz;
       const int64_t o = m.bbase + (i - g) + (int64_t)ex * ((j - g) + (int64_t)ey * (k - g));
-#pragma unroll
-      for (int n = 0; n < F_NUM; n++) reg_add(A.fb[DIR], o, tot, n, A.weight, A.add_flux, F[n]);
+      reg_add_n<F_NUM>(A.fb[DIR], o, tot, 0, F_NUM, A.weight, A.add_flux, F);
     }
   }
 }
@@ -576,8 +592,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
               if (regs && colin && (k == g || k == g + m.nz)) {
                 const int64_t tot = (int64_t)m.nx * m.ny * (m.nz + 1);
                 const int64_t o = m.bbase + (ci - g) + (int64_t)m.nx * ((cj - g) + (int64_t)m.ny * (k - g));
-#pragma unroll
-                for (int n = 0; n < F_NUM; n++) reg_add(A.fb[2], o, tot, n, A.weight, A.add_flux, Hz[n]);
+                reg_add_n<F_NUM>(A.fb[2], o, tot, 0, F_NUM, A.weight, A.add_flux, Hz);
               }
             }
             continue;
@@ -752,9 +767,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
             if (regs && colin && (k == g || k == g + m.nz)) {
               const int64_t tot = (int64_t)m.nx * m.ny * (m.nz + 1);
               const int64_t o = m.bbase + (ci - g) + (int64_t)m.nx * ((cj - g) + (int64_t)m.ny * (k - g));
-#pragma unroll
-              for (int j = 0; j < CH; j++)
-                if (j < nc) reg_add(A.fb[2], o, tot, F_NUM + c0 + j, A.weight, A.add_flux, Hs[j]);
+              reg_add_n<CH>(A.fb[2], o, tot, F_NUM + c0, nc, A.weight, A.add_flux, Hs);
             }
           }
           if (xyk) {
@@ -800,8 +813,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
                     const int ex = m.nx + 1, ey = m.ny;
                     const int64_t tot = (int64_t)ex * ey * m.nz;
                     const int64_t o = m.bbase + (i - g) + (int64_t)ex * ((jj - g) + (int64_t)ey * (k - g));
-                    for (int j = 0; j < nc; j++)
-                      reg_add(A.fb[0], o, tot, F_NUM + c0 + j, A.weight, A.add_flux, fo[j * nf + w]);
+                    reg_add_n<CH>(A.fb[0], o, tot, F_NUM + c0, nc, A.weight, A.add_flux, fo + w, nf);
                   }
                 } else {
                   if ((jj == g ? fj < ty : fj > 0) && i >= g && i < g + m.nx && (jj == g || jj == g + m.ny) &&
@@ -809,8 +821,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
                     const int ex = m.nx, ey = m.ny + 1;
                     const int64_t tot = (int64_t)ex * ey * m.nz;
                     const int64_t o = m.bbase + (i - g) + (int64_t)ex * ((jj - g) + (int64_t)ey * (k - g));
-                    for (int j = 0; j < nc; j++)
-                      reg_add(A.fb[1], o, tot, F_NUM + c0 + j, A.weight, A.add_flux, fo[j * nf + w]);
+                    reg_add_n<CH>(A.fb[1], o, tot, F_NUM + c0, nc, A.weight, A.add_flux, fo + w, nf);
                   }
                 }
               }
@@ -821,12 +832,22 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
           if (upd) {
             const int64_t c = colc + (int64_t)(k - 1) * sz;
             double Xn[8] = {0., 0., 0., 0., 0., 0., 0., 0.};  // updated species of the first chunk
+            // the chunk's U0 and stage-input loads issue before the first
+            // store (which the compiler cannot tell apart from them); a slot
+            // past nc re-reads scalar nc-1
+            double a4[CH], b4[CH];
+#pragma unroll
+            for (int j = 0; j < CH; j++) {
+              const int64_t o = (int64_t)(V_NPROP + c0 + (j < nc ? j : nc - 1)) * cs + c;
+              a4[j] = u0p[o];
+              b4[j] = sin[o];
+            }
 #pragma unroll
             for (int j = 0; j < CH; j++) {
               if (j < nc) {
                 const int n = F_NUM + c0 + j;
-                const double Au = u0p[(V_NPROP + c0 + j) * cs + c] * d0;
-                double Bu = sin[(V_NPROP + c0 + j) * cs + c] * ds;
+                const double Au = a4[j] * d0;
+                double Bu = b4[j] * ds;
                 double inner_n = dUxy[n] - dtdz * ((double)Hs[j] - (double)Hprev[n]);
                 inner_n = inner_n + dt * (0.0 + 0.0);
                 Bu = A.c1 * Au + A.c2 * Bu + A.c3 * inner_n;

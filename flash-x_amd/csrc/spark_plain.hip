@@ -9,6 +9,7 @@
 // bit (IEEE division and square root on both sides).
 #include "spark_physics.hpp"
 #include "spark_launch.hpp"
+#include "spark_regions.hpp"
 
 namespace spark {
 
@@ -291,11 +292,21 @@ __global__ void __launch_bounds__(1024) k_flux(DevParams P, const BlockMeta* __r
       e[DIR] += 1;
       const int64_t bstride = (int64_t)e[0] * e[1] * e[2];
       const int64_t o = m.bbase + fi[0] + (int64_t)e[0] * (fi[1] + (int64_t)e[1] * fi[2]);
+      // every old entry is loaded before the first store (the compiler
+      // cannot tell the entries apart, so a load after a store would wait
+      // for it); a slot past F_NUM + ns re-reads the last entry
+      double prev[NFM];
+#pragma unroll
+      for (int n = 0; n < NFM; n++) prev[n] = 0.0;
+      if (add) {
+#pragma unroll
+        for (int n = 0; n < NFM; n++) prev[n] = fb[o + (n < F_NUM + ns ? n : F_NUM + ns - 1) * bstride];
+      }
 #pragma unroll
       for (int n = 0; n < NFM; n++) {
         if (n < F_NUM + ns) {
           const double w = weight * F[n];
-          fb[o + n * bstride] = add ? fb[o + n * bstride] + w : w;
+          fb[o + n * bstride] = add ? prev[n] + w : w;
         }
       }
     }
@@ -751,23 +762,26 @@ __global__ void __launch_bounds__(256) k_ug_fill(DevParams P, const BlockMeta* _
 // the state is read and written once.  Lanes map to consecutive x cells.
 // With a face mask (bit 2d+s: low / high face of dimension d), only cells
 // on a masked face are corrected (each through all of its faces).
+template <int NSM>
 __global__ void __launch_bounds__(256) k_correct(DevParams P, const BlockMeta* __restrict__ meta,
                                                  const int* __restrict__ blist, double* __restrict__ U,
                                                  const double* __restrict__ fbx, const double* __restrict__ fby,
                                                  const double* __restrict__ fbz, const double* __restrict__ dtp,
                                                  const unsigned char* __restrict__ fmask) {
+  constexpr int NFM = F_NUM + NSM;
   const int bid = blist[blockIdx.y];
   const BlockMeta m = meta[bid];
   Lims L = stage_lims(P, m, 0);
   const int n[3] = {L.hi[0] - L.lo[0] + 1, L.hi[1] - L.lo[1] + 1, L.hi[2] - L.lo[2] + 1};
-  const int total = n[0] * n[1] * n[2];
-  const int nf = F_NUM + P.ns, ns = P.ns, nd = P.ndim;
+  const int ns = NSM > 0 ? P.ns : 0, nd = P.ndim;
+  const int nb = boundary_count(nd, n);  // the interior cells get no thread
   const int mask = fmask ? fmask[bid] : 0x3f;
   const double dt = *dtp;
   const int64_t cs = m.cstride;
   double* up = U + m.base;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-    const int c[3] = {t % n[0], (t / n[0]) % n[1], t / (n[0] * n[1])};
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nb; t += gridDim.x * blockDim.x) {
+    int c[3];
+    boundary_cell(nd, n, t, c);
     bool edge = false;
     for (int d = 0; d < nd; d++)
       edge = edge || (c[d] == 0 && (mask >> (2 * d) & 1)) || (c[d] == n[d] - 1 && (mask >> (2 * d + 1) & 1));
@@ -776,8 +790,9 @@ __global__ void __launch_bounds__(256) k_correct(DevParams P, const BlockMeta* _
     double dens = up[V_DENS * cs + cc];
     double vx = up[V_VELX * cs + cc], vy = up[V_VELY * cs + cc], vz = up[V_VELZ * cs + cc];
     double ener = up[V_ENER * cs + cc];
-    double X[8];  // at most 8 mass scalars (the plain kernels instantiate NSM 0..8)
-    for (int v = 0; v < ns; v++) X[v] = up[(V_NPROP + v) * cs + cc];
+    double X[NSM > 0 ? NSM : 1];  // mass scalars (a slot past ns re-reads scalar ns-1)
+#pragma unroll
+    for (int v = 0; v < NSM; v++) X[v] = up[(V_NPROP + (v < ns ? v : ns - 1)) * cs + cc];
     EosOut o;
     for (int d = 0; d < nd; d++) {
       const double* fb = (d == 0) ? fbx : (d == 1) ? fby : fbz;
@@ -790,14 +805,19 @@ __global__ void __launch_bounds__(256) k_correct(DevParams P, const BlockMeta* _
         int f[3] = {c[0], c[1], c[2]};
         if (side) f[d] += 1;
         const int64_t bo = m.bbase + (int64_t)f[0] + (int64_t)e[0] * ((int64_t)f[1] + (int64_t)e[1] * f[2]);
-        double Us[F_NUM + 8];
+        // the face's register entries, all loaded before the first is used
+        double b[NFM];
+#pragma unroll
+        for (int q = 0; q < NFM; q++) b[q] = fb[bo + (q < F_NUM + ns ? q : F_NUM + ns - 1) * bstride];
+        double Us[NFM];
         const double d0 = dens;
         Us[F_MASS] = d0;
         Us[F_XMOM] = d0 * vx;
         Us[F_YMOM] = d0 * vy;
         Us[F_ZMOM] = d0 * vz;
         Us[F_ENER] = d0 * ener;
-        for (int v = 0; v < ns; v++) Us[F_NUM + v] = X[v] * d0;
+#pragma unroll
+        for (int v = 0; v < NSM; v++) Us[F_NUM + v] = X[v] * d0;
         // geoFacs of the x faces (hy_rk_correctFluxes.F90:137-168, 341-358)
         double fac = 1.0;
         if (d == 0 && P.geom != 0) {
@@ -805,9 +825,9 @@ __global__ void __launch_bounds__(256) k_correct(DevParams P, const BlockMeta* _
           geo_facs(P, m.id, L.lo[0] + c[0], L.lo[1] + c[1], m.dx, facM, facP, rc);
           fac = side ? facP : facM;
         }
-        for (int q = 0; q < nf; q++) {
-          const double b = fb[bo + q * bstride];
-          const double dF = side ? b : -b;
+#pragma unroll
+        for (int q = 0; q < NFM; q++) {
+          const double dF = side ? b[q] : -b[q];
           Us[q] = Us[q] - dt / del * (fac * dF);
         }
         dens = fmax2(Us[F_MASS], P.smlrho);
@@ -817,12 +837,15 @@ __global__ void __launch_bounds__(256) k_correct(DevParams P, const BlockMeta* _
         const double en = fmax2(P.smalle, Us[F_ENER] / dens);
         const double ekin = .5 * (vx * vx + vy * vy + vz * vz);
         const double eint = fmax2(P.smalle, en - ekin);
-        for (int v = 0; v < ns; v++) X[v] = Us[F_NUM + v] / dens;
+#pragma unroll
+        for (int v = 0; v < NSM; v++) X[v] = Us[F_NUM + v] / dens;
         o = eos_dens_ei(P, dens, vx, vy, vz, en, eint, X);
         ener = o.ener;
       }
     }
-    for (int v = 0; v < ns; v++) up[(V_NPROP + v) * cs + cc] = X[v];
+#pragma unroll
+    for (int v = 0; v < NSM; v++)
+      if (v < ns) up[(V_NPROP + v) * cs + cc] = X[v];
     up[V_DENS * cs + cc] = dens;
     up[V_VELX * cs + cc] = vx;
     up[V_VELY * cs + cc] = vy;
@@ -1092,14 +1115,20 @@ hipError_t plain_ug_fill(const PlainLaunch& L, int nbx, int nby, int nbz, double
 
 hipError_t plain_correct(const PlainLaunch& L, double* U, const unsigned char* fmask) {
   const int total = L.nx * (L.P.ndim > 1 ? L.ny : 1) * (L.P.ndim > 2 ? L.nz : 1);
+  const int n3[3] = {L.nx, L.P.ndim > 1 ? L.ny : 1, L.P.ndim > 2 ? L.nz : 1};
   int inner = 1;
   const int n[3] = {L.nx, L.ny, L.nz};
   for (int d = 0; d < 3; d++) inner *= (d < L.P.ndim && n[d] > 2) ? n[d] - 2 : (d < L.P.ndim ? 0 : 1);
   // boundary cells: read 5+ns state fields and the registers, write 10+ns
   const double cells = (double)(total - inner) * L.nblk, nf = 5.0 + L.P.ns;
   if (L.prof) L.prof->begin(L.stream, "k_correct", 8.0 * cells * (nf + nf + 10.0 + L.P.ns), cells * (nf * 4 + 40.0));
-  hipLaunchKernelGGL(k_correct, dim3(grid_x_for(total, 64), L.nblk), dim3(256), 0, L.stream, L.P, L.meta, L.blist, U,
-                     L.fb[0], L.fb[1], L.fb[2], L.dtp, fmask);
+  const int nbc = boundary_count(L.P.ndim, n3);
+  if (L.P.ns == 0)
+    hipLaunchKernelGGL(k_correct<0>, dim3(grid_x_for(nbc, 64), L.nblk), dim3(256), 0, L.stream, L.P, L.meta, L.blist,
+                       U, L.fb[0], L.fb[1], L.fb[2], L.dtp, fmask);
+  else
+    hipLaunchKernelGGL(k_correct<8>, dim3(grid_x_for(nbc, 64), L.nblk), dim3(256), 0, L.stream, L.P, L.meta, L.blist,
+                       U, L.fb[0], L.fb[1], L.fb[2], L.dtp, fmask);
   if (L.prof) L.prof->end(L.stream);
   return hipGetLastError();
 }

@@ -1,12 +1,60 @@
 // spark_regions.hpp -- stage regions of the batched forms as boxes of a
-// block's guard-celled index space (host code, no device dependence).
+// block's guard-celled index space (host code), and the boundary cells of a
+// box (host and device).
 #pragma once
 #include <stdint.h>
 
 #include <utility>
 #include <vector>
 
+// the boundary-cell enumeration below is also used by device code
+#if defined(__HIPCC__)
+#define SPARK_HD __host__ __device__
+#else
+#define SPARK_HD
+#endif
+
 namespace spark {
+
+// Boundary cells of an n[0] x n[1] x n[2] box of nd dimensions (n[d] = 1 for
+// d >= nd), enumerated without the interior: first the outer layer(s) of
+// dimension nd-1 over the whole cross section, then those of dimension nd-2
+// over the cross section less the layers already counted, and so on; a
+// dimension of one cell has one layer, otherwise two.  Within a layer the
+// first dimension runs fastest.
+SPARK_HD inline int boundary_layer_count(int nd, const int n[3], int d, int* layers, int* rest) {
+  *layers = n[d] > 1 ? 2 : 1;
+  int r = 1;
+  for (int e = 0; e < nd; e++)
+    if (e != d) r *= e < d ? n[e] : (n[e] > 2 ? n[e] - 2 : 0);
+  *rest = r;
+  return *layers * r;
+}
+
+SPARK_HD inline int boundary_count(int nd, const int n[3]) {
+  int tot = 0, l, r;
+  for (int d = nd - 1; d >= 0; d--) tot += boundary_layer_count(nd, n, d, &l, &r);
+  return tot;
+}
+
+// cell c[0..2] (0-based in the box) of boundary cell t, 0 <= t < boundary_count
+SPARK_HD inline void boundary_cell(int nd, const int n[3], int t, int c[3]) {
+  c[0] = c[1] = c[2] = 0;
+  for (int d = nd - 1; d >= 0; d--) {
+    int l, r;
+    const int cnt = boundary_layer_count(nd, n, d, &l, &r);
+    if (t >= cnt) { t -= cnt; continue; }
+    c[d] = (t / r) == 0 ? 0 : n[d] - 1;
+    int q = t % r;
+    for (int e = 0; e < nd; e++) {
+      if (e == d) continue;
+      const int m = e < d ? n[e] : n[e] - 2;
+      c[e] = (e < d ? 0 : 1) + q % m;
+      q /= m;
+    }
+    return;
+  }
+}
 
 // a box of a block's guard-celled index space, bounds inclusive
 struct Box {
