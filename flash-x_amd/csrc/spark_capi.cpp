@@ -344,12 +344,13 @@ int run_step_begin(spark_batch* b) {
   // SHOK over the cells the stages' hybrid Riemann solves read: the stage-1
   // region plus one layer (the reference computes NGUARD-1 layers; the
   // outer layers are never read before the next refill).  Without the
-  // hybrid solver the batched forms read SHOK nowhere and compute it for
-  // the interior, where it is part of the solution.  The batched forms read
-  // SHOK from the block's own box, so remapping blocks (no guard cells of
-  // their own in the stages) get it one layer out.
+  // hybrid solver no form reads SHOK, so it is computed for the interior,
+  // where it is part of the solution; guard-layer SHOK, which only the
+  // hybrid solver reads, is left as it was.  The batched forms read SHOK
+  // from the block's own box, so remapping blocks (no guard cells of their
+  // own in the stages) get it one layer out.
   const bool hyb = b->p.use_hybrid_riemann != 0;
-  const int band = b->fused && !hyb ? 0 : (b->p.guard_mode == SPARK_GUARD_REFILL ? 0 : b->limits_array[0]) + 1;
+  const int band = !hyb ? 0 : (b->p.guard_mode == SPARK_GUARD_REFILL ? 0 : b->limits_array[0]) + 1;
   if (amr_remap_on(b)) {
     // a hierarchy has one block shape
     PlainLaunch L = make_launch(b, b->classes[0]);
