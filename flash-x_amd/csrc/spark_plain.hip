@@ -7,8 +7,12 @@
 // Built with -ffp-contract=off: every FP64 expression rounds exactly as the
 // Fortran line it restates, so results match the CPU restatement bit for
 // bit (IEEE division and square root on both sides).
-#include "spark_physics.hpp"
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "spark_launch.hpp"
+#include "spark_physics.hpp"
 #include "spark_regions.hpp"
 
 namespace spark {
@@ -133,19 +137,21 @@ __global__ void __launch_bounds__(1024) k_flux(DevParams P, const BlockMeta* __r
   constexpr int T2 = (DIR == 2) ? 1 : 2;
   const int Lr = L.hi[DIR] - L.lo[DIR] + 3;
   const int nt1 = L.hi[T1] - L.lo[T1] + 1;
-  const int chunks = (nt1 + lanes - 1) / lanes;
-  const int c1 = bx % chunks, c2 = bx / chunks;
+  const int npen = nt1 * (L.hi[T2] - L.lo[T2] + 1);
   const int NT = blockDim.x;
   const int tid = threadIdx.x;
   int s, a;
   if (DIR == 0) { s = tid % Lr; a = tid / Lr; }
   else { a = tid % lanes; s = tid / lanes; }
-  const int t1c = L.lo[T1] + c1 * lanes + a;
-  const bool valid = (a < lanes) && (t1c <= L.hi[T1]) && (s < Lr);
+  // pencils are numbered over the cross-section, T1 fastest, and dealt to
+  // workgroups `lanes` at a time, so a workgroup's pencils may continue into
+  // the next plane and no chunk of a plane is left mostly empty
+  const int pen = bx * lanes + a;
+  const bool valid = (a < lanes) && (pen < npen) && (s < Lr);
   int ijk[3];
   ijk[DIR] = L.lo[DIR] - 1 + s;
-  ijk[T1] = t1c;
-  ijk[T2] = L.lo[T2] + c2;
+  ijk[T1] = L.lo[T1] + pen % nt1;
+  ijk[T2] = L.lo[T2] + pen / nt1;
   const int64_t c = cix(m, ijk[0], ijk[1], ijk[2]);
   const int64_t st = (DIR == 0) ? 1 : (DIR == 1) ? (int64_t)m.NX : (int64_t)m.NX * m.NY;
   const double* sp = S + m.base;
@@ -327,7 +333,6 @@ __global__ void __launch_bounds__(256) k_update(DevParams P, const BlockMeta* __
                                                 const double* __restrict__ fly, const double* __restrict__ flz,
                                                 const double* __restrict__ dtp, double c1, double c2, double c3,
                                                 int ngoff, int final_stage) {
-  constexpr int NFM = F_NUM + NSM;
   int cx, cy;
   xcd_chunk(cx, cy);
   const BlockMeta m = meta[blist[cy]];
@@ -897,6 +902,22 @@ static inline int grid_x_for(int total, int cap) {
   return g < 1 ? 1 : (g > cap ? cap : g);
 }
 
+// waves of one kernel resident on a CU at a block size and dynamic LDS
+// size (the runtime's occupancy calculator, cached per kernel and shape)
+static int resident_waves(const void* kern, int threads, size_t shm) {
+  static std::mutex mu;
+  static std::map<std::tuple<const void*, int, size_t>, int> cache;
+  std::lock_guard<std::mutex> g(mu);
+  const auto key = std::make_tuple(kern, threads, shm);
+  const auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, threads, shm) != hipSuccess || nb < 1) nb = 1;
+  const int res = nb * ((threads + 63) / 64);
+  cache.emplace(key, res);
+  return res;
+}
+
 template <int DIR, int RECON, int RIEMANN, int NSM>
 static hipError_t launch_flux_t(const PlainLaunch& L, int ngoff) {
   // geometry of this stage (all blocks in the list share one shape)
@@ -910,22 +931,45 @@ static hipError_t launch_flux_t(const PlainLaunch& L, int ngoff) {
   const int Lr = hi[DIR] - lo[DIR] + 3;
   const int nt1 = hi[T1] - lo[T1] + 1, nt2 = hi[T2] - lo[T2] + 1;
   constexpr int NRM = R_NUM + NSM;
-  // pencils per workgroup: the LDS exchange buffer stays within 80 KiB (two
-  // workgroups per CU) and the workgroup within 1024 threads; among those
-  // the count that launches the fewest waves per plane of pencils (an
-  // uneven split leaves a chunk of mostly idle lanes), the largest on
-  // ties.  y/z sweeps keep >= 16 x-columns per row where the plane has them,
-  // so each row of a wave's loads fills a 128-byte line.
+  // pencils per workgroup.  The LDS exchange buffer stays within 80 KiB and
+  // the workgroup within 1024 threads.  A workgroup of l pencils runs
+  // w = ceil(l Lr / 64) waves; workgroups of 2, 4 or 8 waves run best (3, 5,
+  // 6, 7 or 9 measured 5-45 % slower per launch in a sweep over pencil
+  // counts), and y/z sweeps keep at least 8 x columns per row of loads.
+  // Among those counts: the fewest waves over the whole cross-section per
+  // wave resident on a CU (so a count whose workgroups leave wave slots of
+  // the CU empty loses), the largest count within 3 % of the best.  Without
+  // such a count, the fewest waves.
+  const int npen = nt1 * nt2;
   const int cap = 81920 / (int)(sizeof(double) * (NRM + 6) * Lr);
-  const int lmax = std::max(1, std::min(std::min(1024 / Lr, cap), nt1));
-  const int lmin = DIR == 0 ? 1 : std::min(std::min(16, nt1), lmax);
-  int lanes = lmax, best = 1 << 30;
-  for (int l = lmin; l <= lmax; l++) {
-    const int waves = ((nt1 + l - 1) / l) * ((l * Lr + 63) / 64);
-    if (waves <= best) { best = waves; lanes = l; }
+  const int lmax = std::max(1, std::min(std::min(1024 / Lr, cap), npen));
+  const int lmin = DIR == 0 ? 1 : std::min(8, lmax);
+  auto wg_waves = [&](int l) { return (l * Lr + 63) / 64; };
+  auto waves_of = [&](int l) { return ((npen + l - 1) / l) * wg_waves(l); };
+  auto shape_ok = [&](int l) {
+    const int w = wg_waves(l);
+    return w == 2 || w == 4 || w == 8;
+  };
+  // cost in units of 1/1000 wave per resident wave
+  auto cost = [&](int l) {
+    const int res = resident_waves(reinterpret_cast<const void*>(&k_flux<DIR, RECON, RIEMANN, NSM>), l * Lr,
+                                   sizeof(double) * (size_t)(NRM + 6) * l * Lr);
+    return (int64_t)waves_of(l) * 1000 / std::max(res, 1);
+  };
+  int64_t best = INT64_MAX;
+  for (int l = lmin; l <= lmax; l++)
+    if (shape_ok(l)) best = std::min(best, cost(l));
+  int lanes = lmax;
+  if (best < INT64_MAX) {
+    for (int l = lmax; l >= lmin; l--)
+      if (shape_ok(l) && cost(l) * 100 <= best * 103) { lanes = l; break; }
+  } else {
+    int fewest = 1 << 30;
+    for (int l = 1; l <= lmax; l++)
+      if (waves_of(l) <= fewest) { fewest = waves_of(l); lanes = l; }
   }
-  const int chunks = (nt1 + lanes - 1) / lanes;
-  dim3 grid(chunks * nt2, L.nblk);
+  const int chunks = (npen + lanes - 1) / lanes;
+  dim3 grid(chunks, L.nblk);
   dim3 block(lanes * Lr);
   const size_t shm = sizeof(double) * (size_t)(NRM + 6) * block.x;
   if (L.prof) {
