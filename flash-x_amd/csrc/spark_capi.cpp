@@ -96,10 +96,12 @@ struct spark_batch {
   bool step_open = false;
   int64_t max_cells = 0;
   spark::Prof prof;
-  // batched FP32-flux form: ping-pong stage buffers S (stage 1) and S2
+  // stage buffers: S (stage 1) and S2 (stage 2 of RK3); star: the output
+  // of the last stage run before the final one (spark_unpack which = 1)
+  double* S2 = nullptr;
+  double* star = nullptr;
   bool fused = false;
   bool fused64 = false;  // batched form in FP64 arithmetic
-  double* S2 = nullptr;
   int cus = 0;            // compute units of the device
   size_t lds_limit = 0;   // LDS bytes a workgroup may allocate
   int64_t* nbr_d = nullptr;  // uniform grid: 27 neighbour pool offsets per block
@@ -374,17 +376,16 @@ int run_step_begin(spark_batch* b) {
   return SPARK_OK;
 }
 
-// stage input/output buffers.  Plain form: U -> S -> S (in place, each
-// thread owns its cell) -> U.  Fused form: U -> S -> S2 -> U (tiles read
-// neighbours' cells, so input and output never alias).
+// stage input/output buffers: U -> S -> S2 -> U (RK3) or U -> S -> U
+// (RK2).  A stage reads its input around each cell while it writes its
+// output (the plain form's last sweep applies the update, the fused form's
+// tiles read neighbours' cells), so input and output never alias.
 const double* stage_in(const spark_batch* b, int stage) {
   if (stage == 1) return b->U;
-  if (!b->fused) return b->S;
   return stage == 2 ? b->S : b->S2;
 }
 double* stage_out(spark_batch* b, int stage) {
   if (stage == b->last_stage) return b->U;
-  if (!b->fused) return b->S;
   return stage == 1 ? b->S : b->S2;
 }
 
@@ -460,6 +461,7 @@ spark::FusedArgs fused_args(spark_batch* b, int stage) {
 // (updateState copies exactly the last stage's region, the interior).
 int run_stage(spark_batch* b, int stage, int ngoff) {
   const double* co = b->coeff[stage - 1];
+  if (stage != b->last_stage) b->star = stage_out(b, stage);
   if (b->fused) {
     if (ngoff != stage_ngoff(b, stage))
       return fail(SPARK_ERR_ARG, "batched form: stage %d runs with ngoff %d", stage, stage_ngoff(b, stage));
@@ -972,7 +974,12 @@ int spark_batch_create(const spark_params* p, int nblocks, const spark_block_des
   ALLOC(b->U, sizeof(double) * b->pool);
   ALLOC(b->S, sizeof(double) * b->pool);
   ALLOC(b->flat, sizeof(double) * b->opool);
-  for (int q = 0; q < 3; q++) ALLOC(b->fl[q], sizeof(double) * b->fpool);
+  ALLOC(b->S2, sizeof(double) * b->pool);
+  // face-flux pools: the plain form's sweeps but the last (which applies
+  // the update from the earlier ones)
+  const bool plain_form = p->form != SPARK_FORM_FP32_FLUX && p->form != SPARK_FORM_BATCHED_FP64;
+  if (plain_form)
+    for (int q = 0; q + 1 < p->ndim; q++) ALLOC(b->fl[q], sizeof(double) * b->fpool);
   if (p->flux_correct) {
     for (int q = 0; q < 3; q++) ALLOC(b->fb[q], sizeof(double) * b->bpool);
     for (int q = 0; q < 3; q++) ALLOC(b->fc[q], sizeof(double) * b->bpool);
@@ -998,7 +1005,6 @@ int spark_batch_create(const spark_params* p, int nblocks, const spark_block_des
   if (p->form == SPARK_FORM_FP32_FLUX || p->form == SPARK_FORM_BATCHED_FP64) {
     b->fused = true;
     b->fused64 = p->form == SPARK_FORM_BATCHED_FP64;
-    ALLOC(b->S2, sizeof(double) * b->pool);
     ALLOC(b->counters, sizeof(int) * 3 * spark::FUSED_XCDS);
     ALLOC(b->tune_err, sizeof(int));
     ALLOC(b->tune_dt, sizeof(double));
@@ -1010,13 +1016,14 @@ int spark_batch_create(const spark_params* p, int nblocks, const spark_block_des
   HIPCHK(hipMemsetAsync(b->U, 0, sizeof(double) * b->pool, g_stream));
   HIPCHK(hipMemsetAsync(b->S, 0, sizeof(double) * b->pool, g_stream));
   HIPCHK(hipMemsetAsync(b->flat, 0, sizeof(double) * b->opool, g_stream));
-  for (int q = 0; q < 3; q++) HIPCHK(hipMemsetAsync(b->fl[q], 0, sizeof(double) * b->fpool, g_stream));
+  HIPCHK(hipMemsetAsync(b->S2, 0, sizeof(double) * b->pool, g_stream));
+  for (int q = 0; q < 3; q++)
+    if (b->fl[q]) HIPCHK(hipMemsetAsync(b->fl[q], 0, sizeof(double) * b->fpool, g_stream));
   if (p->flux_correct)
     for (int q = 0; q < 3; q++) HIPCHK(hipMemsetAsync(b->fb[q], 0, sizeof(double) * b->bpool, g_stream));
   HIPCHK(hipMemsetAsync(b->err, 0, sizeof(int), g_stream));
   HIPCHK(hipMemsetAsync(b->hist_slot, 0, sizeof(int), g_stream));
   if (b->fused) {
-    HIPCHK(hipMemsetAsync(b->S2, 0, sizeof(double) * b->pool, g_stream));
     HIPCHK(hipMemsetAsync(b->counters, 0, sizeof(int) * 3 * spark::FUSED_XCDS, g_stream));
     HIPCHK(hipMemsetAsync(b->tune_err, 0, sizeof(int), g_stream));
     HIPCHK(hipMemsetAsync(b->tune_dt, 0, sizeof(double), g_stream));
@@ -1149,7 +1156,7 @@ int spark_unpack(spark_batch* b, double* const* unk_blocks, int interior_only, i
   for (int i = 0; i < b->nblk; i++)
     if (!unk_blocks[i]) return fail(SPARK_ERR_ARG, "spark_unpack: block %d pointer is null", i);
   HIPCHK(hipStreamSynchronize(g_stream));
-  return transfer(b, unk_blocks, 0, interior_only, which ? b->S : b->U);
+  return transfer(b, unk_blocks, 0, interior_only, which ? (b->star ? b->star : b->S) : b->U);
 }
 
 int spark_set_uniform_grid(spark_batch* b, int nbx, int nby, int nbz) {

@@ -106,6 +106,135 @@ __global__ void __launch_bounds__(256) k_flatten(DevParams P, const BlockMeta* _
 }
 
 // ---------------------------------------------------------------------------
+// conserved update + floors of one cell (hy_rk_updateSoln.F90-mc:211-287)
+// and the Gamma EOS in MODE_DENS_EI (Eos_wrapped over the same limits).
+// lo[d] / hi[d] point at flux 0 of the cell's low / high face in direction
+// d, flux n at lo[d][n * fst[d]].  Every load of the cell (the U0 and
+// stage-input state, the face fluxes) is issued before the first use
+// waits: scalar slots past ns re-read scalar ns-1, so no load sits behind a
+// branch.  u0 and us may alias (the last stage writes U): only the cell's
+// own values are read and written.
+template <int NSM, int NDIM>
+__device__ __forceinline__ void update_cell(const DevParams& P, const BlockMeta& m, int i, int j, int64_t c,
+                                            const double* u0, const double* ui, double* us,
+                                            const double* const (&lo)[3], const double* const (&hi)[3],
+                                            const int64_t (&fst)[3], double dt, double c1, double c2, double c3,
+                                            int final_stage) {
+  const int ns = P.ns;
+  const double dx = m.dx, dy = m.dy, dz = m.dz;
+  const int64_t cs = m.cstride;
+  // hydro part: its 10 state and 10 NDIM face-flux loads issue together
+  double A[F_NUM], B[F_NUM], fl0[F_NUM], fh0[F_NUM], fl1[F_NUM], fh1[F_NUM], fl2[F_NUM], fh2[F_NUM];
+  {
+    const int sv[5] = {V_DENS, V_VELX, V_VELY, V_VELZ, V_ENER};
+#pragma unroll
+    for (int q = 0; q < F_NUM; q++) {
+      A[q] = u0[sv[q] * cs + c];
+      B[q] = ui[sv[q] * cs + c];
+      fl0[q] = lo[0][q * fst[0]];
+      fh0[q] = hi[0][q * fst[0]];
+      fl1[q] = fh1[q] = fl2[q] = fh2[q] = 0.;
+      if (NDIM > 1) { fl1[q] = lo[1][q * fst[1]]; fh1[q] = hi[1][q * fst[1]]; }
+      if (NDIM > 2) { fl2[q] = lo[2][q * fst[2]]; fh2[q] = hi[2][q * fst[2]]; }
+    }
+  }
+  const double d0 = A[F_MASS];
+#pragma unroll
+  for (int q = 1; q < F_NUM; q++) A[q] = d0 * A[q];
+  const double ds = B[F_MASS];
+#pragma unroll
+  for (int q = 1; q < F_NUM; q++) B[q] = ds * B[q];
+  double Sg[F_NUM];
+#pragma unroll
+  for (int n = 0; n < F_NUM; n++) Sg[n] = 0.;
+  if (P.use_grav) {
+    Sg[F_XMOM] = B[F_MASS] * P.grav[0];
+    Sg[F_YMOM] = B[F_MASS] * P.grav[1];
+    Sg[F_ZMOM] = B[F_MASS] * P.grav[2];
+    Sg[F_ENER] = B[F_XMOM] * P.grav[0] + B[F_YMOM] * P.grav[1] + B[F_ZMOM] * P.grav[2];
+  }
+  // geometric factors and sources (geoFacs, hy_rk_updateSoln.F90-mc:98-175)
+  double facM, facP, rc, Sgeo[F_NUM] = {0., 0., 0., 0., 0.};
+  if (geo_facs(P, m.id, i, j, dx, facM, facP, rc) && P.geom != 0)
+    geo_sources(P, m.id, i, dx, rc, ui[V_DENS * cs + c], ui[V_VELX * cs + c], ui[V_VELZ * cs + c],
+                ui[V_PRES * cs + c], B[F_YMOM], Sgeo);
+#pragma unroll
+  for (int n = 0; n < F_NUM; n++) {
+    double inner = -(dt / dx * (facP * fh0[n] - facM * fl0[n]));
+    if (NDIM > 1) inner = inner - dt / dy * (fh1[n] - fl1[n]);
+    if (NDIM == 3) inner = inner - dt / dz * (fh2[n] - fl2[n]);
+    inner = inner + dt * (Sgeo[n] + Sg[n]);
+    B[n] = c1 * A[n] + c2 * B[n] + c3 * inner;
+  }
+  const double dens = fmax2(B[F_MASS], P.smlrho);
+  const double vx = B[F_XMOM] / dens, vy = B[F_YMOM] / dens, vz = B[F_ZMOM] / dens;
+  const double ener = fmax2(P.smalle, B[F_ENER] / dens);
+  const double ekin = .5 * (vx * vx + vy * vy + vz * vz);
+  const double eint = fmax2(P.smalle, ener - ekin);
+  // mass scalars, four at a time: the 8 state and 8 NDIM face-flux loads
+  // of a group issue together (slots past ns re-read scalar ns-1); the
+  // scheduling barriers keep one group's loads in registers at a time (the
+  // caller's occupancy, not this cell's latency, hides the memory)
+  double Xs[NSM > 0 ? NSM : 1];  // the mass fractions (species first) the EOS mixes
+#pragma unroll
+  for (int v0 = 0; v0 < NSM; v0 += 4) {
+    __builtin_amdgcn_sched_barrier(0);
+    double a4[4], b4[4], l0[4], h0[4], l1[4], h1[4], l2[4], h2[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int v = v0 + q < ns ? v0 + q : ns - 1;
+      const int64_t o = (int64_t)(V_NPROP + v) * cs + c;
+      a4[q] = u0[o];
+      b4[q] = ui[o];
+      l0[q] = lo[0][(F_NUM + v) * fst[0]];
+      h0[q] = hi[0][(F_NUM + v) * fst[0]];
+      l1[q] = h1[q] = l2[q] = h2[q] = 0.;
+      if (NDIM > 1) { l1[q] = lo[1][(F_NUM + v) * fst[1]]; h1[q] = hi[1][(F_NUM + v) * fst[1]]; }
+      if (NDIM > 2) { l2[q] = lo[2][(F_NUM + v) * fst[2]]; h2[q] = hi[2][(F_NUM + v) * fst[2]]; }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int v = v0 + q;
+      if (v < ns) {
+        const double As = a4[q] * d0;
+        double Bs = b4[q] * ds;
+        double inner = -(dt / dx * (facP * h0[q] - facM * l0[q]));
+        if (NDIM > 1) inner = inner - dt / dy * (h1[q] - l1[q]);
+        if (NDIM == 3) inner = inner - dt / dz * (h2[q] - l2[q]);
+        inner = inner + dt * (0.0 + 0.0);
+        Bs = c1 * As + c2 * Bs + c3 * inner;
+        Xs[v] = Bs / dens;
+        us[(V_NPROP + v) * cs + c] = Xs[v];
+      }
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  EosOut e = eos_dens_ei(P, dens, vx, vy, vz, ener, eint, NSM > 0 ? Xs : nullptr);
+  us[V_DENS * cs + c] = dens;
+  us[V_VELX * cs + c] = vx;
+  us[V_VELY * cs + c] = vy;
+  us[V_VELZ * cs + c] = vz;
+  us[V_PRES * cs + c] = e.pres;
+  us[V_GAMC * cs + c] = e.gamc;
+  us[V_EINT * cs + c] = e.eint;
+  us[V_ENER * cs + c] = e.ener;
+  if (final_stage) {
+    us[V_TEMP * cs + c] = e.temp;
+    us[V_GAME * cs + c] = e.game;
+  }
+}
+
+// what the sweep of the last direction needs to apply the stage update
+struct UpdArgs {
+  const double* U0;     // step input (U); may alias Sout
+  double* Sout;         // stage output state
+  const double* fl[3];  // face fluxes of the earlier sweeps
+  const double* dtp;
+  double c1, c2, c3;
+  int final_stage;
+};
+
+// ---------------------------------------------------------------------------
 // face fluxes for one sweep direction.  A workgroup owns `lanes` pencils
 // (DIR 0: rows of x-pencils; DIR 1/2: x-columns of y/z pencils, so adjacent
 // lanes always touch adjacent x cells and every global access coalesces).
@@ -118,12 +247,12 @@ __global__ void __launch_bounds__(256) k_flatten(DevParams P, const BlockMeta* _
 // current one is reconstructed: staging them in LDS, one variable per
 // barrier, was measured slower (the kernel is bound by FP64 VALU issue,
 // not by its loads).
-template <int DIR, int RECON, int RIEMANN, int NSM>
+template <int DIR, int RECON, int RIEMANN, int NSM, bool UPD>
 __global__ void __launch_bounds__(1024) k_flux(DevParams P, const BlockMeta* __restrict__ meta,
                                                const int* __restrict__ blist, const double* __restrict__ S,
                                                const double* __restrict__ Ushok, const double* __restrict__ flat,
                                                double* __restrict__ fl, int ngoff, int lanes, int* __restrict__ err,
-                                               double* __restrict__ fb, double weight, int add) {
+                                               double* __restrict__ fb, double weight, int add, UpdArgs UA) {
   constexpr int R = ReconRadius<RECON>::R;
   constexpr int NRM = R_NUM + NSM;
   constexpr int NFM = F_NUM + NSM;
@@ -203,252 +332,160 @@ __global__ void __launch_bounds__(1024) k_flux(DevParams P, const BlockMeta* __r
     sm[(NRM + 5) * NT + tid] = Ushok[m.base + V_SHOK * cs + c];
   }
   __syncthreads();
-  if (!valid || s < 1) return;
-  const int tn = (DIR == 0) ? tid - 1 : tid - lanes;
-  double left[NRM], right[NRM];
-#pragma unroll
-  for (int v = 0; v < NRM; v++) {
-    if (v < R_NUM + ns) {
-      left[v] = sm[v * NT + tn];
-      right[v] = umr[v];
-    }
-  }
-  left[R_DENS] = fmax2(P.smlrho, left[R_DENS]);
-  left[R_PRES] = fmax2(P.smallp, left[R_PRES]);
-  right[R_DENS] = fmax2(P.smlrho, right[R_DENS]);
-  right[R_PRES] = fmax2(P.smallp, right[R_PRES]);
-  if (NSM > 0 && P.nspec > 0) {
-    // species: clamp to [smallx, 1] and renormalise (Spark/Hydro.ini:227-235)
-    double sl = 0.0, sr = 0.0;
-#pragma unroll
-    for (int v = 0; v < NSM; v++) {
-      if (v < P.nspec) {
-        left[R_NUM + v] = fmax2(P.smallx, fmin2(1.0, left[R_NUM + v]));
-        right[R_NUM + v] = fmax2(P.smallx, fmin2(1.0, right[R_NUM + v]));
-        sl = sl + left[R_NUM + v];
-        sr = sr + right[R_NUM + v];
-      }
-    }
-    const double il = 1. / sl, ir = 1. / sr;
-#pragma unroll
-    for (int v = 0; v < NSM; v++) {
-      if (v < P.nspec) {
-        left[R_NUM + v] = left[R_NUM + v] * il;
-        right[R_NUM + v] = right[R_NUM + v] * ir;
-      }
-    }
-  }
-  const bool inShock = (sm[(NRM + 5) * NT + tn] != 0.0) || (sm[(NRM + 5) * NT + tid] != 0.0);
+  const bool face = valid && s >= 1;
   double F[NFM];
-  int ierr;
-  if (RIEMANN == RIEMANN_HLL) ierr = hll<DIR, double>(left, right, F);
-  else ierr = hllc<DIR, double>(left, right, P.hybrid != 0, inShock, P.tiny, F);
-  if (ierr) {
+  if (face) {
+    const int tn = (DIR == 0) ? tid - 1 : tid - lanes;
+    double left[NRM], right[NRM];
 #pragma unroll
-    for (int n = 0; n < F_NUM; n++) F[n] = 0.;
-    atomicAdd(err, 1);
-  }
-  // artificial viscosity (Hydro.ini:285-301)
-  {
-    const double dL = sm[(NRM + 0) * NT + tn], dH = sm[(NRM + 0) * NT + tid];
-    const double xL = sm[(NRM + 1) * NT + tn], xH = sm[(NRM + 1) * NT + tid];
-    const double yL = sm[(NRM + 2) * NT + tn], yH = sm[(NRM + 2) * NT + tid];
-    const double zL = sm[(NRM + 3) * NT + tn], zH = sm[(NRM + 3) * NT + tid];
-    const double eL = sm[(NRM + 4) * NT + tn], eH = sm[(NRM + 4) * NT + tid];
-    const double nL = (DIR == 0) ? xL : (DIR == 1) ? yL : zL;
-    const double nH = (DIR == 0) ? xH : (DIR == 1) ? yH : zH;
-    double cvisc = P.cvisc * fmax2(-(nH - nL), 0.);
-    double VenerLo = dL * 0.5 * (xL * xL + yL * yL + zL * zL) + eL;
-    double VenerHi = dH * 0.5 * (xH * xH + yH * yH + zH * zH) + eH;
-    F[F_MASS] = F[F_MASS] + cvisc * (dL - dH);
-    F[F_XMOM] = F[F_XMOM] + cvisc * (dL * xL - dH * xH);
-    F[F_YMOM] = F[F_YMOM] + cvisc * (dL * yL - dH * yH);
-    F[F_ZMOM] = F[F_ZMOM] + cvisc * (dL * zL - dH * zH);
-    F[F_ENER] = F[F_ENER] + cvisc * (VenerLo - VenerHi);
-  }
+    for (int v = 0; v < NRM; v++) {
+      if (v < R_NUM + ns) {
+        left[v] = sm[v * NT + tn];
+        right[v] = umr[v];
+      }
+    }
+    left[R_DENS] = fmax2(P.smlrho, left[R_DENS]);
+    left[R_PRES] = fmax2(P.smallp, left[R_PRES]);
+    right[R_DENS] = fmax2(P.smlrho, right[R_DENS]);
+    right[R_PRES] = fmax2(P.smallp, right[R_PRES]);
+    if (NSM > 0 && P.nspec > 0) {
+      // species: clamp to [smallx, 1] and renormalise (Spark/Hydro.ini:227-235)
+      double sl = 0.0, sr = 0.0;
 #pragma unroll
-  for (int v = 0; v < NSM; v++) {
-    if (v < ns) {
-      if (F[F_MASS] > 0.) F[F_NUM + v] = left[R_NUM + v] * F[F_MASS];
-      else F[F_NUM + v] = right[R_NUM + v] * F[F_MASS];
+      for (int v = 0; v < NSM; v++) {
+        if (v < P.nspec) {
+          left[R_NUM + v] = fmax2(P.smallx, fmin2(1.0, left[R_NUM + v]));
+          right[R_NUM + v] = fmax2(P.smallx, fmin2(1.0, right[R_NUM + v]));
+          sl = sl + left[R_NUM + v];
+          sr = sr + right[R_NUM + v];
+        }
+      }
+      const double il = 1. / sl, ir = 1. / sr;
+#pragma unroll
+      for (int v = 0; v < NSM; v++) {
+        if (v < P.nspec) {
+          left[R_NUM + v] = left[R_NUM + v] * il;
+          right[R_NUM + v] = right[R_NUM + v] * ir;
+        }
+      }
+    }
+    const bool inShock = (sm[(NRM + 5) * NT + tn] != 0.0) || (sm[(NRM + 5) * NT + tid] != 0.0);
+    int ierr;
+    if (RIEMANN == RIEMANN_HLL) ierr = hll<DIR, double>(left, right, F);
+    else ierr = hllc<DIR, double>(left, right, P.hybrid != 0, inShock, P.tiny, F);
+    if (ierr) {
+#pragma unroll
+      for (int n = 0; n < F_NUM; n++) F[n] = 0.;
+      atomicAdd(err, 1);
+    }
+    // artificial viscosity (Hydro.ini:285-301)
+    {
+      const double dL = sm[(NRM + 0) * NT + tn], dH = sm[(NRM + 0) * NT + tid];
+      const double xL = sm[(NRM + 1) * NT + tn], xH = sm[(NRM + 1) * NT + tid];
+      const double yL = sm[(NRM + 2) * NT + tn], yH = sm[(NRM + 2) * NT + tid];
+      const double zL = sm[(NRM + 3) * NT + tn], zH = sm[(NRM + 3) * NT + tid];
+      const double eL = sm[(NRM + 4) * NT + tn], eH = sm[(NRM + 4) * NT + tid];
+      const double nL = (DIR == 0) ? xL : (DIR == 1) ? yL : zL;
+      const double nH = (DIR == 0) ? xH : (DIR == 1) ? yH : zH;
+      double cvisc = P.cvisc * fmax2(-(nH - nL), 0.);
+      double VenerLo = dL * 0.5 * (xL * xL + yL * yL + zL * zL) + eL;
+      double VenerHi = dH * 0.5 * (xH * xH + yH * yH + zH * zH) + eH;
+      F[F_MASS] = F[F_MASS] + cvisc * (dL - dH);
+      F[F_XMOM] = F[F_XMOM] + cvisc * (dL * xL - dH * xH);
+      F[F_YMOM] = F[F_YMOM] + cvisc * (dL * yL - dH * yH);
+      F[F_ZMOM] = F[F_ZMOM] + cvisc * (dL * zL - dH * zH);
+      F[F_ENER] = F[F_ENER] + cvisc * (VenerLo - VenerHi);
+    }
+#pragma unroll
+    for (int v = 0; v < NSM; v++) {
+      if (v < ns) {
+        if (F[F_MASS] > 0.) F[F_NUM + v] = left[R_NUM + v] * F[F_MASS];
+        else F[F_NUM + v] = right[R_NUM + v] * F[F_MASS];
+      }
+    }
+    if (!UPD) {
+      const int64_t f = m.fbase + fix(m, ijk[0], ijk[1], ijk[2]);
+#pragma unroll
+      for (int n = 0; n < NFM; n++)
+        if (n < F_NUM + ns) fl[f + n * m.fstride] = F[n];
+    }
+    // flux registers (addFluxes, Hydro_funcs.F90-mc:481-507) of the block's
+    // two boundary faces of this direction: the register hand-off reads no
+    // other face (Grid_putFluxData_block.F90:200-202, 278-279), so the
+    // interior entries are never accumulated and stay zero
+    if (fb != nullptr) {
+      const int g = P.ng;
+      const int nn[3] = {m.nx, m.ny, m.nz};
+      bool bnd = ijk[DIR] == g || ijk[DIR] == g + nn[DIR];
+#pragma unroll
+      for (int d = 0; d < 3; d++)
+        if (d != DIR && d < P.ndim) bnd = bnd && ijk[d] >= g && ijk[d] < g + nn[d];
+      if (bnd) {
+        int e[3], fi[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          e[d] = d < P.ndim ? nn[d] : 1;
+          fi[d] = d < P.ndim ? ijk[d] - g : 0;
+        }
+        e[DIR] += 1;
+        const int64_t bstride = (int64_t)e[0] * e[1] * e[2];
+        const int64_t o = m.bbase + fi[0] + (int64_t)e[0] * (fi[1] + (int64_t)e[1] * fi[2]);
+        // every old entry is loaded before the first store (the compiler
+        // cannot tell the entries apart, so a load after a store would wait
+        // for it); a slot past F_NUM + ns re-reads the last entry
+        double prev[NFM];
+#pragma unroll
+        for (int n = 0; n < NFM; n++) prev[n] = 0.0;
+        if (add) {
+#pragma unroll
+          for (int n = 0; n < NFM; n++) prev[n] = fb[o + (n < F_NUM + ns ? n : F_NUM + ns - 1) * bstride];
+        }
+#pragma unroll
+        for (int n = 0; n < NFM; n++) {
+          if (n < F_NUM + ns) {
+            const double w = weight * F[n];
+            fb[o + n * bstride] = add ? prev[n] + w : w;
+          }
+        }
+      }
     }
   }
-  const int64_t f = m.fbase + fix(m, ijk[0], ijk[1], ijk[2]);
+  if (UPD) {
+    // the sweep of the last direction also applies the stage's conserved
+    // update to the region's cells (s = 1 .. Lr - 2): the face fluxes of
+    // this direction come through LDS (cell s owns its low face, thread
+    // s + 1 computed its high face), the others from the earlier sweeps
+    __syncthreads();  // every read of the exchange buffer is done
+    if (face) {
 #pragma unroll
-  for (int n = 0; n < NFM; n++)
-    if (n < F_NUM + ns) fl[f + n * m.fstride] = F[n];
-  // flux registers (addFluxes, Hydro_funcs.F90-mc:481-507) of the block's
-  // two boundary faces of this direction: the register hand-off reads no
-  // other face (Grid_putFluxData_block.F90:200-202, 278-279), so the
-  // interior entries are never accumulated and stay zero
-  if (fb != nullptr) {
-    const int g = P.ng;
-    const int nn[3] = {m.nx, m.ny, m.nz};
-    bool bnd = ijk[DIR] == g || ijk[DIR] == g + nn[DIR];
-#pragma unroll
-    for (int d = 0; d < 3; d++)
-      if (d != DIR && d < P.ndim) bnd = bnd && ijk[d] >= g && ijk[d] < g + nn[d];
-    if (bnd) {
-      int e[3], fi[3];
+      for (int n = 0; n < NFM; n++)
+        if (n < F_NUM + ns) sm[n * NT + tid] = F[n];
+    }
+    __syncthreads();
+    if (valid && s >= 1 && s <= Lr - 2) {
+      const int th = (DIR == 0) ? tid + 1 : tid + lanes;
+      const double* lo[3];
+      const double* hi[3];
+      int64_t fst[3];
 #pragma unroll
       for (int d = 0; d < 3; d++) {
-        e[d] = d < P.ndim ? nn[d] : 1;
-        fi[d] = d < P.ndim ? ijk[d] - g : 0;
-      }
-      e[DIR] += 1;
-      const int64_t bstride = (int64_t)e[0] * e[1] * e[2];
-      const int64_t o = m.bbase + fi[0] + (int64_t)e[0] * (fi[1] + (int64_t)e[1] * fi[2]);
-      // every old entry is loaded before the first store (the compiler
-      // cannot tell the entries apart, so a load after a store would wait
-      // for it); a slot past F_NUM + ns re-reads the last entry
-      double prev[NFM];
-#pragma unroll
-      for (int n = 0; n < NFM; n++) prev[n] = 0.0;
-      if (add) {
-#pragma unroll
-        for (int n = 0; n < NFM; n++) prev[n] = fb[o + (n < F_NUM + ns ? n : F_NUM + ns - 1) * bstride];
-      }
-#pragma unroll
-      for (int n = 0; n < NFM; n++) {
-        if (n < F_NUM + ns) {
-          const double w = weight * F[n];
-          fb[o + n * bstride] = add ? prev[n] + w : w;
+        if (d > DIR) {  // no such direction (DIR is the last)
+          lo[d] = hi[d] = nullptr;
+          fst[d] = 0;
+        } else if (d == DIR) {
+          lo[d] = sm + tid;
+          hi[d] = sm + th;
+          fst[d] = NT;
+        } else {
+          int up[3] = {ijk[0], ijk[1], ijk[2]};
+          up[d] += 1;
+          lo[d] = UA.fl[d] + m.fbase + fix(m, ijk[0], ijk[1], ijk[2]);
+          hi[d] = UA.fl[d] + m.fbase + fix(m, up[0], up[1], up[2]);
+          fst[d] = m.fstride;
         }
       }
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// conserved update + floors (hy_rk_updateSoln.F90-mc:211-287) and the
-// Gamma EOS in MODE_DENS_EI (Eos_wrapped over the same limits).  Every
-// load of a cell (the U0 and stage-input state, the low and high face
-// fluxes of each direction) is issued before the first use waits: the
-// dimension count is a template parameter and scalar slots past ns re-read
-// scalar ns-1, so no load sits behind a branch.
-template <int NSM, int NDIM>
-__global__ void __launch_bounds__(256) k_update(DevParams P, const BlockMeta* __restrict__ meta,
-                                                const int* __restrict__ blist, const double* U0,
-                                                const double* Sin, double* Sout, const double* __restrict__ flx,
-                                                const double* __restrict__ fly, const double* __restrict__ flz,
-                                                const double* __restrict__ dtp, double c1, double c2, double c3,
-                                                int ngoff, int final_stage) {
-  int cx, cy;
-  xcd_chunk(cx, cy);
-  const BlockMeta m = meta[blist[cy]];
-  Lims L = stage_lims(P, m, ngoff);
-  const int ex = L.hi[0] - L.lo[0] + 1, ey = L.hi[1] - L.lo[1] + 1, ez = L.hi[2] - L.lo[2] + 1;
-  const int total = ex * ey * ez;
-  const int ns = P.ns;
-  const double dt = *dtp;
-  const double dx = m.dx, dy = m.dy, dz = m.dz;
-  const int64_t cs = m.cstride, fs = m.fstride;
-  // U0, Sin and Sout may alias (stage 1 reads U twice; the last stage
-  // writes U): every thread reads and writes only its own cell
-  const double* u0 = U0 + m.base;
-  const double* ui = Sin + m.base;
-  double* us = Sout + m.base;
-  for (int t = cx * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-    const int i = L.lo[0] + t % ex, j = L.lo[1] + (t / ex) % ey, k = L.lo[2] + t / (ex * ey);
-    const int64_t c = cix(m, i, j, k);
-    const int64_t fm = m.fbase + fix(m, i, j, k);
-    const int64_t fpx = m.fbase + fix(m, i + 1, j, k);
-    const int64_t fpy = m.fbase + fix(m, i, j + 1, k);
-    const int64_t fpz = m.fbase + fix(m, i, j, k + 1);
-    // hydro part: its 10 state and 10 NDIM face-flux loads issue together
-    double A[F_NUM], B[F_NUM], fl0[F_NUM], fh0[F_NUM], fl1[F_NUM], fh1[F_NUM], fl2[F_NUM], fh2[F_NUM];
-    {
-      const int sv[5] = {V_DENS, V_VELX, V_VELY, V_VELZ, V_ENER};
-#pragma unroll
-      for (int q = 0; q < F_NUM; q++) {
-        A[q] = u0[sv[q] * cs + c];
-        B[q] = ui[sv[q] * cs + c];
-        fl0[q] = flx[fm + q * fs];
-        fh0[q] = flx[fpx + q * fs];
-        fl1[q] = fh1[q] = fl2[q] = fh2[q] = 0.;
-        if (NDIM > 1) { fl1[q] = fly[fm + q * fs]; fh1[q] = fly[fpy + q * fs]; }
-        if (NDIM > 2) { fl2[q] = flz[fm + q * fs]; fh2[q] = flz[fpz + q * fs]; }
-      }
-    }
-    const double d0 = A[F_MASS];
-#pragma unroll
-    for (int q = 1; q < F_NUM; q++) A[q] = d0 * A[q];
-    const double ds = B[F_MASS];
-#pragma unroll
-    for (int q = 1; q < F_NUM; q++) B[q] = ds * B[q];
-    double Sg[F_NUM];
-#pragma unroll
-    for (int n = 0; n < F_NUM; n++) Sg[n] = 0.;
-    if (P.use_grav) {
-      Sg[F_XMOM] = B[F_MASS] * P.grav[0];
-      Sg[F_YMOM] = B[F_MASS] * P.grav[1];
-      Sg[F_ZMOM] = B[F_MASS] * P.grav[2];
-      Sg[F_ENER] = B[F_XMOM] * P.grav[0] + B[F_YMOM] * P.grav[1] + B[F_ZMOM] * P.grav[2];
-    }
-    // geometric factors and sources (geoFacs, hy_rk_updateSoln.F90-mc:98-175)
-    double facM, facP, rc, Sgeo[F_NUM] = {0., 0., 0., 0., 0.};
-    if (geo_facs(P, m.id, i, j, dx, facM, facP, rc) && P.geom != 0)
-      geo_sources(P, m.id, i, dx, rc, ui[V_DENS * cs + c], ui[V_VELX * cs + c], ui[V_VELZ * cs + c],
-                  ui[V_PRES * cs + c], B[F_YMOM], Sgeo);
-#pragma unroll
-    for (int n = 0; n < F_NUM; n++) {
-      double inner = -(dt / dx * (facP * fh0[n] - facM * fl0[n]));
-      if (NDIM > 1) inner = inner - dt / dy * (fh1[n] - fl1[n]);
-      if (NDIM == 3) inner = inner - dt / dz * (fh2[n] - fl2[n]);
-      inner = inner + dt * (Sgeo[n] + Sg[n]);
-      B[n] = c1 * A[n] + c2 * B[n] + c3 * inner;
-    }
-    const double dens = fmax2(B[F_MASS], P.smlrho);
-    const double vx = B[F_XMOM] / dens, vy = B[F_YMOM] / dens, vz = B[F_ZMOM] / dens;
-    const double ener = fmax2(P.smalle, B[F_ENER] / dens);
-    const double ekin = .5 * (vx * vx + vy * vy + vz * vz);
-    const double eint = fmax2(P.smalle, ener - ekin);
-    // mass scalars, four at a time: the 8 state and 8 NDIM face-flux loads
-    // of a group issue together (slots past ns re-read scalar ns-1)
-    double Xs[NSM > 0 ? NSM : 1];  // the mass fractions (species first) the EOS mixes
-#pragma unroll
-    for (int v0 = 0; v0 < NSM; v0 += 4) {
-      double a4[4], b4[4], l0[4], h0[4], l1[4], h1[4], l2[4], h2[4];
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int v = v0 + q < ns ? v0 + q : ns - 1;
-        const int64_t o = (int64_t)(V_NPROP + v) * cs + c, of = (int64_t)(F_NUM + v) * fs;
-        a4[q] = u0[o];
-        b4[q] = ui[o];
-        l0[q] = flx[fm + of];
-        h0[q] = flx[fpx + of];
-        l1[q] = h1[q] = l2[q] = h2[q] = 0.;
-        if (NDIM > 1) { l1[q] = fly[fm + of]; h1[q] = fly[fpy + of]; }
-        if (NDIM > 2) { l2[q] = flz[fm + of]; h2[q] = flz[fpz + of]; }
-      }
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int v = v0 + q;
-        if (v < ns) {
-          const double As = a4[q] * d0;
-          double Bs = b4[q] * ds;
-          double inner = -(dt / dx * (facP * h0[q] - facM * l0[q]));
-          if (NDIM > 1) inner = inner - dt / dy * (h1[q] - l1[q]);
-          if (NDIM == 3) inner = inner - dt / dz * (h2[q] - l2[q]);
-          inner = inner + dt * (0.0 + 0.0);
-          Bs = c1 * As + c2 * Bs + c3 * inner;
-          Xs[v] = Bs / dens;
-          us[(V_NPROP + v) * cs + c] = Xs[v];
-        }
-      }
-    }
-    EosOut e = eos_dens_ei(P, dens, vx, vy, vz, ener, eint, NSM > 0 ? Xs : nullptr);
-    us[V_DENS * cs + c] = dens;
-    us[V_VELX * cs + c] = vx;
-    us[V_VELY * cs + c] = vy;
-    us[V_VELZ * cs + c] = vz;
-    us[V_PRES * cs + c] = e.pres;
-    us[V_GAMC * cs + c] = e.gamc;
-    us[V_EINT * cs + c] = e.eint;
-    us[V_ENER * cs + c] = e.ener;
-    if (final_stage) {
-      us[V_TEMP * cs + c] = e.temp;
-      us[V_GAME * cs + c] = e.game;
+      update_cell<NSM, DIR + 1>(P, m, ijk[0], ijk[1], c, UA.U0 + m.base, sp, UA.Sout + m.base, lo, hi, fst,
+                                *UA.dtp, UA.c1, UA.c2, UA.c3, UA.final_stage);
     }
   }
 }
@@ -918,8 +955,8 @@ static int resident_waves(const void* kern, int threads, size_t shm) {
   return res;
 }
 
-template <int DIR, int RECON, int RIEMANN, int NSM>
-static hipError_t launch_flux_t(const PlainLaunch& L, int ngoff) {
+template <int DIR, int RECON, int RIEMANN, int NSM, bool UPD>
+static hipError_t launch_flux_t(const PlainLaunch& L, int ngoff, const UpdArgs& UA) {
   // geometry of this stage (all blocks in the list share one shape)
   const int n[3] = {L.nx, L.ny, L.nz};
   int lo[3], hi[3];
@@ -952,7 +989,7 @@ static hipError_t launch_flux_t(const PlainLaunch& L, int ngoff) {
   };
   // cost in units of 1/1000 wave per resident wave
   auto cost = [&](int l) {
-    const int res = resident_waves(reinterpret_cast<const void*>(&k_flux<DIR, RECON, RIEMANN, NSM>), l * Lr,
+    const int res = resident_waves(reinterpret_cast<const void*>(&k_flux<DIR, RECON, RIEMANN, NSM, UPD>), l * Lr,
                                    sizeof(double) * (size_t)(NRM + 6) * l * Lr);
     return (int64_t)waves_of(l) * 1000 / std::max(res, 1);
   };
@@ -981,44 +1018,53 @@ static hipError_t launch_flux_t(const PlainLaunch& L, int ngoff) {
     const double recon_cells = (double)Lr * nt1 * nt2;
     const double faces = (double)(Lr - 1) * nt1 * nt2;
     const int ns = L.P.ns;
-    const double bytes = 8.0 * L.nblk * (cells_in * (8 + ns + (RECON == RECON_WENO5 && L.P.use_flat)) + faces * (5 + ns));
-    const double flops = (double)L.nblk * (recon_cells * (7 + ns) * 117.0 + faces * (175.0 + ns));
+    double bytes = 8.0 * L.nblk * (cells_in * (8 + ns + (RECON == RECON_WENO5 && L.P.use_flat)) +
+                                   (UPD ? 0.0 : faces * (5 + ns)));
+    double flops = (double)L.nblk * (recon_cells * (7 + ns) * 117.0 + faces * (175.0 + ns));
+    if (UPD) {
+      // the update of the region's cells: U0 and the stage input (5 + ns
+      // each) and the earlier sweeps' face fluxes read, the stage output
+      // (8 + ns, + temp and game at the last stage) written
+      const double box = (double)(Lr - 2) * nt1 * nt2;
+      double ofaces = 0.;
+      for (int d = 0; d < DIR; d++) ofaces += box / (hi[d] - lo[d] + 1) * (hi[d] - lo[d] + 2);
+      bytes += 8.0 * L.nblk * (box * (2 * (5 + ns) + 8 + ns + 2 * UA.final_stage) + ofaces * (5 + ns));
+      flops += (double)L.nblk * box * (60.0 + 10.0 * (DIR + 1) * (5 + ns));
+    }
     static const char* names[3] = {"k_flux_x", "k_flux_y", "k_flux_z"};
     L.prof->begin(L.stream, names[DIR], bytes, flops);
   }
-  hipLaunchKernelGGL((k_flux<DIR, RECON, RIEMANN, NSM>), grid, block, shm, L.stream, L.P, L.meta, L.blist, L.Sin,
-                     L.U0, L.flat, L.fl[DIR], ngoff, lanes, L.err, L.fb[DIR], L.weight, L.add_flux);
+  hipLaunchKernelGGL((k_flux<DIR, RECON, RIEMANN, NSM, UPD>), grid, block, shm, L.stream, L.P, L.meta, L.blist,
+                     L.Sin, L.U0, L.flat, UPD ? nullptr : L.fl[DIR], ngoff, lanes, L.err, L.fb[DIR], L.weight,
+                     L.add_flux, UA);
   if (L.prof) L.prof->end(L.stream);
   return hipGetLastError();
 }
 
-template <int DIR, int NSM>
-static hipError_t launch_flux_r(const PlainLaunch& L, int ngoff) {
+template <int DIR, int NSM, bool UPD>
+static hipError_t launch_flux_r(const PlainLaunch& L, int ngoff, const UpdArgs& UA) {
   const int r = L.P.recon, q = L.P.riemann;
   if (q == RIEMANN_HLL) {
     switch (r) {
-      case RECON_MP5: return launch_flux_t<DIR, RECON_MP5, RIEMANN_HLL, NSM>(L, ngoff);
-      case RECON_PLM: return launch_flux_t<DIR, RECON_PLM, RIEMANN_HLL, NSM>(L, ngoff);
-      case RECON_FOG: return launch_flux_t<DIR, RECON_FOG, RIEMANN_HLL, NSM>(L, ngoff);
-      default: return launch_flux_t<DIR, RECON_WENO5, RIEMANN_HLL, NSM>(L, ngoff);
+      case RECON_MP5: return launch_flux_t<DIR, RECON_MP5, RIEMANN_HLL, NSM, UPD>(L, ngoff, UA);
+      case RECON_PLM: return launch_flux_t<DIR, RECON_PLM, RIEMANN_HLL, NSM, UPD>(L, ngoff, UA);
+      case RECON_FOG: return launch_flux_t<DIR, RECON_FOG, RIEMANN_HLL, NSM, UPD>(L, ngoff, UA);
+      default: return launch_flux_t<DIR, RECON_WENO5, RIEMANN_HLL, NSM, UPD>(L, ngoff, UA);
     }
   }
   switch (r) {
-    case RECON_MP5: return launch_flux_t<DIR, RECON_MP5, RIEMANN_HLLC, NSM>(L, ngoff);
-    case RECON_PLM: return launch_flux_t<DIR, RECON_PLM, RIEMANN_HLLC, NSM>(L, ngoff);
-    case RECON_FOG: return launch_flux_t<DIR, RECON_FOG, RIEMANN_HLLC, NSM>(L, ngoff);
-    default: return launch_flux_t<DIR, RECON_WENO5, RIEMANN_HLLC, NSM>(L, ngoff);
+    case RECON_MP5: return launch_flux_t<DIR, RECON_MP5, RIEMANN_HLLC, NSM, UPD>(L, ngoff, UA);
+    case RECON_PLM: return launch_flux_t<DIR, RECON_PLM, RIEMANN_HLLC, NSM, UPD>(L, ngoff, UA);
+    case RECON_FOG: return launch_flux_t<DIR, RECON_FOG, RIEMANN_HLLC, NSM, UPD>(L, ngoff, UA);
+    default: return launch_flux_t<DIR, RECON_WENO5, RIEMANN_HLLC, NSM, UPD>(L, ngoff, UA);
   }
 }
 
 hipError_t plain_stage(const PlainLaunch& L, int ngoff, double c1, double c2, double c3, int final_stage) {
   hipError_t e;
   const int n[3] = {L.nx, L.ny, L.nz};
-  int box = 1, boxf = 1;
-  for (int d = 0; d < 3; d++) {
-    if (d < L.P.ndim) { box *= n[d] + 2 * ngoff; boxf *= n[d] + 2 * ngoff + 2; }
-  }
-  const int nsv = L.P.ns;
+  int boxf = 1;
+  for (int d = 0; d < L.P.ndim; d++) boxf *= n[d] + 2 * ngoff + 2;
   if (L.P.use_flat && L.P.recon == RECON_WENO5) {
     if (L.prof) {
       int bx = 1, bxr = 1;
@@ -1036,43 +1082,26 @@ hipError_t plain_stage(const PlainLaunch& L, int ngoff, double c1, double c2, do
     if (L.prof) L.prof->end(L.stream);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
-  for (int dir = 0; dir < L.P.ndim; dir++) {
+  // the sweep of the last direction applies the update; it reads the stage
+  // input around each cell while writing the stage output, so the two
+  // must be different buffers
+  if (L.Sin == L.Sout) return hipErrorInvalidValue;
+  const UpdArgs UA{L.U0, L.Sout, {L.fl[0], L.fl[1], L.fl[2]}, L.dtp, c1, c2, c3, final_stage};
+  const int nd = L.P.ndim;
+  for (int dir = 0; dir < nd; dir++) {
+    const bool last = dir == nd - 1;
     if (L.P.ns == 0) {
-      e = (dir == 0) ? launch_flux_r<0, 0>(L, ngoff) : (dir == 1) ? launch_flux_r<1, 0>(L, ngoff)
-                                                                : launch_flux_r<2, 0>(L, ngoff);
+      if (dir == 0) e = last ? launch_flux_r<0, 0, true>(L, ngoff, UA) : launch_flux_r<0, 0, false>(L, ngoff, UA);
+      else if (dir == 1) e = last ? launch_flux_r<1, 0, true>(L, ngoff, UA) : launch_flux_r<1, 0, false>(L, ngoff, UA);
+      else e = launch_flux_r<2, 0, true>(L, ngoff, UA);
     } else {
-      e = (dir == 0) ? launch_flux_r<0, 8>(L, ngoff) : (dir == 1) ? launch_flux_r<1, 8>(L, ngoff)
-                                                                : launch_flux_r<2, 8>(L, ngoff);
+      if (dir == 0) e = last ? launch_flux_r<0, 8, true>(L, ngoff, UA) : launch_flux_r<0, 8, false>(L, ngoff, UA);
+      else if (dir == 1) e = last ? launch_flux_r<1, 8, true>(L, ngoff, UA) : launch_flux_r<1, 8, false>(L, ngoff, UA);
+      else e = launch_flux_r<2, 8, true>(L, ngoff, UA);
     }
     if (e != hipSuccess) return e;
   }
-  if (L.prof) {
-    // reads U0 and S (5+ns each) and one face flux per face of the box,
-    // writes dens vel(3) pres gamc eint ener (+temp game) and the scalars
-    double faces = 0.;
-    for (int d = 0; d < L.P.ndim; d++) faces += (double)box / (n[d] + 2 * ngoff) * (n[d] + 2 * ngoff + 1);
-    const double bytes = 8.0 * L.nblk * ((double)box * (2 * (5 + nsv) + 8 + nsv + 2 * final_stage) + faces * (5 + nsv));
-    L.prof->begin(L.stream, "k_update", bytes, (double)L.nblk * box * (60.0 + 10.0 * L.P.ndim * (5 + nsv)));
-  }
-  {
-    const dim3 grid(grid_x_for(box, 512), L.nblk), blk(256);
-#define SPARK_UPDATE(NSM_, ND_)                                                                              \
-  hipLaunchKernelGGL((k_update<NSM_, ND_>), grid, blk, 0, L.stream, L.P, L.meta, L.blist, L.U0, L.Sin, L.Sout, \
-                     L.fl[0], L.fl[1], L.fl[2], L.dtp, c1, c2, c3, ngoff, final_stage)
-    const int nd = L.P.ndim;
-    if (L.P.ns == 0) {
-      if (nd == 3) SPARK_UPDATE(0, 3);
-      else if (nd == 2) SPARK_UPDATE(0, 2);
-      else SPARK_UPDATE(0, 1);
-    } else {
-      if (nd == 3) SPARK_UPDATE(8, 3);
-      else if (nd == 2) SPARK_UPDATE(8, 2);
-      else SPARK_UPDATE(8, 1);
-    }
-#undef SPARK_UPDATE
-  }
-  if (L.prof) L.prof->end(L.stream);
-  return hipGetLastError();
+  return hipSuccess;
 }
 
 // band: SHOK is computed over interior +/- band (the reference uses

@@ -135,7 +135,8 @@ int spark_pack(spark_batch* b, const double* const* unk_blocks);
 int spark_pack_noalloc(spark_batch* b, const double* const* unk_blocks);
 /* Device -> host unk blocks; interior_only mirrors updateState
  * (Spark/Hydro_funcs.F90-mc:573-574).  which = 0: solution, 1: the stage
- * state of the last stage run (hy_starState). */
+ * state (hy_starState) written by the last stage run before a step's final
+ * stage. */
 int spark_unpack(spark_batch* b, double* const* unk_blocks, int interior_only, int which);
 
 /* Declare the batch a uniform nbx x nby x nbz block grid (blocks numbered
