@@ -496,7 +496,8 @@ def main():
         bound = "valu" if fl_flops / max(fl_bytes, 1.0) > FP64_PEAK_TFS * 1e12 / (HBM_PEAK_GBS * 1e9) else "hbm"
     roofline = {
         "bound": bound,
-        "kernel": "k_flux (x, y, z sweeps: WENO5-Z + HLLC face fluxes)",
+        "kernel": "k_flux (x, y, z sweeps: WENO5-Z + HLLC face fluxes; the last sweep also applies the "
+                  "conserved update + EOS)",
         "achieved": round(alg_tfs, 2) if bound == "valu" else round(alg_gbs, 1),
         "peak": FP64_PEAK_TFS if bound == "valu" else HBM_PEAK_GBS,
         "unit": "TFLOP/s" if bound == "valu" else "GB/s",

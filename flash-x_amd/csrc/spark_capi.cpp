@@ -78,11 +78,11 @@ struct spark_batch {
   std::vector<BlockMeta> meta_h;
   BlockMeta* meta_d = nullptr;
   std::vector<ShapeClass> classes;
-  int64_t pool = 0, opool = 0, fpool = 0, bpool = 0;  // sizes in doubles
+  int64_t pool = 0, opool = 0, dpool = 0, bpool = 0;  // sizes in doubles
   double* U = nullptr;
   double* S = nullptr;
   double* flat = nullptr;
-  double* fl[3] = {nullptr, nullptr, nullptr};
+  double* div = nullptr;  // plain form: the sweeps' divergence terms per cell
   double* fb[3] = {nullptr, nullptr, nullptr};
   double* dtp = nullptr;
   long long* loc = nullptr;
@@ -197,7 +197,8 @@ PlainLaunch make_launch(spark_batch* b, const ShapeClass& c) {
   L.Sin = b->S;
   L.Sout = b->S;
   L.flat = b->flat;
-  for (int d = 0; d < 3; d++) { L.fl[d] = b->fl[d]; L.fb[d] = b->p.flux_correct ? b->fb[d] : nullptr; }
+  L.div = b->div;
+  for (int d = 0; d < 3; d++) L.fb[d] = b->p.flux_correct ? b->fb[d] : nullptr;
   L.weight = 0.;
   L.add_flux = 0;
   L.dtp = b->dtp;
@@ -541,7 +542,8 @@ void free_batch(spark_batch* b) {
   F(b->meta_d);
   for (auto& c : b->classes) F(c.ids_d);
   F(b->U); F(b->S); F(b->flat);
-  for (int d = 0; d < 3; d++) { F(b->fl[d]); F(b->fb[d]); }
+  F(b->div);
+  for (int d = 0; d < 3; d++) F(b->fb[d]);
   F(b->dtp); F(b->loc); F(b->dt_hist); F(b->dtparts); F(b->err);
   delete b;
 }
@@ -938,9 +940,8 @@ int spark_batch_create(const spark_params* p, int nblocks, const spark_block_des
     b->pool += (int64_t)nvar * m.cstride;
     m.obase = b->opool;
     b->opool += m.cstride;
-    m.fstride = round_up((int64_t)(m.NX + 1) * (m.NY + 1) * (m.NZ + 1), 64);
-    m.fbase = b->fpool;
-    b->fpool += (int64_t)nf * m.fstride;
+    m.dbase = b->dpool;
+    b->dpool += (int64_t)nf * m.cstride;
     int64_t reg = 1;
     for (int q = 0; q < 3; q++) reg *= n[q] + (q < p->ndim ? 1 : 0);
     m.bbase = b->bpool;
@@ -975,11 +976,10 @@ int spark_batch_create(const spark_params* p, int nblocks, const spark_block_des
   ALLOC(b->S, sizeof(double) * b->pool);
   ALLOC(b->flat, sizeof(double) * b->opool);
   ALLOC(b->S2, sizeof(double) * b->pool);
-  // face-flux pools: the plain form's sweeps but the last (which applies
-  // the update from the earlier ones)
+  // the plain form's divergence terms (the sweeps before the last write
+  // them; 1D has no earlier sweep)
   const bool plain_form = p->form != SPARK_FORM_FP32_FLUX && p->form != SPARK_FORM_BATCHED_FP64;
-  if (plain_form)
-    for (int q = 0; q + 1 < p->ndim; q++) ALLOC(b->fl[q], sizeof(double) * b->fpool);
+  if (plain_form && p->ndim > 1) ALLOC(b->div, sizeof(double) * b->dpool);
   if (p->flux_correct) {
     for (int q = 0; q < 3; q++) ALLOC(b->fb[q], sizeof(double) * b->bpool);
     for (int q = 0; q < 3; q++) ALLOC(b->fc[q], sizeof(double) * b->bpool);
@@ -1017,8 +1017,7 @@ int spark_batch_create(const spark_params* p, int nblocks, const spark_block_des
   HIPCHK(hipMemsetAsync(b->S, 0, sizeof(double) * b->pool, g_stream));
   HIPCHK(hipMemsetAsync(b->flat, 0, sizeof(double) * b->opool, g_stream));
   HIPCHK(hipMemsetAsync(b->S2, 0, sizeof(double) * b->pool, g_stream));
-  for (int q = 0; q < 3; q++)
-    if (b->fl[q]) HIPCHK(hipMemsetAsync(b->fl[q], 0, sizeof(double) * b->fpool, g_stream));
+  if (b->div) HIPCHK(hipMemsetAsync(b->div, 0, sizeof(double) * b->dpool, g_stream));
   if (p->flux_correct)
     for (int q = 0; q < 3; q++) HIPCHK(hipMemsetAsync(b->fb[q], 0, sizeof(double) * b->bpool, g_stream));
   HIPCHK(hipMemsetAsync(b->err, 0, sizeof(int), g_stream));

@@ -50,7 +50,7 @@ struct PlainLaunch {
   const double* Sin;      // stage input state (U at stage 1, S after)
   double* Sout;           // stage output state (U at the last stage, S before)
   double* flat;           // flattening coefficients / scratch (cell pool)
-  double* fl[3];          // face-flux pools
+  double* div;            // divergence terms of the sweeps so far (cell pool, 5 + ns fields)
   double* fb[3];          // flux-register pools (nullptr when not correcting)
   double weight;          // flux-register weight of this stage
   int add_flux;           // 0: overwrite (stage 1), 1: accumulate

@@ -391,8 +391,8 @@ struct BlockMeta {
   int64_t base;     // offset (doubles) of variable 0 in the state pools
   int64_t cstride;  // variable stride (doubles) in the state pools
   int64_t obase;    // offset in one-variable cell pools (flattening, scratch)
-  int64_t fbase;    // offset of flux variable 0 in the face pools
-  int64_t fstride;  // flux variable stride in the face pools
+  int64_t dbase;    // offset in the plain form's divergence-term pool (5 + ns
+                    // fields of cstride doubles per block)
   int64_t bbase;    // offset in the flux-register pools (interior faces)
   int nx, ny, nz;   // interior cells
   int NX, NY, NZ;   // guard-celled extents

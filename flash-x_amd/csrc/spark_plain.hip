@@ -1,9 +1,13 @@
 // spark_plain.hip -- the plain FP64 form of the Spark RK stage on gfx950.
 //
 // One launch per kernel per stage over all blocks of one shape (blockIdx.y
-// walks the block list).  The dataflow mirrors the reference per stage
-// (Spark/Hydro.F90-mc:144-194): flattening -> face fluxes per direction
-// (materialised in face arrays) -> conserved update + floors + Gamma EOS.
+// walks the block list).  The dataflow follows the reference per stage
+// (Spark/Hydro.F90-mc:144-194): flattening -> face fluxes per direction ->
+// conserved update + floors + Gamma EOS.  Each sweep turns its face fluxes
+// into its term of the update's flux divergence per cell (the x sweep
+// writes -dt/dx (F+ - F-), the y sweep subtracts dt/dy (G+ - G-), in the
+// update's own order) and the last sweep finishes the sum and applies the
+// update, so no face array reaches HBM.
 // Built with -ffp-contract=off: every FP64 expression rounds exactly as the
 // Fortran line it restates, so results match the CPU restatement bit for
 // bit (IEEE division and square root on both sides).
@@ -108,34 +112,29 @@ __global__ void __launch_bounds__(256) k_flatten(DevParams P, const BlockMeta* _
 // ---------------------------------------------------------------------------
 // conserved update + floors of one cell (hy_rk_updateSoln.F90-mc:211-287)
 // and the Gamma EOS in MODE_DENS_EI (Eos_wrapped over the same limits).
-// lo[d] / hi[d] point at flux 0 of the cell's low / high face in direction
-// d, flux n at lo[d][n * fst[d]].  Every load of the cell (the U0 and
-// stage-input state, the face fluxes) is issued before the first use
-// waits: scalar slots past ns re-read scalar ns-1, so no load sits behind a
-// branch.  u0 and us may alias (the last stage writes U): only the cell's
-// own values are read and written.
-template <int NSM, int NDIM>
+// inner_of(n) gives flux n's divergence term of the cell before sources,
+// -dt/dx (facP F+ - facM F-) - dt/dy (G+ - G-) - dt/dz (H+ - H-), in that
+// order.  Every load of the cell (the U0 and stage-input state, the
+// divergence terms) is issued before the first use waits: scalar slots
+// past ns re-read scalar ns-1, so no load sits behind a branch.  u0 and us
+// may alias (the last stage writes U): only the cell's own values are read
+// and written.
+template <int NSM, class InnerF>
 __device__ __forceinline__ void update_cell(const DevParams& P, const BlockMeta& m, int i, int j, int64_t c,
-                                            const double* u0, const double* ui, double* us,
-                                            const double* const (&lo)[3], const double* const (&hi)[3],
-                                            const int64_t (&fst)[3], double dt, double c1, double c2, double c3,
-                                            int final_stage) {
+                                            const double* u0, const double* ui, double* us, InnerF inner_of,
+                                            double dt, double c1, double c2, double c3, int final_stage) {
   const int ns = P.ns;
-  const double dx = m.dx, dy = m.dy, dz = m.dz;
+  const double dx = m.dx;
   const int64_t cs = m.cstride;
-  // hydro part: its 10 state and 10 NDIM face-flux loads issue together
-  double A[F_NUM], B[F_NUM], fl0[F_NUM], fh0[F_NUM], fl1[F_NUM], fh1[F_NUM], fl2[F_NUM], fh2[F_NUM];
+  // hydro part: its 10 state loads and 5 divergence terms issue together
+  double A[F_NUM], B[F_NUM], div[F_NUM];
   {
     const int sv[5] = {V_DENS, V_VELX, V_VELY, V_VELZ, V_ENER};
 #pragma unroll
     for (int q = 0; q < F_NUM; q++) {
       A[q] = u0[sv[q] * cs + c];
       B[q] = ui[sv[q] * cs + c];
-      fl0[q] = lo[0][q * fst[0]];
-      fh0[q] = hi[0][q * fst[0]];
-      fl1[q] = fh1[q] = fl2[q] = fh2[q] = 0.;
-      if (NDIM > 1) { fl1[q] = lo[1][q * fst[1]]; fh1[q] = hi[1][q * fst[1]]; }
-      if (NDIM > 2) { fl2[q] = lo[2][q * fst[2]]; fh2[q] = hi[2][q * fst[2]]; }
+      div[q] = inner_of(q);
     }
   }
   const double d0 = A[F_MASS];
@@ -160,10 +159,7 @@ __device__ __forceinline__ void update_cell(const DevParams& P, const BlockMeta&
                 ui[V_PRES * cs + c], B[F_YMOM], Sgeo);
 #pragma unroll
   for (int n = 0; n < F_NUM; n++) {
-    double inner = -(dt / dx * (facP * fh0[n] - facM * fl0[n]));
-    if (NDIM > 1) inner = inner - dt / dy * (fh1[n] - fl1[n]);
-    if (NDIM == 3) inner = inner - dt / dz * (fh2[n] - fl2[n]);
-    inner = inner + dt * (Sgeo[n] + Sg[n]);
+    const double inner = div[n] + dt * (Sgeo[n] + Sg[n]);
     B[n] = c1 * A[n] + c2 * B[n] + c3 * inner;
   }
   const double dens = fmax2(B[F_MASS], P.smlrho);
@@ -171,26 +167,21 @@ __device__ __forceinline__ void update_cell(const DevParams& P, const BlockMeta&
   const double ener = fmax2(P.smalle, B[F_ENER] / dens);
   const double ekin = .5 * (vx * vx + vy * vy + vz * vz);
   const double eint = fmax2(P.smalle, ener - ekin);
-  // mass scalars, four at a time: the 8 state and 8 NDIM face-flux loads
+  // mass scalars, four at a time: the 8 state loads and 4 divergence terms
   // of a group issue together (slots past ns re-read scalar ns-1); the
-  // scheduling barriers keep one group's loads in registers at a time (the
-  // caller's occupancy, not this cell's latency, hides the memory)
+  // scheduling barriers keep one group's values in registers at a time
   double Xs[NSM > 0 ? NSM : 1];  // the mass fractions (species first) the EOS mixes
 #pragma unroll
   for (int v0 = 0; v0 < NSM; v0 += 4) {
     __builtin_amdgcn_sched_barrier(0);
-    double a4[4], b4[4], l0[4], h0[4], l1[4], h1[4], l2[4], h2[4];
+    double a4[4], b4[4], d4[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const int v = v0 + q < ns ? v0 + q : ns - 1;
       const int64_t o = (int64_t)(V_NPROP + v) * cs + c;
       a4[q] = u0[o];
       b4[q] = ui[o];
-      l0[q] = lo[0][(F_NUM + v) * fst[0]];
-      h0[q] = hi[0][(F_NUM + v) * fst[0]];
-      l1[q] = h1[q] = l2[q] = h2[q] = 0.;
-      if (NDIM > 1) { l1[q] = lo[1][(F_NUM + v) * fst[1]]; h1[q] = hi[1][(F_NUM + v) * fst[1]]; }
-      if (NDIM > 2) { l2[q] = lo[2][(F_NUM + v) * fst[2]]; h2[q] = hi[2][(F_NUM + v) * fst[2]]; }
+      d4[q] = inner_of(F_NUM + v);
     }
 #pragma unroll
     for (int q = 0; q < 4; q++) {
@@ -198,10 +189,7 @@ __device__ __forceinline__ void update_cell(const DevParams& P, const BlockMeta&
       if (v < ns) {
         const double As = a4[q] * d0;
         double Bs = b4[q] * ds;
-        double inner = -(dt / dx * (facP * h0[q] - facM * l0[q]));
-        if (NDIM > 1) inner = inner - dt / dy * (h1[q] - l1[q]);
-        if (NDIM == 3) inner = inner - dt / dz * (h2[q] - l2[q]);
-        inner = inner + dt * (0.0 + 0.0);
+        const double inner = d4[q] + dt * (0.0 + 0.0);
         Bs = c1 * As + c2 * Bs + c3 * inner;
         Xs[v] = Bs / dens;
         us[(V_NPROP + v) * cs + c] = Xs[v];
@@ -224,11 +212,11 @@ __device__ __forceinline__ void update_cell(const DevParams& P, const BlockMeta&
   }
 }
 
-// what the sweep of the last direction needs to apply the stage update
+// what a sweep needs for its part of the stage update
 struct UpdArgs {
-  const double* U0;     // step input (U); may alias Sout
-  double* Sout;         // stage output state
-  const double* fl[3];  // face fluxes of the earlier sweeps
+  const double* U0;  // step input (U); may alias Sout
+  double* Sout;      // stage output state
+  double* div;       // the divergence terms of the sweeps so far (5 + ns per cell)
   const double* dtp;
   double c1, c2, c3;
   int final_stage;
@@ -247,11 +235,11 @@ struct UpdArgs {
 // current one is reconstructed: staging them in LDS, one variable per
 // barrier, was measured slower (the kernel is bound by FP64 VALU issue,
 // not by its loads).
-template <int DIR, int RECON, int RIEMANN, int NSM, bool UPD>
+template <int DIR, int RECON, int RIEMANN, int NSM, int STEP>
 __global__ void __launch_bounds__(1024) k_flux(DevParams P, const BlockMeta* __restrict__ meta,
                                                const int* __restrict__ blist, const double* __restrict__ S,
                                                const double* __restrict__ Ushok, const double* __restrict__ flat,
-                                               double* __restrict__ fl, int ngoff, int lanes, int* __restrict__ err,
+                                               int ngoff, int lanes, int* __restrict__ err,
                                                double* __restrict__ fb, double weight, int add, UpdArgs UA) {
   constexpr int R = ReconRadius<RECON>::R;
   constexpr int NRM = R_NUM + NSM;
@@ -403,12 +391,6 @@ __global__ void __launch_bounds__(1024) k_flux(DevParams P, const BlockMeta* __r
         else F[F_NUM + v] = right[R_NUM + v] * F[F_MASS];
       }
     }
-    if (!UPD) {
-      const int64_t f = m.fbase + fix(m, ijk[0], ijk[1], ijk[2]);
-#pragma unroll
-      for (int n = 0; n < NFM; n++)
-        if (n < F_NUM + ns) fl[f + n * m.fstride] = F[n];
-    }
     // flux registers (addFluxes, Hydro_funcs.F90-mc:481-507) of the block's
     // two boundary faces of this direction: the register hand-off reads no
     // other face (Grid_putFluxData_block.F90:200-202, 278-279), so the
@@ -450,43 +432,46 @@ __global__ void __launch_bounds__(1024) k_flux(DevParams P, const BlockMeta* __r
       }
     }
   }
-  if (UPD) {
-    // the sweep of the last direction also applies the stage's conserved
-    // update to the region's cells (s = 1 .. Lr - 2): the face fluxes of
-    // this direction come through LDS (cell s owns its low face, thread
-    // s + 1 computed its high face), the others from the earlier sweeps
-    __syncthreads();  // every read of the exchange buffer is done
-    if (face) {
+  // the face fluxes of this direction through LDS (cell s owns its low
+  // face, thread s + 1 computed its high face), then per cell of the region
+  // (s = 1 .. Lr - 2) this direction's divergence term: STEP 1 (x, not the
+  // last direction) writes it, STEP 2 (y before z) adds it to the x term,
+  // STEP 3 (the last direction) adds it to the earlier terms and applies
+  // the update
+  __syncthreads();  // every read of the exchange buffer is done
+  if (face) {
 #pragma unroll
-      for (int n = 0; n < NFM; n++)
-        if (n < F_NUM + ns) sm[n * NT + tid] = F[n];
-    }
-    __syncthreads();
-    if (valid && s >= 1 && s <= Lr - 2) {
-      const int th = (DIR == 0) ? tid + 1 : tid + lanes;
-      const double* lo[3];
-      const double* hi[3];
-      int64_t fst[3];
+    for (int n = 0; n < NFM; n++)
+      if (n < F_NUM + ns) sm[n * NT + tid] = F[n];
+  }
+  __syncthreads();
+  if (!valid || s < 1 || s > Lr - 2) return;
+  const int th = (DIR == 0) ? tid + 1 : tid + lanes;
+  const double dt = *UA.dtp;
+  const double dd = DIR == 0 ? m.dx : DIR == 1 ? m.dy : m.dz;
+  double facM = 1.0, facP = 1.0, rc = 0.0;
+  if (DIR == 0) (void)geo_facs(P, m.id, ijk[0], ijk[1], m.dx, facM, facP, rc);
+  double* dv = UA.div + m.dbase + c;
+  // flux n's term, added to the earlier directions' (prev) as the update
+  // writes it: -(dt/dx (facP F+ - facM F-)), then - dt/dy (G+ - G-), ...
+  auto term = [&](int n, double prev) -> double {
+    const double lo = sm[n * NT + tid], hi = sm[n * NT + th];
+    if (DIR == 0) return -(dt / dd * (facP * hi - facM * lo));
+    return prev - dt / dd * (hi - lo);
+  };
+  if (STEP == 3) {
+    auto inner_of = [&](int n) -> double { return term(n, DIR == 0 ? 0.0 : dv[n * m.cstride]); };
+    update_cell<NSM>(P, m, ijk[0], ijk[1], c, UA.U0 + m.base, sp, UA.Sout + m.base, inner_of, dt, UA.c1, UA.c2,
+                     UA.c3, UA.final_stage);
+  } else {
+    // every earlier term is loaded before the first store (a slot past
+    // F_NUM + ns re-reads the last)
+    double prev[NFM];
 #pragma unroll
-      for (int d = 0; d < 3; d++) {
-        if (d > DIR) {  // no such direction (DIR is the last)
-          lo[d] = hi[d] = nullptr;
-          fst[d] = 0;
-        } else if (d == DIR) {
-          lo[d] = sm + tid;
-          hi[d] = sm + th;
-          fst[d] = NT;
-        } else {
-          int up[3] = {ijk[0], ijk[1], ijk[2]};
-          up[d] += 1;
-          lo[d] = UA.fl[d] + m.fbase + fix(m, ijk[0], ijk[1], ijk[2]);
-          hi[d] = UA.fl[d] + m.fbase + fix(m, up[0], up[1], up[2]);
-          fst[d] = m.fstride;
-        }
-      }
-      update_cell<NSM, DIR + 1>(P, m, ijk[0], ijk[1], c, UA.U0 + m.base, sp, UA.Sout + m.base, lo, hi, fst,
-                                *UA.dtp, UA.c1, UA.c2, UA.c3, UA.final_stage);
-    }
+    for (int n = 0; n < NFM; n++) prev[n] = STEP == 2 ? dv[(n < F_NUM + ns ? n : F_NUM + ns - 1) * m.cstride] : 0.0;
+#pragma unroll
+    for (int n = 0; n < NFM; n++)
+      if (n < F_NUM + ns) dv[n * m.cstride] = term(n, prev[n]);
   }
 }
 
@@ -955,7 +940,7 @@ static int resident_waves(const void* kern, int threads, size_t shm) {
   return res;
 }
 
-template <int DIR, int RECON, int RIEMANN, int NSM, bool UPD>
+template <int DIR, int RECON, int RIEMANN, int NSM, int STEP>
 static hipError_t launch_flux_t(const PlainLaunch& L, int ngoff, const UpdArgs& UA) {
   // geometry of this stage (all blocks in the list share one shape)
   const int n[3] = {L.nx, L.ny, L.nz};
@@ -989,7 +974,7 @@ static hipError_t launch_flux_t(const PlainLaunch& L, int ngoff, const UpdArgs& 
   };
   // cost in units of 1/1000 wave per resident wave
   auto cost = [&](int l) {
-    const int res = resident_waves(reinterpret_cast<const void*>(&k_flux<DIR, RECON, RIEMANN, NSM, UPD>), l * Lr,
+    const int res = resident_waves(reinterpret_cast<const void*>(&k_flux<DIR, RECON, RIEMANN, NSM, STEP>), l * Lr,
                                    sizeof(double) * (size_t)(NRM + 6) * l * Lr);
     return (int64_t)waves_of(l) * 1000 / std::max(res, 1);
   };
@@ -1011,52 +996,51 @@ static hipError_t launch_flux_t(const PlainLaunch& L, int ngoff, const UpdArgs& 
   const size_t shm = sizeof(double) * (size_t)(NRM + 6) * block.x;
   if (L.prof) {
     // algorithmic traffic: every state value of the stencil box read once
-    // (dens velx vely velz pres gamc eint shok, scalars, flat), every face
-    // flux written once; flops: 117 per variable per reconstructed cell
-    // (WENO5-Z) + 175 per face (HLLC, viscosity, floors)
+    // (dens velx vely velz pres gamc eint shok, scalars, flat); per region
+    // cell the divergence terms (5 + ns) written (STEP 1), read and written
+    // (STEP 2) or read (STEP 3 after another direction); STEP 3 also reads
+    // U0 and the stage input (5 + ns each) and writes the stage output
+    // (8 + ns, + temp and game at the last stage).  Flops: 117 per variable
+    // per reconstructed cell (WENO5-Z) + 175 per face (HLLC, viscosity,
+    // floors) + 3 per flux per cell for the divergence term, + the update
     const double cells_in = (double)(Lr + 2 * ReconRadius<RECON>::R) * nt1 * nt2;
     const double recon_cells = (double)Lr * nt1 * nt2;
     const double faces = (double)(Lr - 1) * nt1 * nt2;
+    const double box = (double)(Lr - 2) * nt1 * nt2;
     const int ns = L.P.ns;
+    const double ndiv = (STEP == 1 ? 1 : STEP == 2 ? 2 : DIR > 0 ? 1 : 0);
     double bytes = 8.0 * L.nblk * (cells_in * (8 + ns + (RECON == RECON_WENO5 && L.P.use_flat)) +
-                                   (UPD ? 0.0 : faces * (5 + ns)));
-    double flops = (double)L.nblk * (recon_cells * (7 + ns) * 117.0 + faces * (175.0 + ns));
-    if (UPD) {
-      // the update of the region's cells: U0 and the stage input (5 + ns
-      // each) and the earlier sweeps' face fluxes read, the stage output
-      // (8 + ns, + temp and game at the last stage) written
-      const double box = (double)(Lr - 2) * nt1 * nt2;
-      double ofaces = 0.;
-      for (int d = 0; d < DIR; d++) ofaces += box / (hi[d] - lo[d] + 1) * (hi[d] - lo[d] + 2);
-      bytes += 8.0 * L.nblk * (box * (2 * (5 + ns) + 8 + ns + 2 * UA.final_stage) + ofaces * (5 + ns));
-      flops += (double)L.nblk * box * (60.0 + 10.0 * (DIR + 1) * (5 + ns));
+                                   box * ndiv * (5 + ns));
+    double flops = (double)L.nblk * (recon_cells * (7 + ns) * 117.0 + faces * (175.0 + ns) + box * 3.0 * (5 + ns));
+    if (STEP == 3) {
+      bytes += 8.0 * L.nblk * box * (2 * (5 + ns) + 8 + ns + 2 * UA.final_stage);
+      flops += (double)L.nblk * box * (60.0 + 4.0 * (5 + ns));
     }
     static const char* names[3] = {"k_flux_x", "k_flux_y", "k_flux_z"};
     L.prof->begin(L.stream, names[DIR], bytes, flops);
   }
-  hipLaunchKernelGGL((k_flux<DIR, RECON, RIEMANN, NSM, UPD>), grid, block, shm, L.stream, L.P, L.meta, L.blist,
-                     L.Sin, L.U0, L.flat, UPD ? nullptr : L.fl[DIR], ngoff, lanes, L.err, L.fb[DIR], L.weight,
-                     L.add_flux, UA);
+  hipLaunchKernelGGL((k_flux<DIR, RECON, RIEMANN, NSM, STEP>), grid, block, shm, L.stream, L.P, L.meta, L.blist,
+                     L.Sin, L.U0, L.flat, ngoff, lanes, L.err, L.fb[DIR], L.weight, L.add_flux, UA);
   if (L.prof) L.prof->end(L.stream);
   return hipGetLastError();
 }
 
-template <int DIR, int NSM, bool UPD>
+template <int DIR, int NSM, int STEP>
 static hipError_t launch_flux_r(const PlainLaunch& L, int ngoff, const UpdArgs& UA) {
   const int r = L.P.recon, q = L.P.riemann;
   if (q == RIEMANN_HLL) {
     switch (r) {
-      case RECON_MP5: return launch_flux_t<DIR, RECON_MP5, RIEMANN_HLL, NSM, UPD>(L, ngoff, UA);
-      case RECON_PLM: return launch_flux_t<DIR, RECON_PLM, RIEMANN_HLL, NSM, UPD>(L, ngoff, UA);
-      case RECON_FOG: return launch_flux_t<DIR, RECON_FOG, RIEMANN_HLL, NSM, UPD>(L, ngoff, UA);
-      default: return launch_flux_t<DIR, RECON_WENO5, RIEMANN_HLL, NSM, UPD>(L, ngoff, UA);
+      case RECON_MP5: return launch_flux_t<DIR, RECON_MP5, RIEMANN_HLL, NSM, STEP>(L, ngoff, UA);
+      case RECON_PLM: return launch_flux_t<DIR, RECON_PLM, RIEMANN_HLL, NSM, STEP>(L, ngoff, UA);
+      case RECON_FOG: return launch_flux_t<DIR, RECON_FOG, RIEMANN_HLL, NSM, STEP>(L, ngoff, UA);
+      default: return launch_flux_t<DIR, RECON_WENO5, RIEMANN_HLL, NSM, STEP>(L, ngoff, UA);
     }
   }
   switch (r) {
-    case RECON_MP5: return launch_flux_t<DIR, RECON_MP5, RIEMANN_HLLC, NSM, UPD>(L, ngoff, UA);
-    case RECON_PLM: return launch_flux_t<DIR, RECON_PLM, RIEMANN_HLLC, NSM, UPD>(L, ngoff, UA);
-    case RECON_FOG: return launch_flux_t<DIR, RECON_FOG, RIEMANN_HLLC, NSM, UPD>(L, ngoff, UA);
-    default: return launch_flux_t<DIR, RECON_WENO5, RIEMANN_HLLC, NSM, UPD>(L, ngoff, UA);
+    case RECON_MP5: return launch_flux_t<DIR, RECON_MP5, RIEMANN_HLLC, NSM, STEP>(L, ngoff, UA);
+    case RECON_PLM: return launch_flux_t<DIR, RECON_PLM, RIEMANN_HLLC, NSM, STEP>(L, ngoff, UA);
+    case RECON_FOG: return launch_flux_t<DIR, RECON_FOG, RIEMANN_HLLC, NSM, STEP>(L, ngoff, UA);
+    default: return launch_flux_t<DIR, RECON_WENO5, RIEMANN_HLLC, NSM, STEP>(L, ngoff, UA);
   }
 }
 
@@ -1082,26 +1066,24 @@ hipError_t plain_stage(const PlainLaunch& L, int ngoff, double c1, double c2, do
     if (L.prof) L.prof->end(L.stream);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
-  // the sweep of the last direction applies the update; it reads the stage
-  // input around each cell while writing the stage output, so the two
-  // must be different buffers
+  // the last sweep applies the update; it reads the stage input around each
+  // cell while writing the stage output, so the two must be different
+  // buffers
   if (L.Sin == L.Sout) return hipErrorInvalidValue;
-  const UpdArgs UA{L.U0, L.Sout, {L.fl[0], L.fl[1], L.fl[2]}, L.dtp, c1, c2, c3, final_stage};
+  const UpdArgs UA{L.U0, L.Sout, L.div, L.dtp, c1, c2, c3, final_stage};
   const int nd = L.P.ndim;
-  for (int dir = 0; dir < nd; dir++) {
-    const bool last = dir == nd - 1;
-    if (L.P.ns == 0) {
-      if (dir == 0) e = last ? launch_flux_r<0, 0, true>(L, ngoff, UA) : launch_flux_r<0, 0, false>(L, ngoff, UA);
-      else if (dir == 1) e = last ? launch_flux_r<1, 0, true>(L, ngoff, UA) : launch_flux_r<1, 0, false>(L, ngoff, UA);
-      else e = launch_flux_r<2, 0, true>(L, ngoff, UA);
-    } else {
-      if (dir == 0) e = last ? launch_flux_r<0, 8, true>(L, ngoff, UA) : launch_flux_r<0, 8, false>(L, ngoff, UA);
-      else if (dir == 1) e = last ? launch_flux_r<1, 8, true>(L, ngoff, UA) : launch_flux_r<1, 8, false>(L, ngoff, UA);
-      else e = launch_flux_r<2, 8, true>(L, ngoff, UA);
-    }
-    if (e != hipSuccess) return e;
+  if (L.P.ns == 0) {
+    if (nd == 1) return launch_flux_r<0, 0, 3>(L, ngoff, UA);
+    if ((e = launch_flux_r<0, 0, 1>(L, ngoff, UA)) != hipSuccess) return e;
+    if (nd == 2) return launch_flux_r<1, 0, 3>(L, ngoff, UA);
+    if ((e = launch_flux_r<1, 0, 2>(L, ngoff, UA)) != hipSuccess) return e;
+    return launch_flux_r<2, 0, 3>(L, ngoff, UA);
   }
-  return hipSuccess;
+  if (nd == 1) return launch_flux_r<0, 8, 3>(L, ngoff, UA);
+  if ((e = launch_flux_r<0, 8, 1>(L, ngoff, UA)) != hipSuccess) return e;
+  if (nd == 2) return launch_flux_r<1, 8, 3>(L, ngoff, UA);
+  if ((e = launch_flux_r<1, 8, 2>(L, ngoff, UA)) != hipSuccess) return e;
+  return launch_flux_r<2, 8, 3>(L, ngoff, UA);
 }
 
 // band: SHOK is computed over interior +/- band (the reference uses
