@@ -328,6 +328,12 @@ template <int NSM, int RECON, int RIEMANN>
 #ifndef FUSED_XY_UNROLL
 #define FUSED_XY_UNROLL 6
 #endif
+#ifndef FUSED_EARLY_Z
+#define FUSED_EARLY_Z kF64
+#endif
+#ifndef EARLY_Z_BARRIER
+#define EARLY_Z_BARRIER
+#endif
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_WAVES_OF(NSM)))) FUSED_NAME(k_fused, FUSED_SFX)(FusedArgs A) {
   constexpr int NFM = F_NUM + NSM;
   constexpr int NSA = NSM > 0 ? NSM : 1;
@@ -700,30 +706,38 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
           {
             const int nrxp = rup64(nrx);
             const int utot = Cp + (xyk ? nrxp + nry : 0);
-            for (int u = tid; u < utot; u += NT) {
+            // the column's z stencils (task u = tid < C); a slot past nc
+            // loads scalar nc-1 again.  The FP64 form loads them first and
+            // runs the thread's x/y tasks (u >= Cp, stencils in LDS) while
+            // they are in flight; the FP32 form (two waves per SIMD, no
+            // registers to spare) loads and reconstructs them in its column
+            // task
+            Real u5s[CH][5];
+            auto load_z = [&]() {
+              int64_t zc[5];
+#pragma unroll
+              for (int w = 0; w < 5; w++) zc[w] = cell_at(m, nbt, g, ci, cj, k - 2 + w);
+#pragma unroll
+              for (int j = 0; j < CH; j++) {
+                const double* sp = S + (int64_t)(V_NPROP + c0 + (j < nc ? j : nc - 1)) * cs;
+#pragma unroll
+                for (int w = 0; w < 5; w++) u5s[j][w] = (Real)sp[zc[w]];
+              }
+              EARLY_Z_BARRIER;
+            };
+            auto recon_z = [&]() {
+              const Real fl = use_flat ? flatb[fcc] : Real(1.);
+#pragma unroll
+              for (int j = 0; j < CH; j++)
+                if (j < nc) recon1<RECON, Real>(u5s[j], fl, upsk[j], ums[j]);
+            };
+            constexpr bool early = FUSED_EARLY_Z;
+            if (early && col) load_z();
+            for (int u = early && tid < Cp ? tid + NT : tid; u < utot; u += NT) {
               if (u < Cp) {
                 if (u < C) {
-                  const Real fl = use_flat ? flatb[fcc] : Real(1.);
-                  int64_t zc[5];
-#pragma unroll
-                  for (int w = 0; w < 5; w++) zc[w] = cell_at(m, nbt, g, ci, cj, k - 2 + w);
-                  // the z stencils of two scalars load before the first of
-                  // their reconstructions waits (a slot past nc loads scalar
-                  // nc-1 again)
-#pragma unroll
-                  for (int j0 = 0; j0 < CH; j0 += 2) {
-                    Real u5s[2][5];
-#pragma unroll
-                    for (int jj = 0; jj < 2; jj++) {
-                      const int j = j0 + jj;
-                      const double* sp = S + (int64_t)(V_NPROP + c0 + (j < nc ? j : nc - 1)) * cs;
-#pragma unroll
-                      for (int w = 0; w < 5; w++) u5s[jj][w] = (Real)sp[zc[w]];
-                    }
-#pragma unroll
-                    for (int jj = 0; jj < 2; jj++)
-                      if (j0 + jj < nc) recon1<RECON, Real>(u5s[jj], fl, upsk[j0 + jj], ums[j0 + jj]);
-                  }
+                  load_z();
+                  recon_z();
                 }
                 continue;
               }
@@ -746,6 +760,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FUSED_W
                 op[j * nr + w] = up;
               }
             }
+            if (early && col) recon_z();
           }
           __syncthreads();
           // S2: upwinded scalar fluxes
