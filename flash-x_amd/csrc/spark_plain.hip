@@ -236,13 +236,12 @@ struct UpdArgs {
 // barrier, was measured slower (the kernel is bound by FP64 VALU issue,
 // not by its loads).
 template <int DIR, int RECON, int RIEMANN, int NSM, int STEP>
-__global__ void __launch_bounds__(1024) k_flux(DevParams P, const BlockMeta* __restrict__ meta,
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(STEP == 3 ? 4 : 5))) k_flux(DevParams P, const BlockMeta* __restrict__ meta,
                                                const int* __restrict__ blist, const double* __restrict__ S,
                                                const double* __restrict__ Ushok, const double* __restrict__ flat,
                                                int ngoff, int lanes, int* __restrict__ err,
                                                double* __restrict__ fb, double weight, int add, UpdArgs UA) {
   constexpr int R = ReconRadius<RECON>::R;
-  constexpr int NRM = R_NUM + NSM;
   constexpr int NFM = F_NUM + NSM;
   extern __shared__ double sm[];
   const int ns = P.ns;
@@ -274,90 +273,67 @@ __global__ void __launch_bounds__(1024) k_flux(DevParams P, const BlockMeta* __r
   const double* sp = S + m.base;
   const int64_t cs = m.cstride;
 
-  double umr[NRM];
+  // hydro variables first: their face states and centred values go through
+  // LDS, the Riemann solve runs; the mass scalars follow through the same
+  // LDS slots (the solve reads only the hydro states, the scalar fluxes
+  // only the solve's mass flux), so neither the LDS buffer nor the
+  // registers hold all variables at once
+  const double fl_c = (valid && RECON == RECON_WENO5 && P.use_flat) ? flat[m.obase + c] : 1.0;
+  // variables in order dens, velx, vely, velz, pres, gamc, eint (rhoe =
+  // dens * eint per cell), then the scalars; the stencil of variable v+1 is
+  // loaded before variable v is reconstructed (software pipelining: the
+  // scheduling barrier keeps the loads ahead of the reconstruction)
+  auto var_off = [&](int v) -> int64_t {
+    const int sv[7] = {V_DENS, V_VELX, V_VELY, V_VELZ, V_PRES, V_GAMC, V_EINT};
+    return (int64_t)(v < R_NUM ? sv[v < R_NUM ? v : 0] : V_NPROP + (v - R_NUM < ns ? v - R_NUM : ns - 1)) * cs;
+  };
+  auto stencil = [&](int v, double (&u)[5]) {
+#pragma unroll
+    for (int q = 0; q < 5; q++) u[q] = (q >= 2 - R && q <= 2 + R) ? sp[var_off(v) + c + (q - 2) * st] : 0.0;
+  };
+  double umr[R_NUM];
   if (valid) {
-    const double fl_c = (RECON == RECON_WENO5 && P.use_flat) ? flat[m.obase + c] : 1.0;
-    // variables in order dens, velx, vely, velz, pres, gamc, eint (rhoe =
-    // dens * eint per cell), the scalars; the stencil of variable v+1 is
-    // loaded before variable v is reconstructed (software pipelining: the
-    // scheduling barrier keeps the loads ahead of the reconstruction)
-    constexpr int NV = R_NUM + NSM;
-    const int nv = R_NUM + ns;
-    auto var_off = [&](int v) -> int64_t {
-      const int sv[7] = {V_DENS, V_VELX, V_VELY, V_VELZ, V_PRES, V_GAMC, V_EINT};
-      return (int64_t)(v < R_NUM ? sv[v < R_NUM ? v : 0] : V_NPROP + (v - R_NUM < ns ? v - R_NUM : ns - 1)) * cs;
-    };
     double ud[5], cur[5], nxt[5], up, um;
+    stencil(0, nxt);
 #pragma unroll
-    for (int q = 0; q < 5; q++) nxt[q] = (q >= 2 - R && q <= 2 + R) ? sp[var_off(0) + c + (q - 2) * st] : 0.0;
+    for (int v = 0; v < R_NUM; v++) {
 #pragma unroll
-    for (int v = 0; v < NV; v++) {
-      if (v < nv) {
+      for (int q = 0; q < 5; q++) cur[q] = nxt[q];
+      if (v + 1 < R_NUM) stencil(v + 1, nxt);
+      __builtin_amdgcn_sched_barrier(0);
+      if (v == R_DENS) {
 #pragma unroll
-        for (int q = 0; q < 5; q++) cur[q] = nxt[q];
-        if (v + 1 < NV) {
-#pragma unroll
-          for (int q = 0; q < 5; q++)
-            nxt[q] = (q >= 2 - R && q <= 2 + R) ? sp[var_off(v + 1) + c + (q - 2) * st] : 0.0;
-        }
-        __builtin_amdgcn_sched_barrier(0);
-        if (v == R_DENS) {
-#pragma unroll
-          for (int q = 0; q < 5; q++) ud[q] = cur[q];
-        }
-        if (v == R_RHOE) {
-#pragma unroll
-          for (int q = -R; q <= R; q++) cur[q + 2] = ud[q + 2] * cur[q + 2];
-        }
-        recon1<RECON>(cur, fl_c, up, um);
-        sm[v * NT + tid] = up;
-        umr[v] = um;
-        if (v == R_DENS) sm[(NRM + 0) * NT + tid] = cur[2];
-        else if (v <= R_VELZ) sm[(NRM + v) * NT + tid] = cur[2];
-        else if (v == R_RHOE) sm[(NRM + 4) * NT + tid] = cur[2];
+        for (int q = 0; q < 5; q++) ud[q] = cur[q];
       }
+      if (v == R_RHOE) {
+#pragma unroll
+        for (int q = -R; q <= R; q++) cur[q + 2] = ud[q + 2] * cur[q + 2];
+      }
+      recon1<RECON>(cur, fl_c, up, um);
+      sm[v * NT + tid] = up;
+      umr[v] = um;
+      if (v == R_DENS) sm[(R_NUM + 0) * NT + tid] = cur[2];
+      else if (v <= R_VELZ) sm[(R_NUM + v) * NT + tid] = cur[2];
+      else if (v == R_RHOE) sm[(R_NUM + 4) * NT + tid] = cur[2];
     }
-    sm[(NRM + 5) * NT + tid] = Ushok[m.base + V_SHOK * cs + c];
+    sm[(R_NUM + 5) * NT + tid] = Ushok[m.base + V_SHOK * cs + c];
   }
   __syncthreads();
   const bool face = valid && s >= 1;
+  const int tn = (DIR == 0) ? tid - 1 : tid - lanes;
   double F[NFM];
   if (face) {
-    const int tn = (DIR == 0) ? tid - 1 : tid - lanes;
-    double left[NRM], right[NRM];
+    double left[R_NUM], right[R_NUM];
 #pragma unroll
-    for (int v = 0; v < NRM; v++) {
-      if (v < R_NUM + ns) {
-        left[v] = sm[v * NT + tn];
-        right[v] = umr[v];
-      }
+    for (int v = 0; v < R_NUM; v++) {
+      left[v] = sm[v * NT + tn];
+      right[v] = umr[v];
     }
     left[R_DENS] = fmax2(P.smlrho, left[R_DENS]);
     left[R_PRES] = fmax2(P.smallp, left[R_PRES]);
     right[R_DENS] = fmax2(P.smlrho, right[R_DENS]);
     right[R_PRES] = fmax2(P.smallp, right[R_PRES]);
-    if (NSM > 0 && P.nspec > 0) {
-      // species: clamp to [smallx, 1] and renormalise (Spark/Hydro.ini:227-235)
-      double sl = 0.0, sr = 0.0;
-#pragma unroll
-      for (int v = 0; v < NSM; v++) {
-        if (v < P.nspec) {
-          left[R_NUM + v] = fmax2(P.smallx, fmin2(1.0, left[R_NUM + v]));
-          right[R_NUM + v] = fmax2(P.smallx, fmin2(1.0, right[R_NUM + v]));
-          sl = sl + left[R_NUM + v];
-          sr = sr + right[R_NUM + v];
-        }
-      }
-      const double il = 1. / sl, ir = 1. / sr;
-#pragma unroll
-      for (int v = 0; v < NSM; v++) {
-        if (v < P.nspec) {
-          left[R_NUM + v] = left[R_NUM + v] * il;
-          right[R_NUM + v] = right[R_NUM + v] * ir;
-        }
-      }
-    }
-    const bool inShock = (sm[(NRM + 5) * NT + tn] != 0.0) || (sm[(NRM + 5) * NT + tid] != 0.0);
+    const bool inShock = (sm[(R_NUM + 5) * NT + tn] != 0.0) || (sm[(R_NUM + 5) * NT + tid] != 0.0);
     int ierr;
     if (RIEMANN == RIEMANN_HLL) ierr = hll<DIR, double>(left, right, F);
     else ierr = hllc<DIR, double>(left, right, P.hybrid != 0, inShock, P.tiny, F);
@@ -368,11 +344,11 @@ __global__ void __launch_bounds__(1024) k_flux(DevParams P, const BlockMeta* __r
     }
     // artificial viscosity (Hydro.ini:285-301)
     {
-      const double dL = sm[(NRM + 0) * NT + tn], dH = sm[(NRM + 0) * NT + tid];
-      const double xL = sm[(NRM + 1) * NT + tn], xH = sm[(NRM + 1) * NT + tid];
-      const double yL = sm[(NRM + 2) * NT + tn], yH = sm[(NRM + 2) * NT + tid];
-      const double zL = sm[(NRM + 3) * NT + tn], zH = sm[(NRM + 3) * NT + tid];
-      const double eL = sm[(NRM + 4) * NT + tn], eH = sm[(NRM + 4) * NT + tid];
+      const double dL = sm[(R_NUM + 0) * NT + tn], dH = sm[(R_NUM + 0) * NT + tid];
+      const double xL = sm[(R_NUM + 1) * NT + tn], xH = sm[(R_NUM + 1) * NT + tid];
+      const double yL = sm[(R_NUM + 2) * NT + tn], yH = sm[(R_NUM + 2) * NT + tid];
+      const double zL = sm[(R_NUM + 3) * NT + tn], zH = sm[(R_NUM + 3) * NT + tid];
+      const double eL = sm[(R_NUM + 4) * NT + tn], eH = sm[(R_NUM + 4) * NT + tid];
       const double nL = (DIR == 0) ? xL : (DIR == 1) ? yL : zL;
       const double nH = (DIR == 0) ? xH : (DIR == 1) ? yH : zH;
       double cvisc = P.cvisc * fmax2(-(nH - nL), 0.);
@@ -384,13 +360,67 @@ __global__ void __launch_bounds__(1024) k_flux(DevParams P, const BlockMeta* __r
       F[F_ZMOM] = F[F_ZMOM] + cvisc * (dL * zL - dH * zH);
       F[F_ENER] = F[F_ENER] + cvisc * (VenerLo - VenerHi);
     }
+  }
+  if (NSM > 0) {
+    __syncthreads();  // every read of the hydro exchange is done
+    double ums[NSM > 0 ? NSM : 1];
+    if (valid) {
+      double cur[5], nxt[5], up, um;
+      stencil(R_NUM, nxt);
 #pragma unroll
-    for (int v = 0; v < NSM; v++) {
-      if (v < ns) {
-        if (F[F_MASS] > 0.) F[F_NUM + v] = left[R_NUM + v] * F[F_MASS];
-        else F[F_NUM + v] = right[R_NUM + v] * F[F_MASS];
+      for (int v = 0; v < NSM; v++) {
+        if (v < ns) {
+#pragma unroll
+          for (int q = 0; q < 5; q++) cur[q] = nxt[q];
+          if (v + 1 < NSM) stencil(R_NUM + v + 1, nxt);
+          __builtin_amdgcn_sched_barrier(0);
+          recon1<RECON>(cur, fl_c, up, um);
+          sm[v * NT + tid] = up;
+          ums[v] = um;
+        }
       }
     }
+    __syncthreads();
+    if (face) {
+      double xl[NSM > 0 ? NSM : 1], xr[NSM > 0 ? NSM : 1];
+#pragma unroll
+      for (int v = 0; v < NSM; v++) {
+        if (v < ns) {
+          xl[v] = sm[v * NT + tn];
+          xr[v] = ums[v];
+        }
+      }
+      if (P.nspec > 0) {
+        // species: clamp to [smallx, 1] and renormalise (Spark/Hydro.ini:227-235)
+        double sl = 0.0, sr = 0.0;
+#pragma unroll
+        for (int v = 0; v < NSM; v++) {
+          if (v < P.nspec) {
+            xl[v] = fmax2(P.smallx, fmin2(1.0, xl[v]));
+            xr[v] = fmax2(P.smallx, fmin2(1.0, xr[v]));
+            sl = sl + xl[v];
+            sr = sr + xr[v];
+          }
+        }
+        const double il = 1. / sl, ir = 1. / sr;
+#pragma unroll
+        for (int v = 0; v < NSM; v++) {
+          if (v < P.nspec) {
+            xl[v] = xl[v] * il;
+            xr[v] = xr[v] * ir;
+          }
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < NSM; v++) {
+        if (v < ns) {
+          if (F[F_MASS] > 0.) F[F_NUM + v] = xl[v] * F[F_MASS];
+          else F[F_NUM + v] = xr[v] * F[F_MASS];
+        }
+      }
+    }
+  }
+  if (face) {
     // flux registers (addFluxes, Hydro_funcs.F90-mc:481-507) of the block's
     // two boundary faces of this direction: the register hand-off reads no
     // other face (Grid_putFluxData_block.F90:200-202, 278-279), so the
@@ -952,7 +982,9 @@ static hipError_t launch_flux_t(const PlainLaunch& L, int ngoff, const UpdArgs& 
   const int T1 = (DIR == 0) ? 1 : 0, T2 = (DIR == 2) ? 1 : 2;
   const int Lr = hi[DIR] - lo[DIR] + 3;
   const int nt1 = hi[T1] - lo[T1] + 1, nt2 = hi[T2] - lo[T2] + 1;
-  constexpr int NRM = R_NUM + NSM;
+  // LDS slots per thread: the hydro face states, 5 centred values and the
+  // shock flag; then the scalars' face states; then the face fluxes
+  constexpr int LDSV = (R_NUM + 6 > F_NUM + NSM) ? R_NUM + 6 : F_NUM + NSM;
   // pencils per workgroup.  The LDS exchange buffer stays within 80 KiB and
   // the workgroup within 1024 threads.  A workgroup of l pencils runs
   // w = ceil(l Lr / 64) waves; workgroups of 2, 4 or 8 waves run best (3, 5,
@@ -963,7 +995,7 @@ static hipError_t launch_flux_t(const PlainLaunch& L, int ngoff, const UpdArgs& 
   // the CU empty loses), the largest count within 3 % of the best.  Without
   // such a count, the fewest waves.
   const int npen = nt1 * nt2;
-  const int cap = 81920 / (int)(sizeof(double) * (NRM + 6) * Lr);
+  const int cap = 81920 / (int)(sizeof(double) * LDSV * Lr);
   const int lmax = std::max(1, std::min(std::min(1024 / Lr, cap), npen));
   const int lmin = DIR == 0 ? 1 : std::min(8, lmax);
   auto wg_waves = [&](int l) { return (l * Lr + 63) / 64; };
@@ -975,7 +1007,7 @@ static hipError_t launch_flux_t(const PlainLaunch& L, int ngoff, const UpdArgs& 
   // cost in units of 1/1000 wave per resident wave
   auto cost = [&](int l) {
     const int res = resident_waves(reinterpret_cast<const void*>(&k_flux<DIR, RECON, RIEMANN, NSM, STEP>), l * Lr,
-                                   sizeof(double) * (size_t)(NRM + 6) * l * Lr);
+                                   sizeof(double) * (size_t)LDSV * l * Lr);
     return (int64_t)waves_of(l) * 1000 / std::max(res, 1);
   };
   int64_t best = INT64_MAX;
@@ -993,7 +1025,7 @@ static hipError_t launch_flux_t(const PlainLaunch& L, int ngoff, const UpdArgs& 
   const int chunks = (npen + lanes - 1) / lanes;
   dim3 grid(chunks, L.nblk);
   dim3 block(lanes * Lr);
-  const size_t shm = sizeof(double) * (size_t)(NRM + 6) * block.x;
+  const size_t shm = sizeof(double) * (size_t)LDSV * block.x;
   if (L.prof) {
     // algorithmic traffic: every state value of the stencil box read once
     // (dens velx vely velz pres gamc eint shok, scalars, flat); per region
