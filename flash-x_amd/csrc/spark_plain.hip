@@ -123,6 +123,8 @@ template <int NSM, class InnerF>
 __device__ __forceinline__ void update_cell(const DevParams& P, const BlockMeta& m, int i, int j, int64_t c,
                                             const double* u0, const double* ui, double* us, InnerF inner_of,
                                             double dt, double c1, double c2, double c3, int final_stage) {
+  // the first stage's input is U0 itself: its values are loaded once
+  const bool same = u0 == ui;
   const int ns = P.ns;
   const double dx = m.dx;
   const int64_t cs = m.cstride;
@@ -133,8 +135,14 @@ __device__ __forceinline__ void update_cell(const DevParams& P, const BlockMeta&
 #pragma unroll
     for (int q = 0; q < F_NUM; q++) {
       A[q] = u0[sv[q] * cs + c];
-      B[q] = ui[sv[q] * cs + c];
       div[q] = inner_of(q);
+    }
+    if (same) {
+#pragma unroll
+      for (int q = 0; q < F_NUM; q++) B[q] = A[q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < F_NUM; q++) B[q] = ui[sv[q] * cs + c];
     }
   }
   const double d0 = A[F_MASS];
@@ -180,8 +188,12 @@ __device__ __forceinline__ void update_cell(const DevParams& P, const BlockMeta&
       const int v = v0 + q < ns ? v0 + q : ns - 1;
       const int64_t o = (int64_t)(V_NPROP + v) * cs + c;
       a4[q] = u0[o];
-      b4[q] = ui[o];
       d4[q] = inner_of(F_NUM + v);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int v = v0 + q < ns ? v0 + q : ns - 1;
+      b4[q] = same ? a4[q] : ui[(int64_t)(V_NPROP + v) * cs + c];
     }
 #pragma unroll
     for (int q = 0; q < 4; q++) {
