@@ -166,6 +166,35 @@ def test_stage_level_entry_points():
     assert dt == dto
 
 
+def test_stage_level_entry_points_two_stages():
+    """Stages 1 and 2 through the stage-level entry points: the stage state
+    read back after each (stage 2 writes its own buffer on the device, the
+    oracle updates one buffer in place) matches the oracle's over the
+    region the stage writes."""
+    prob = problems.sedov3d(nb=1, n=16, nguard=10)
+    b = capi.Batch(prob.cfg, prob.blocks)
+    try:
+        b.pack(prob.arrays)
+        dt, _ = b.compute_dt()
+        b.step_begin()
+        b.stage(1, dt, 6)
+        S1 = clone(prob.arrays)
+        b.unpack(S1, which=1)
+        b.stage(2, dt, 3)
+        S2 = clone(prob.arrays)
+        b.unpack(S2, which=1)
+    finally:
+        b.close()
+    U = prob.arrays[0].copy()
+    pyoracle.shock_detect(prob.cfg, prob.blocks[0], U)
+    S_ref = U.copy()
+    idx = [capi.VI[v] for v in ("dens", "velx", "vely", "velz", "pres", "eint", "ener", "gamc")]
+    pyoracle.stage(prob.cfg, prob.blocks[0], U, S_ref, 1, 6, dt)
+    compare([interior_views(prob, S1, 6)[0][..., idx]], [interior_views(prob, [S_ref], 6)[0][..., idx]], "stage 1")
+    pyoracle.stage(prob.cfg, prob.blocks[0], U, S_ref, 2, 3, dt)
+    compare([interior_views(prob, S2, 3)[0][..., idx]], [interior_views(prob, [S_ref], 3)[0][..., idx]], "stage 2")
+
+
 def test_flux_correction_matches_oracle():
     """Flux registers after one step and hy_rk_correctFluxes with given
     correction data match the oracle."""
