@@ -285,43 +285,6 @@ __global__ void __launch_bounds__(256) k_flux_exchange(DevParams P, const BlockM
   }
 }
 
-// guard cells of blocks whose 26 guard sectors all face leaves of their own
-// level, to `depth` layers: plain copies from those leaves' interiors
-// through the neighbour table (what k_amr_fill computes for a same-level
-// source), with no leaf lookup.  Sources are interior cells and
-// destinations guard cells, so no cell is both.
-__global__ void __launch_bounds__(256) k_halo_copy(const BlockMeta* __restrict__ meta, const int* __restrict__ blist,
-                                                   const int64_t* __restrict__ nbr, double* __restrict__ U,
-                                                   VarList vars, int g, int depth) {
-  const int bid = blist[blockIdx.y];
-  const BlockMeta m = meta[bid];
-  const int ex = m.nx + 2 * depth, ey = m.ny + 2 * depth, ez = m.nz + 2 * depth;
-  const int total = ex * ey * ez;
-  const int64_t* row = nbr + (int64_t)bid * 27;
-  const int64_t sy = m.NX, sz = (int64_t)m.NX * m.NY;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-    const int i = g - depth + t % ex, j = g - depth + (t / ex) % ey, k = g - depth + t / (ex * ey);
-    const int ox = i < g ? -1 : (i >= g + m.nx ? 1 : 0);
-    const int oy = j < g ? -1 : (j >= g + m.ny ? 1 : 0);
-    const int oz = k < g ? -1 : (k >= g + m.nz ? 1 : 0);
-    if ((ox | oy | oz) == 0) continue;
-    const int64_t src = row[13 + ox + 3 * oy + 9 * oz] + (i - ox * m.nx) + sy * (j - oy * m.ny) + sz * (k - oz * m.nz);
-    const int64_t dst = m.base + i + sy * j + sz * k;
-    // the loads of a chunk of variables issue before its stores (a store to
-    // U could alias a later load, so the compiler keeps them in order); a
-    // slot past vars.n re-reads the last variable
-    constexpr int CHV = 8;
-    for (int q0 = 0; q0 < vars.n; q0 += CHV) {
-      double val[CHV];
-#pragma unroll
-      for (int u = 0; u < CHV; u++) val[u] = U[src + vars.v[q0 + u < vars.n ? q0 + u : vars.n - 1] * m.cstride];
-#pragma unroll
-      for (int u = 0; u < CHV; u++)
-        if (q0 + u < vars.n) U[dst + vars.v[q0 + u] * m.cstride] = val[u];
-    }
-  }
-}
-
 inline int grid_x(int total, int cap) {
   int g = (total + 255) / 256;
   return g < 1 ? 1 : (g > cap ? cap : g);
@@ -337,17 +300,6 @@ hipError_t amr_fill(const PlainLaunch& L, const AmrGrid& G, double* U, const Var
   if (L.prof) L.prof->begin(L.stream, "k_amr_fill", 16.0 * L.nblk * (double)(box - nin) * vars.n, 0.0);
   hipLaunchKernelGGL(k_amr_fill, dim3(grid_x(box, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta, L.blist, G, U,
                      vars, depth, smask, keep);
-  if (L.prof) L.prof->end(L.stream);
-  return hipGetLastError();
-}
-
-hipError_t amr_halo_copy(const PlainLaunch& L, const int64_t* nbr, double* U, const VarList& vars, int depth) {
-  const int ext = L.nx + 2 * depth;
-  const int box = ext * ext * ext;
-  const int nin = L.nx * L.nx * L.nx;
-  if (L.prof) L.prof->begin(L.stream, "k_halo_copy", 16.0 * L.nblk * (double)(box - nin) * vars.n, 0.0);
-  hipLaunchKernelGGL(k_halo_copy, dim3(grid_x(box, 512), L.nblk), dim3(256), 0, L.stream, L.meta, L.blist, nbr, U, vars,
-                     L.P.ng, depth);
   if (L.prof) L.prof->end(L.stream);
   return hipGetLastError();
 }

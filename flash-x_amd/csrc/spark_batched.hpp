@@ -25,6 +25,41 @@ struct FusedItem {
 // neighbour-table entry of a uniform grid's guard sector past the domain edge
 constexpr int64_t NBR_EDGE = INT64_MIN;
 
+__device__ __forceinline__ int64_t bcix(const BlockMeta& m, int i, int j, int k) {
+  return (int64_t)i + (int64_t)m.NX * ((int64_t)j + (int64_t)m.NY * (int64_t)k);
+}
+
+// pool offset of cell (i, j, k) of a block's guard-celled box.  With a
+// neighbour table a cell outside the interior is taken from the adjacent
+// block of its guard sector, or, past a uniform grid's domain edge
+// (NBR_EDGE), from the block's own boundary layer (outflow: the guard copies
+// the first interior layer, Grid_bcApplyToRegion.F90:526-532), one
+// dimension at a time as the guard fill clamps the global index.  A sector
+// the block computes itself holds its own base shifted by the sector's
+// offset, so the same arithmetic lands in its own guard cells.
+__device__ __forceinline__ int64_t cell_at(const BlockMeta& m, const int64_t* __restrict__ nbt, int g, int i, int j,
+                                           int k) {
+  if (nbt == nullptr) return m.base + bcix(m, i, j, k);
+  int ox = 0, oy = 0, oz = 0;
+  if (i < g) ox = -1; else if (i >= g + m.nx) ox = 1;
+  if (j < g) oy = -1; else if (j >= g + m.ny) oy = 1;
+  if (k < g) oz = -1; else if (k >= g + m.nz) oz = 1;
+  if ((ox | oy | oz) == 0) return m.base + bcix(m, i, j, k);
+  if (ox != 0) {
+    if (nbt[13 + ox] == NBR_EDGE) { i = ox < 0 ? g : g + m.nx - 1; ox = 0; }
+    else i -= ox * m.nx;
+  }
+  if (oy != 0) {
+    if (nbt[13 + 3 * oy] == NBR_EDGE) { j = oy < 0 ? g : g + m.ny - 1; oy = 0; }
+    else j -= oy * m.ny;
+  }
+  if (oz != 0) {
+    if (nbt[13 + 9 * oz] == NBR_EDGE) { k = oz < 0 ? g : g + m.nz - 1; oz = 0; }
+    else k -= oz * m.nz;
+  }
+  return nbt[13 + ox + 3 * oy + 9 * oz] + bcix(m, i, j, k);
+}
+
 struct FusedArgs {
   DevParams P;
   const BlockMeta* meta;

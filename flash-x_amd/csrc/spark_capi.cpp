@@ -259,9 +259,11 @@ int run_fill(spark_batch* b, double* pool, bool full, int depth) {
     for (int s = 0; s < b->p.nscalars; s++) vl.v[vl.n++] = spark::V_NPROP + s;
   }
   if (b->amr && !full && amr_remap_on(b)) {
-    // remapping blocks read no guard cell of their own in the stages: fill
-    // only what shockDetect of their interior reads (and of the layer
-    // around it, which the hybrid Riemann solver reads SHOK of)
+    // remapping blocks read no guard cell of their own in the stages.
+    // Those whose guard sectors all face same-level leaves get none: their
+    // shockDetect reads the leaves around them through the neighbour table.
+    // The others get what shockDetect of their interior reads (and of the
+    // layer around it, which the hybrid Riemann solver reads SHOK of)
     if (b->n_tele > 0) {
       // blocks that read some sectors from same-level leaves: those sectors
       // only to the layer shockDetect of the interior reads (the hybrid
@@ -269,17 +271,6 @@ int run_fill(spark_batch* b, double* pool, bool full, int depth) {
       L.blist = b->tele_ids_d;
       L.nblk = b->n_tele;
       HIPCHK(spark::amr_fill(L, b->amr_g, pool, vl, depth, b->p.use_hybrid_riemann ? nullptr : b->amr_smask_d, 1));
-    }
-    spark::VarList vs;  // shockDetect: dens, gamc, pres and the velocities one layer out
-    vs.n = 0;
-    for (int v : {spark::V_DENS, spark::V_GAMC, spark::V_PRES, spark::V_VELX, spark::V_VELY, spark::V_VELZ})
-      vs.v[vs.n++] = v;
-    if (b->amr_remap_count > 0) {
-      // every guard sector faces a same-level leaf: copies through the
-      // neighbour table
-      L.blist = b->remap_ids_d;
-      L.nblk = b->amr_remap_count;
-      HIPCHK(spark::amr_halo_copy(L, b->nbr_d, pool, vs, std::min(depth, b->p.use_hybrid_riemann ? 2 : 1)));
     }
   } else if (b->amr) {
     HIPCHK(spark::amr_fill(L, b->amr_g, pool, vl, depth));
@@ -365,7 +356,7 @@ int run_step_begin(spark_batch* b) {
     if (b->amr_remap_count > 0) {
       L.blist = b->remap_ids_d;
       L.nblk = b->amr_remap_count;
-      HIPCHK(spark::plain_shock_detect(L, b->U, b->flat, hyb ? 1 : 0));
+      HIPCHK(spark::plain_shock_detect(L, b->U, b->flat, hyb ? 1 : 0, b->nbr_d));
     }
   } else {
     for (auto& c : b->classes) {

@@ -60,7 +60,10 @@ struct PlainLaunch {
 };
 
 hipError_t plain_stage(const PlainLaunch& L, int ngoff, double c1, double c2, double c3, int final_stage);
-hipError_t plain_shock_detect(const PlainLaunch& L, double* U, double* scratch, int band);
+// nbr (3D, optional): per block 27 pool offsets of the same-level leaves
+// around it; the block's guard cells are then read from those leaves
+hipError_t plain_shock_detect(const PlainLaunch& L, double* U, double* scratch, int band,
+                              const int64_t* nbr = nullptr);
 hipError_t plain_dt_part(const PlainLaunch& L, const double* U, void* parts, int chunks);
 // the driver's limits on the step's dt (Driver_computeDt.F90:450-460 and
 // the first step's dtinit, Driver_evolveAll), kept on the device: `first` > 0
@@ -96,9 +99,6 @@ struct AmrGrid {
 enum { AMR_BC_OUTFLOW = 0, AMR_BC_PERIODIC = 1, AMR_BC_REFLECT = 2 };
 // smask (optional, per block): guard sectors filled to `depth`; the others
 // only to `keep` layers
-// guard cells of the listed blocks (cubic, all 26 neighbour-table entries
-// same-level leaves) copied from their neighbours' interiors, `depth` layers
-hipError_t amr_halo_copy(const PlainLaunch& L, const int64_t* nbr, double* U, const VarList& vars, int depth);
 hipError_t amr_fill(const PlainLaunch& L, const AmrGrid& G, double* U, const VarList& vars, int depth,
                     const uint32_t* smask = nullptr, int keep = 0);
 hipError_t amr_flux_exchange(const PlainLaunch& L, const AmrGrid& G, double* const fb[3], double* const fc[3]);

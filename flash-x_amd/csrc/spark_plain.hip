@@ -15,6 +15,7 @@
 #include <mutex>
 #include <tuple>
 
+#include "spark_batched.hpp"
 #include "spark_launch.hpp"
 #include "spark_physics.hpp"
 #include "spark_regions.hpp"
@@ -583,30 +584,41 @@ __global__ void __launch_bounds__(256) k_shock(DevParams P, const BlockMeta* __r
 }
 
 // shockDetect of 3D blocks in one pass: TW x TH column tiles of the
-// (band + 1) box marched over z, one thread per column, with the pressure
-// and sound speed of three z planes (tile + 1-cell ring) in LDS, so the
-// sound speed is computed once per cell and never stored.  Cells of the
-// band box get the flag, the outer layer 0, as the two-pass kernels above
-// give.  Tiles are as wide as the box up to SHK_T per side.
-constexpr int SHK_T = 32;
+// (band + 1) box marched over z, one thread per column, with three z planes
+// (tile + 1-cell ring) of pressure, sound speed and the three velocities in
+// LDS, so every value is loaded once and the sound speed computed once per
+// cell.  Cells of the band box get the flag, the outer layer 0, as the
+// two-pass kernels above give.  Tiles are as wide as the box up to SHK_T
+// per side.  NB: cells outside the interior are read through a neighbour
+// table (nbr: 27 pool offsets per block, every guard sector a same-level
+// leaf) from the leaves around the block, so its own guard cells need no
+// fill; the flags are still written into the block's own box.
+constexpr int SHK_T = 20;
+constexpr int SHK_F = 5;  // staged fields: pres, sound speed, velx, vely, velz
+template <bool NB>
 __global__ void __launch_bounds__(1024) k_shock3(DevParams P, const BlockMeta* __restrict__ meta,
                                                  const int* __restrict__ blist, double* __restrict__ U, int band,
-                                                 int TW, int TH) {
+                                                 int TW, int TH, const int64_t* __restrict__ nbr) {
   extern __shared__ double shk_lds[];
+  __shared__ int64_t s_nb[27];
   const int PW = TW + 2, PS = PW * (TH + 2);
-  double* sp = shk_lds;           // [3][PS]
-  double* sc = shk_lds + 3 * PS;  // [3][PS]
-  const BlockMeta m = meta[blist[blockIdx.y]];
+  double* sp = shk_lds;           // [3][PS] pressure
+  double* sc = shk_lds + 3 * PS;  // [3][PS] sound speed
+  double* sv = shk_lds + 6 * PS;  // [3 velocities][3][PS]
+  const int bid = blist[blockIdx.y];
+  const BlockMeta m = meta[bid];
   const Lims L1 = stage_lims(P, m, band + 1), L0 = stage_lims(P, m, band);
+  if (NB) {
+    if (threadIdx.x < 27) s_nb[threadIdx.x] = nbr[(int64_t)bid * 27 + threadIdx.x];
+    __syncthreads();
+  }
   const int ex = L1.hi[0] - L1.lo[0] + 1;
   const int ntx = (ex + TW - 1) / TW;
   const int x0 = L1.lo[0] + (int)(blockIdx.x % ntx) * TW, y0 = L1.lo[1] + (int)(blockIdx.x / ntx) * TH;
   const int tid = threadIdx.x;
   const int i = x0 + tid % TW, j = y0 + tid / TW;
   const bool own = tid < TW * TH && i <= L1.hi[0] && j <= L1.hi[1];
-  double* up = U + m.base;
   const int64_t cs = m.cstride;
-  const int64_t sx = 1, sy = m.NX, sz = (int64_t)m.NX * m.NY;
   const double beta = 0.5, delta = 0.1;
   // plane k into its slot: cells of the (band + 1) box only (the others
   // are never read by a flagged cell)
@@ -614,14 +626,20 @@ __global__ void __launch_bounds__(1024) k_shock3(DevParams P, const BlockMeta* _
     const int s = (k - L1.lo[2]) % 3;
     for (int u = tid; u < PS; u += blockDim.x) {
       const int ii = x0 - 1 + u % PW, jj = y0 - 1 + u / PW;
-      double pv = 0., cv = 0.;
+      double pv = 0., cv = 0., vx = 0., vy = 0., vz = 0.;
       if (ii >= L1.lo[0] && ii <= L1.hi[0] && jj >= L1.lo[1] && jj <= L1.hi[1] && k <= L1.hi[2]) {
-        const int64_t c = cix(m, ii, jj, k);
-        pv = up[V_PRES * cs + c];
-        cv = Num<double>::sqr(up[V_GAMC * cs + c] * pv / fmax2(up[V_DENS * cs + c], P.tiny));
+        const int64_t c = NB ? cell_at(m, s_nb, P.ng, ii, jj, k) : m.base + cix(m, ii, jj, k);
+        pv = U[V_PRES * cs + c];
+        cv = Num<double>::sqr(U[V_GAMC * cs + c] * pv / fmax2(U[V_DENS * cs + c], P.tiny));
+        vx = U[V_VELX * cs + c];
+        vy = U[V_VELY * cs + c];
+        vz = U[V_VELZ * cs + c];
       }
       sp[s * PS + u] = pv;
       sc[s * PS + u] = cv;
+      sv[(0 * 3 + s) * PS + u] = vx;
+      sv[(1 * 3 + s) * PS + u] = vy;
+      sv[(2 * 3 + s) * PS + u] = vz;
     }
   };
   load(L1.lo[2]);
@@ -629,7 +647,6 @@ __global__ void __launch_bounds__(1024) k_shock3(DevParams P, const BlockMeta* _
     load(k + 1);
     __syncthreads();
     if (own) {
-      const int64_t c = cix(m, i, j, k);
       double shok = 0.;
       if (i >= L0.lo[0] && i <= L0.hi[0] && j >= L0.lo[1] && j <= L0.hi[1] && k >= L0.lo[2] && k <= L0.hi[2]) {
         const int q = (j - y0 + 1) * PW + (i - x0 + 1);
@@ -644,19 +661,23 @@ __global__ void __launch_bounds__(1024) k_shock3(DevParams P, const BlockMeta* _
               else { minP = fmin2(minP, pv); minC = fmin2(minC, cv); }
             }
         }
-        const double* p = up + V_PRES * cs;
-        double divv = up[V_VELX * cs + c + sx] - up[V_VELX * cs + c - sx];
-        divv = divv + up[V_VELY * cs + c + sy] - up[V_VELY * cs + c - sy];
-        divv = divv + up[V_VELZ * cs + c + sz] - up[V_VELZ * cs + c - sz];
+        // slots of levels k-1, k, k+1 (k-1 >= L1.lo: a band cell is inside L1)
+        const int sm = (k - 1 - L1.lo[2]) % 3, s0 = (k - L1.lo[2]) % 3, sq = (k + 1 - L1.lo[2]) % 3;
+        const double* vxp = sv + (0 * 3 + s0) * PS + q;
+        const double* vyp = sv + (1 * 3 + s0) * PS + q;
+        const double* p0 = sp + s0 * PS + q;
+        double divv = vxp[1] - vxp[-1];
+        divv = divv + vyp[PW] - vyp[-PW];
+        divv = divv + sv[(2 * 3 + sq) * PS + q] - sv[(2 * 3 + sm) * PS + q];
         divv = 0.5 * divv;
-        const double gradPx = 0.5 * (p[c + sx] - p[c - sx]);
-        const double gradPy = 0.5 * (p[c + sy] - p[c - sy]);
-        const double gradPz = 0.5 * (p[c + sz] - p[c - sz]);
+        const double gradPx = 0.5 * (p0[1] - p0[-1]);
+        const double gradPy = 0.5 * (p0[PW] - p0[-PW]);
+        const double gradPz = 0.5 * (sp[sq * PS + q] - sp[sm * PS + q]);
         const bool SW1 = (__builtin_fabs(gradPx) + __builtin_fabs(gradPy) + __builtin_fabs(gradPz) >= beta * minP);
         const bool SW2 = (-delta * minC >= divv);
         if (SW1 && SW2) shok = 1.;
       }
-      up[V_SHOK * cs + c] = shok;
+      U[m.base + V_SHOK * cs + cix(m, i, j, k)] = shok;
     }
     __syncthreads();
   }
@@ -1132,7 +1153,7 @@ hipError_t plain_stage(const PlainLaunch& L, int ngoff, double c1, double c2, do
 
 // band: SHOK is computed over interior +/- band (the reference uses
 // NGUARD-1; only the cells the stage-1 face fluxes read are needed).
-hipError_t plain_shock_detect(const PlainLaunch& L, double* U, double* scratch, int band) {
+hipError_t plain_shock_detect(const PlainLaunch& L, double* U, double* scratch, int band, const int64_t* nbr) {
   const int n[3] = {L.nx, L.P.ndim > 1 ? L.ny : 1, L.P.ndim > 2 ? L.nz : 1};
   int b1 = 1, b0 = 1;
   for (int d = 0; d < L.P.ndim; d++) { b1 *= n[d] + 2 * band + 2; b0 *= n[d] + 2 * band; }
@@ -1142,13 +1163,18 @@ hipError_t plain_shock_detect(const PlainLaunch& L, double* U, double* scratch, 
     const int tw = e1 < SHK_T ? e1 : SHK_T, th = e2 < SHK_T ? e2 : SHK_T;
     const int tiles = ((e1 + tw - 1) / tw) * ((e2 + th - 1) / th);
     const int threads = (tw * th + 63) / 64 * 64;
-    const size_t lds = sizeof(double) * 6 * (tw + 2) * (th + 2);
+    const size_t lds = sizeof(double) * 3 * SHK_F * (tw + 2) * (th + 2);
     if (L.prof) L.prof->begin(L.stream, "k_shock", 8.0 * nc1 * 7, nc1 * 6 + nc0 * 70);
-    hipLaunchKernelGGL(k_shock3, dim3(tiles, L.nblk), dim3(threads), lds, L.stream, L.P, L.meta, L.blist, U, band,
-                       tw, th);
+    if (nbr)
+      hipLaunchKernelGGL(k_shock3<true>, dim3(tiles, L.nblk), dim3(threads), lds, L.stream, L.P, L.meta, L.blist, U,
+                         band, tw, th, nbr);
+    else
+      hipLaunchKernelGGL(k_shock3<false>, dim3(tiles, L.nblk), dim3(threads), lds, L.stream, L.P, L.meta, L.blist, U,
+                         band, tw, th, nbr);
     if (L.prof) L.prof->end(L.stream);
     return hipGetLastError();
   }
+  if (nbr) return hipErrorInvalidValue;  // the neighbour-table path is 3D only
   if (L.prof) L.prof->begin(L.stream, "k_shock_vc", 8.0 * nc1 * 5, nc1 * 6);
   hipLaunchKernelGGL(k_shock_vc, dim3(grid_x_for(b1, 512), L.nblk), dim3(256), 0, L.stream, L.P, L.meta,
                      L.blist, U, scratch, band);
