@@ -739,29 +739,42 @@ __global__ void __launch_bounds__(256) k_dt_part(DevParams P, const BlockMeta* _
   }
 }
 
-__global__ void __launch_bounds__(256) k_dt_final(DevParams P, const DtPart* __restrict__ parts, int nblocks,
-                                                  int chunks, double* __restrict__ dt_out,
-                                                  long long* __restrict__ loc_out, double* __restrict__ dt_hist,
-                                                  int* __restrict__ hist_slot, DtCtl* __restrict__ ctl) {
-  __shared__ double sdt[4];
-  __shared__ int sb[4];
-  __shared__ long long sc[4];
+// A block's chunks are all loaded before the first is compared (the
+// comparison picks the largest key, then the smallest index: a total order,
+// so the result does not depend on the order or the thread count).
+constexpr int DT_FINAL_THREADS = 1024;
+constexpr int DT_MAX_CHUNKS = 16;
+__global__ void __launch_bounds__(DT_FINAL_THREADS) k_dt_final(DevParams P, const DtPart* __restrict__ parts,
+                                                               int nblocks, int chunks, double* __restrict__ dt_out,
+                                                               long long* __restrict__ loc_out,
+                                                               double* __restrict__ dt_hist,
+                                                               int* __restrict__ hist_slot, DtCtl* __restrict__ ctl) {
+  __shared__ double sdt[DT_FINAL_THREADS / 64];
+  __shared__ int sb[DT_FINAL_THREADS / 64];
+  __shared__ long long sc[DT_FINAL_THREADS / 64];
   double best = 1.7976931348623157e308;
   int bb = 0x7fffffff;
   long long bc = 0;
   for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
     double bk = 0., bl = 0.;
     long long bi = 0x7fffffffffffffffLL;
-    for (int q = 0; q < chunks; q++) {
-      const DtPart r = parts[(size_t)b * chunks + q];
-      if (r.key > bk || (r.key == bk && r.idx < bi)) { bk = r.key; bl = r.lt; bi = r.idx; }
-    }
+    DtPart r[DT_MAX_CHUNKS];
+#pragma unroll
+    for (int q = 0; q < DT_MAX_CHUNKS; q++)
+      if (q < chunks) r[q] = parts[(size_t)b * chunks + q];
+#pragma unroll
+    for (int q = 0; q < DT_MAX_CHUNKS; q++)
+      if (q < chunks && (r[q].key > bk || (r[q].key == bk && r[q].idx < bi))) {
+        bk = r[q].key;
+        bl = r[q].lt;
+        bi = r[q].idx;
+      }
     const double dtb = (bl != 0.0) ? P.cfl / bl : 1.7976931348623157e308;
     if (dtb < best || (dtb == best && b < bb)) { best = dtb; bb = b; bc = bi; }
   }
   // smaller dt, then the lower block index (Driver_computeDt's strict '<'
   // over blocks in order): a total order, reduced per wavefront by a
-  // butterfly and then over the 4 wavefronts
+  // butterfly and then over the wavefronts
   for (int off = 32; off > 0; off >>= 1) {
     const double d2 = __shfl_xor(best, off, 64);
     const int b2 = __shfl_xor(bb, off, 64);
@@ -1199,8 +1212,9 @@ hipError_t plain_dt_part(const PlainLaunch& L, const double* U, void* parts, int
 
 hipError_t plain_dt_final(hipStream_t st, const DevParams& P, const void* parts, int nblk_total, int chunks,
                           double* dt_out, long long* loc_out, double* dt_hist, int* hist_slot, DtCtl* ctl) {
-  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, st, P, (const DtPart*)parts, nblk_total, chunks, dt_out,
-                     loc_out, dt_hist, hist_slot, ctl);
+  if (chunks < 1 || chunks > DT_MAX_CHUNKS) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(DT_FINAL_THREADS), 0, st, P, (const DtPart*)parts, nblk_total, chunks,
+                     dt_out, loc_out, dt_hist, hist_slot, ctl);
   return hipGetLastError();
 }
 
