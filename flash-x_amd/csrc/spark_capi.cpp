@@ -951,7 +951,11 @@ int spark_batch_create(const spark_params* p, int nblocks, const spark_block_des
     }
     b->classes[it->second].ids.push_back(i);
   }
-  b->dtchunks = (int)std::min<int64_t>(16, std::max<int64_t>(1, (max_int + 255) / 256));
+  // CFL partials: a block's cells split into chunks (one workgroup each)
+  // only until the batch has about 4096 workgroups; fewer partials keep the
+  // one-workgroup final reduction short
+  b->dtchunks = (int)std::min<int64_t>({16, std::max<int64_t>(1, (max_int + 255) / 256),
+                                        std::max<int64_t>(1, (4096 + b->nblk - 1) / b->nblk)});
 
 #define ALLOC(ptr, bytes)                                                              \
   do {                                                                                 \
