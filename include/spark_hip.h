@@ -158,7 +158,9 @@ int spark_set_uniform_grid(spark_batch* b, int nbx, int nby, int nbz);
  * and, with flux_correct, the device flux-register exchange after every
  * step.  In the batched forms a leaf whose 26 neighbours are leaves of its
  * own level reads its halo from them and updates only its interior in every
- * stage (the values telescoping would give); with flux_correct the
+ * stage (the values telescoping would give; such a leaf's own guard cells
+ * are not filled by the step: spark_fill_guardcells fills them on request);
+ * with flux_correct the
  * FP32-flux form corrects only the cells on faces toward a finer leaf (the
  * correction is zero elsewhere; the FP64 forms still pass those cells
  * through the reference's conserved/primitive round trip).
